@@ -1,0 +1,248 @@
+/*
+ * mpm.h -- C ABI of the MI355X explicit MLS-MPM simulator (libmpm_hip.so).
+ *
+ * The reference simulator (JerryYan97/MPM_CUDA) exposes a C++ class, not a C
+ * ABI.  Each entry point below names the reference member or free function it
+ * replaces.  Conventions that differ from the reference on purpose:
+ *   - every call returns an int status (MPM_OK = 0) and never exits the
+ *     process; mpm_last_error() describes the last failure of this thread
+ *     (the reference prints and calls exit(1), e.g. MPMCUDA.cu:800-803);
+ *   - one HIP stream per simulator; mpm_step is asynchronous, every read
+ *     synchronizes (the reference synchronizes after each kernel,
+ *     MPMCUDA.cu:830, 855, 898);
+ *   - plain pointers and sizes only; no C++ or torch types cross the ABI.
+ * A simulator handle must not be used from two threads at once.
+ */
+#ifndef MPM_MPM_H
+#define MPM_MPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+enum {
+    MPM_OK = 0,
+    MPM_ERR_INVALID = 1,    /* bad argument (range, null pointer, unknown enum) */
+    MPM_ERR_HIP = 2,        /* a HIP runtime call failed */
+    MPM_ERR_IO = 3,         /* file could not be read or written */
+    MPM_ERR_RANGE = 4,      /* object/particles outside the grid (MPMSimulator.cpp:210-222) */
+    MPM_ERR_STATE = 5,      /* health check failed (see mpm_health) */
+    MPM_ERR_NOMEM = 6
+};
+
+/* Material types, in the order of `enum MaterialType` (Elasiticity.cuh:10-12). */
+enum { MPM_JELLO = 0, MPM_SNOW = 1, MPM_WATER = 2 };
+
+/* Grid boundary conditions (reference: per-axis zeroing, MPMCUDA.cu:463-469). */
+enum { MPM_BC_AXIS = 0, MPM_BC_STICKY = 1 };
+
+/* Water affine matrix: APIC C (reference, MPMCUDA.cu:520-532) or C = grad v. */
+enum { MPM_WATER_C_APIC = 0, MPM_WATER_C_GRADV = 1 };
+
+/* Execution form of the step.
+ * MPM_FORM_PLAIN: unsorted particles, P2G scatters with global atomics, the
+ *   stress SVD runs in P2G and (snow) again in G2P -- the structure of
+ *   MPMCUDA.cu:310-741 in FP32 SoA.
+ * MPM_FORM_FAST: particles cell-block sorted every `sort_every` steps, P2G
+ *   accumulates per block in LDS and flushes once per node, G2P is fused with
+ *   the stress evaluation so each step needs one SVD. */
+enum { MPM_FORM_PLAIN = 0, MPM_FORM_FAST = 1 };
+
+/* Warm start for the polar decomposition in G2P. */
+enum { MPM_WARM_NONE = 0, MPM_WARM_MLP = 1 };
+
+/* Particle volume: reference quirk V_p = h^3 (MPMSimulator.cpp:263, occupancy
+ * counts particles), or h^3 / ppc.  Dynamics do not depend on the choice. */
+enum { MPM_VOLUME_REF = 0, MPM_VOLUME_PPC = 1 };
+
+typedef struct mpm_config {
+    double h;                 /* grid spacing, "gap" (MPMSimulator.cuh:92) */
+    int32_t nx, ny, nz;       /* nodes per axis (MPMSimulator.cuh:94) */
+    double origin[3];         /* grid origin; (0,0,0) in the reference (MPMSimulator.cpp:129) */
+    double dt;                /* step used by mpm_step when dt <= 0 ("max_dt", :184) */
+    double gravity[3];        /* (0,-9.8,0) (MPMSimulator.cpp:186) */
+    int32_t bc_pad;           /* wall thickness in cells: 10 (MPMCUDA.cu:845-850) */
+    int32_t bc_mode;          /* MPM_BC_* */
+    int32_t ppc;              /* samples per cell for mpm_add_object (MPMSimulator.cuh:95) */
+    int32_t volume_mode;      /* MPM_VOLUME_* */
+    uint64_t seed;            /* jitter seed for mpm_add_object */
+    int32_t form;             /* MPM_FORM_* */
+    int32_t sort_every;       /* FAST: re-sort particles every k steps */
+    int32_t svd_sweeps;       /* FP32 Jacobi sweeps (plain solve) */
+    int32_t warm_start;       /* MPM_WARM_* */
+    int32_t warm_sweeps;      /* Jacobi sweeps after a warm start */
+    int32_t tile;             /* FAST: particles per P2G tile, 0 = tuned default */
+    int32_t threads;          /* threads per workgroup for particle kernels, 0 = default */
+    int32_t reserved[5];
+} mpm_config;
+
+typedef struct mpm_material {
+    int32_t type;             /* MPM_JELLO / MPM_SNOW / MPM_WATER */
+    int32_t water_c_mode;     /* MPM_WATER_C_* */
+    double E, nu, rho;        /* Material(E, nu, rho, type) (Elasiticity.cuh:30) */
+    double theta_c, theta_s;  /* snow clamp [1-theta_c, 1+theta_s]: 8e-2, 2e-2 (MPMCUDA.cu:658-660) */
+    double xi;                /* snow hardening: 10 (MPMCUDA.cu:344) */
+    double eos_k, eos_gamma;  /* water: 2.51, 7 (MPMCUDA.cu:353) */
+} mpm_material;
+
+/* Shapes for mpm_add_object. */
+enum { MPM_SHAPE_MESH = 0, MPM_SHAPE_BOX = 1, MPM_SHAPE_SPHERE = 2 };
+
+/* One object to sample, after `ObjInitInfo` (MPMSimulator.cuh:19-27). */
+typedef struct mpm_object {
+    int32_t shape;            /* MPM_SHAPE_* */
+    int32_t reserved0;
+    const char *obj_path;     /* MESH: Wavefront OBJ (v/f records) */
+    double shape_param[6];    /* BOX: local min xyz, max xyz; SPHERE: local centre xyz, radius */
+    double init_vel[3];       /* initVel */
+    float translation[3];     /* initTranslation */
+    float scale[3];           /* initScale */
+    float rotation_axis[3];   /* initRotationAxis (normalised by the caller, main.cpp:146) */
+    float rotation_deg;       /* initRotationDegree */
+    mpm_material material;    /* mMaterial */
+    int64_t max_particles;    /* >0: keep a seeded random subset of this size */
+} mpm_object;
+
+/* Particle fields for mpm_read_particles / mpm_write_particles.
+ * Layout of the host buffer: n rows of the field's width, row-major 3x3. */
+enum {
+    MPM_P_POS = 0,      /* 3 floats  pPosVec */
+    MPM_P_VEL = 1,      /* 3         pVelVec */
+    MPM_P_C = 2,        /* 9         pAffineVel */
+    MPM_P_F = 3,        /* 9         pElasticityDeformationGradient */
+    MPM_P_JP = 4,       /* 1         snow: det Fp; water: J (pJVec) */
+    MPM_P_TAU = 5       /* 6 (read-only) Kirchhoff stress xx,yy,zz,xy,xz,yz used by the next P2G */
+};
+
+/* Grid fields for mpm_read_grid, reference node order z*NX*NY + y*NX + x
+ * (MPMCUDA.cu:375). */
+enum {
+    MPM_G_MASS = 0,     /* 1 float per node */
+    MPM_G_MOMENTUM = 1, /* 3: after MPM_PHASE_P2G */
+    MPM_G_VELOCITY = 2  /* 3: after MPM_PHASE_GRID */
+};
+
+enum { MPM_PHASE_P2G = 0, MPM_PHASE_GRID = 1, MPM_PHASE_G2P = 2 };
+
+/* Health bits, accumulated on the device since the last mpm_health call
+ * (the reference asserts these, MPMCUDA.cu:376-386, 431, 546, 614, 702, 722-739). */
+enum {
+    MPM_H_STENCIL_OOB = 1,  /* a stencil left the grid (clamped) */
+    MPM_H_WEIGHT_SUM = 2,
+    MPM_H_DET_NONPOS = 4,
+    MPM_H_LEFT_DOMAIN = 8,
+    MPM_H_NONFINITE = 16,
+    MPM_H_ARENA_SPILL = 32  /* FAST: particles drifted beyond their tile arena (handled) */
+};
+
+typedef struct mpm_sim mpm_sim;
+
+/* Fill `cfg` with the reference's defaults (MPMSimulator.cpp:184-186,
+ * MPMCUDA.cu:845-850) and this library's tuned defaults. */
+void mpm_config_default(mpm_config *cfg);
+
+/* Material(E, nu, rho, type), Elasiticity.cuh:30-45: range checks of
+ * :34-41 return MPM_ERR_INVALID instead of exiting. */
+int mpm_material_init(mpm_material *m, double E, double nu, double rho, int type);
+
+/* MPMSimulator::MPMSimulator grid part (MPMSimulator.cpp:178-197, initGrid :122-146). */
+int mpm_create(const mpm_config *cfg, mpm_sim **out);
+
+/* Per-object sampling of MPMSimulator.cpp:198-276 (mesh or procedural
+ * shape, per-cell jittered samples, inside test, V_p and m_p). Returns the
+ * new group's index through *group (may be NULL). */
+int mpm_add_object(mpm_sim *sim, const mpm_object *obj, int32_t *group);
+
+/* Add a group from explicit particles (positions xyz[3n]; vel[3n] or NULL
+ * to use v0; particle_volume <= 0 means h^3).  initParticles,
+ * MPMSimulator.cpp:10-120: F = I, Fp = I / J = 1, C = 0. */
+int mpm_add_particles(mpm_sim *sim, const mpm_material *mat, int64_t n, const float *xyz,
+                      const float *vel, const double v0[3], double particle_volume, int32_t *group);
+
+/* MPMSimulator::step (MPMCUDA.cu:793-925); dt <= 0 uses cfg.dt. Asynchronous. */
+int mpm_step(mpm_sim *sim, double dt);
+
+/* nsteps calls of mpm_step, launched back to back (graph-replayed in FAST form). */
+int mpm_run(mpm_sim *sim, int32_t nsteps, double dt);
+
+/* One phase of the step, for phase-by-phase parity checks. */
+int mpm_step_phase(mpm_sim *sim, int32_t phase, double dt);
+
+/* MPMSimulator::totalParticlesNum (MPMSimulator.cuh:99). */
+int64_t mpm_total_particles(const mpm_sim *sim);
+int32_t mpm_group_count(const mpm_sim *sim);
+int64_t mpm_group_size(const mpm_sim *sim, int32_t group);
+
+/* Particle state of one group, in the group's original particle order. */
+int mpm_read_particles(mpm_sim *sim, int32_t group, int32_t field, float *out);
+int mpm_write_particles(mpm_sim *sim, int32_t group, int32_t field, const float *in);
+
+/* Positions of all groups concatenated: MPMSimulator::getGLParticlesPos
+ * (MPMSimulator.cpp:362-371). out holds 3 * mpm_total_particles floats. */
+int mpm_get_positions(mpm_sim *sim, float *out);
+
+/* Dense grid field in reference node order; out holds width * nx*ny*nz floats. */
+int mpm_read_grid(mpm_sim *sim, int32_t field, float *out);
+
+/* Health bits since the last call (cleared on read), and counters. */
+int mpm_health(mpm_sim *sim, uint32_t *bits);
+/* stats[0] steps, [1] active grid blocks of the last step, [2] FP64 SVD
+ * fallbacks, [3] warm-start rejections, [4] sorts, [5] time (s). */
+int mpm_stats(mpm_sim *sim, double stats[8]);
+
+int mpm_synchronize(mpm_sim *sim);
+
+/* Per-phase device timing with HIP events recorded on the simulator's
+ * stream (no reference counterpart; the reference has only an FPS readout,
+ * main.cpp:252-253).  Phases: [0] sort + tile build, [1] P2G, [2] grid
+ * update (incl. active-block compaction), [3] G2P.  mpm_phase_times returns
+ * milliseconds and launch counts accumulated since the previous call and
+ * clears them.  stats_nodes (may be NULL) receives the number of grid nodes
+ * with mass in the last grid update. */
+int mpm_set_timing(mpm_sim *sim, int32_t enable);
+int mpm_phase_times(mpm_sim *sim, double ms[4], int64_t counts[4], int64_t *stats_nodes);
+
+/* Simulation clock: current_time / current_frame (MPMSimulator.cuh:86-87). */
+double mpm_current_time(const mpm_sim *sim);
+int64_t mpm_current_frame(const mpm_sim *sim);
+
+/* OutputBego (FilesIO.h:10): write positions as a Houdini BGEO particle file
+ * with one "position" attribute. */
+int mpm_write_bgeo(const char *path, const float *xyz, int64_t n);
+
+/* Load MLP warm-start weights (fp32, layout documented in mlp.h). */
+int mpm_set_warm_start_weights(mpm_sim *sim, const float *weights, int64_t count);
+
+/* MPMSimulator::~MPMSimulator (MPMSimulator.cpp:288-352). */
+void mpm_destroy(mpm_sim *sim);
+
+/* ---- host-only sampling and device probes (no reference counterpart; they
+ * expose single stages of the step for parity tests) ---- */
+
+/* The sampling of mpm_add_object (MPMSimulator.cpp:198-265) on the host
+ * only: with xyz == NULL stores the particle count in *n; otherwise writes
+ * min(*n, count) positions and stores the count. */
+int mpm_sample_object(const mpm_config *cfg, const mpm_object *obj, float *xyz, int64_t *n);
+
+/* Device FP32 base-node index max(0, floor((x - origin) * inv_h - 0.5)) of
+ * n coordinates, as used by P2G / G2P / the sort keys. */
+int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, int32_t *out);
+
+/* Device SVD of n row-major 3x3 FP32 matrices: `sweeps` Jacobi sweeps plus
+ * the FP64 fallback of the step kernels. U, V row-major, s[3] per matrix. */
+int mpm_probe_svd(const float *F, int64_t n, int32_t sweeps, float *U, float *s, float *V);
+
+/* Device stable LSD radix sort of n keys on their low `bits` bits; writes
+ * the sorted keys and the permutation (sorted[i] = keys[perm[i]]). */
+int mpm_probe_sort(const uint32_t *keys, int64_t n, int32_t bits, uint32_t *sorted, uint32_t *perm);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mpm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

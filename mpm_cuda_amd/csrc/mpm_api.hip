@@ -1,0 +1,630 @@
+// mpm_api.hip -- C ABI of libmpm_hip (declared in include/mpm/mpm.h).
+//
+// Host orchestration of the step: MPMSimulator (reference
+// src/simulator/MPMSimulator.cpp, MPMCUDA.cu:793-925) restated over one HIP
+// stream per simulator, with status codes instead of exit(1).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mpm_launch.h"
+
+namespace mpm {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return set_error(MPM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static int check_launch(int rc, const char *what)
+{
+    if (rc != 0) return hip_fail((hipError_t)rc, what);
+    return MPM_OK;
+}
+
+static int width_of(int field)
+{
+    switch (field) {
+    case MPM_P_POS: case MPM_P_VEL: return 3;
+    case MPM_P_C: case MPM_P_F: return 9;
+    case MPM_P_JP: return 1;
+    case MPM_P_TAU: return 6;
+    default: return -1;
+    }
+}
+
+static int plane_of(int field)
+{
+    switch (field) {
+    case MPM_P_POS: return PL_X;
+    case MPM_P_VEL: return PL_V;
+    case MPM_P_C: return PL_C;
+    case MPM_P_F: return PL_F;
+    case MPM_P_JP: return PL_S;
+    case MPM_P_TAU: return PL_TAU;
+    default: return -1;
+    }
+}
+
+PartView view_of(mpm_sim *s, int g)
+{
+    PartView pv;
+    pv.P = s->P;
+    pv.pid = s->pid;
+    pv.cap = s->cap;
+    pv.off = s->groups[g].off;
+    pv.n = (int)s->groups[g].n;
+    return pv;
+}
+
+// Grow particle storage to hold `need` particles (planes keep their data).
+static int ensure_capacity(mpm_sim *s, int64_t need)
+{
+    if (need <= s->cap) return MPM_OK;
+    int64_t ncap = std::max<int64_t>(need, s->cap + s->cap / 2);
+    ncap = (ncap + 255) / 256 * 256;
+    float *P = nullptr, *Pa = nullptr;
+    uint32_t *pid = nullptr, *pida = nullptr;
+    MPM_HIP(hipMalloc(&P, (size_t)ncap * PL_COUNT * sizeof(float)));
+    MPM_HIP(hipMalloc(&Pa, (size_t)ncap * PL_COUNT * sizeof(float)));
+    MPM_HIP(hipMalloc(&pid, (size_t)ncap * sizeof(uint32_t)));
+    MPM_HIP(hipMalloc(&pida, (size_t)ncap * sizeof(uint32_t)));
+    MPM_HIP(hipMemsetAsync(P, 0, (size_t)ncap * PL_COUNT * sizeof(float), s->stream));
+    if (s->P && s->total > 0) {
+        for (int p = 0; p < PL_COUNT; ++p)
+            MPM_HIP(hipMemcpyAsync(P + (size_t)p * ncap, s->P + (size_t)p * s->cap,
+                                   (size_t)s->total * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+        MPM_HIP(hipMemcpyAsync(pid, s->pid, (size_t)s->total * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               s->stream));
+    }
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    if (s->P) { (void)hipFree(s->P); (void)hipFree(s->P_alt); (void)hipFree(s->pid); (void)hipFree(s->pid_alt); }
+    s->P = P; s->P_alt = Pa; s->pid = pid; s->pid_alt = pida;
+    s->cap = ncap;
+    return MPM_OK;
+}
+
+static MatP make_matp(const mpm_material &m, double mass, double vol)
+{
+    MatP p;
+    p.type = m.type;
+    p.water_c_mode = m.water_c_mode;
+    p.mass = (float)mass;
+    p.vol = (float)vol;
+    p.mu = (float)(m.E / (2.0 * (1.0 + m.nu)));                          // Elasiticity.cuh:42
+    p.lam = (float)(m.E * m.nu / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu)));   // Elasiticity.cuh:43
+    p.theta_c = (float)m.theta_c;
+    p.theta_s = (float)m.theta_s;
+    p.xi = (float)m.xi;
+    p.eos_k = (float)m.eos_k;
+    p.eos_gamma = (float)m.eos_gamma;
+    return p;
+}
+
+static int validate_material(const mpm_material *m)
+{
+    if (!m) return set_error(MPM_ERR_INVALID, "material is null");
+    if (m->type < MPM_JELLO || m->type > MPM_WATER)
+        return set_error(MPM_ERR_INVALID, "unknown material type");
+    if (!(m->E >= 0)) return set_error(MPM_ERR_INVALID, "Youngs Modulus is out of range.");   // Elasiticity.cuh:34-37
+    if (!(m->nu >= 0 && m->nu < 0.5)) return set_error(MPM_ERR_INVALID, "Poison Ratio is out of range."); // :38-41
+    if (!(m->rho > 0)) return set_error(MPM_ERR_INVALID, "density must be positive");
+    return MPM_OK;
+}
+
+static int run_phase(mpm_sim *s, int phase, float dt);
+
+}  // namespace mpm
+
+using namespace mpm;
+
+extern "C" {
+
+const char *mpm_last_error(void) { return g_err.c_str(); }
+
+void mpm_config_default(mpm_config *c)
+{
+    if (!c) return;
+    memset(c, 0, sizeof *c);
+    c->h = 0.04;                 // main.cpp:161
+    c->nx = c->ny = c->nz = 64;
+    c->dt = 1.0 / 5000.0;        // main.cpp:161
+    c->gravity[1] = -9.8;        // MPMSimulator.cpp:186
+    c->bc_pad = 10;              // MPMCUDA.cu:845-850
+    c->bc_mode = MPM_BC_AXIS;
+    c->ppc = 10;                 // main.cpp:161
+    c->volume_mode = MPM_VOLUME_REF;
+    c->seed = 20261202ull;
+    c->form = MPM_FORM_FAST;
+    c->sort_every = 8;
+    c->svd_sweeps = 4;
+    c->warm_start = MPM_WARM_NONE;
+    c->warm_sweeps = 1;
+    c->tile = 0;
+    c->threads = 256;
+}
+
+int mpm_material_init(mpm_material *m, double E, double nu, double rho, int type)
+{
+    if (!m) return set_error(MPM_ERR_INVALID, "material is null");
+    memset(m, 0, sizeof *m);
+    m->type = type;
+    m->water_c_mode = MPM_WATER_C_APIC;
+    m->E = E; m->nu = nu; m->rho = rho;
+    m->theta_c = 8e-2; m->theta_s = 2e-2;   // MPMCUDA.cu:658-660
+    m->xi = 10.0;                           // MPMCUDA.cu:344
+    m->eos_k = 2.51; m->eos_gamma = 7.0;    // MPMCUDA.cu:353
+    return validate_material(m);
+}
+
+int mpm_create(const mpm_config *cfg, mpm_sim **out)
+{
+    if (!cfg || !out) return set_error(MPM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (!(cfg->h > 0)) return set_error(MPM_ERR_INVALID, "grid spacing must be positive");
+    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4) return set_error(MPM_ERR_INVALID, "grid needs >= 4 nodes per axis");
+    if ((int64_t)cfg->nx * cfg->ny * cfg->nz > (1ll << 31))
+        return set_error(MPM_ERR_INVALID, "grid too large");
+    if (cfg->form != MPM_FORM_PLAIN && cfg->form != MPM_FORM_FAST) return set_error(MPM_ERR_INVALID, "unknown form");
+    if (cfg->bc_mode != MPM_BC_AXIS && cfg->bc_mode != MPM_BC_STICKY) return set_error(MPM_ERR_INVALID, "unknown bc_mode");
+    if (cfg->svd_sweeps < 1 || cfg->svd_sweeps > 16) return set_error(MPM_ERR_INVALID, "svd_sweeps out of range [1,16]");
+    int threads = cfg->threads ? cfg->threads : 256;
+    if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+        return set_error(MPM_ERR_INVALID, "threads must be 64, 128, 256 or 512");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(MPM_ERR_HIP, "no HIP device");
+    mpm_sim *s = new mpm_sim();
+    s->cfg = *cfg;
+    s->cfg.threads = threads;
+    if (s->cfg.sort_every < 1) s->cfg.sort_every = 1;
+    GridP &g = s->g;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
+    g.nbx = (g.nx + BLK - 1) / BLK; g.nby = (g.ny + BLK - 1) / BLK; g.nbz = (g.nz + BLK - 1) / BLK;
+    g.h = (float)cfg->h;
+    g.inv_h = (float)(1.0 / cfg->h);
+    g.ox = (float)cfg->origin[0]; g.oy = (float)cfg->origin[1]; g.oz = (float)cfg->origin[2];
+    g.gx = (float)cfg->gravity[0]; g.gy = (float)cfg->gravity[1]; g.gz = (float)cfg->gravity[2];
+    g.pad = cfg->bc_pad; g.bc_mode = cfg->bc_mode;
+    s->nblocks = (int64_t)g.nbx * g.nby * g.nbz;
+    int rc = MPM_OK;
+    auto fail = [&](hipError_t e, const char *w) { rc = hip_fail(e, w); };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) fail(e, "hipStreamCreate");
+    size_t gbytes = (size_t)s->nblocks * BLK_NODES * sizeof(float4);
+    if (!rc && (e = hipMalloc(&s->grid_mp, gbytes)) != hipSuccess) fail(e, "hipMalloc grid_mp");
+    if (!rc && (e = hipMalloc(&s->grid_v, gbytes)) != hipSuccess) fail(e, "hipMalloc grid_v");
+    if (!rc && (e = hipMalloc(&s->flags, (size_t)s->nblocks * 4)) != hipSuccess) fail(e, "hipMalloc flags");
+    if (!rc && (e = hipMalloc(&s->list, (size_t)s->nblocks * 4)) != hipSuccess) fail(e, "hipMalloc list");
+    if (!rc && (e = hipMalloc(&s->counters, CT_COUNT * 4)) != hipSuccess) fail(e, "hipMalloc counters");
+    if (!rc && (e = hipMemsetAsync(s->grid_mp, 0, gbytes, s->stream)) != hipSuccess) fail(e, "memset");
+    if (!rc && (e = hipMemsetAsync(s->grid_v, 0, gbytes, s->stream)) != hipSuccess) fail(e, "memset");
+    if (!rc && (e = hipMemsetAsync(s->flags, 0, (size_t)s->nblocks * 4, s->stream)) != hipSuccess) fail(e, "memset");
+    if (!rc && (e = hipMemsetAsync(s->counters, 0, CT_COUNT * 4, s->stream)) != hipSuccess) fail(e, "memset");
+    if (!rc && (e = hipStreamSynchronize(s->stream)) != hipSuccess) fail(e, "sync");
+    if (rc) { mpm_destroy(s); return rc; }
+    *out = s;
+    return MPM_OK;
+}
+
+int mpm_add_particles(mpm_sim *s, const mpm_material *mat, int64_t n, const float *xyz, const float *vel,
+                      const double v0[3], double particle_volume, int32_t *group)
+{
+    if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
+    int rc = validate_material(mat);
+    if (rc) return rc;
+    if (n <= 0 || !xyz) return set_error(MPM_ERR_INVALID, "no particles");
+    if (s->total + n > (1ll << 31) - 1) return set_error(MPM_ERR_INVALID, "too many particles");
+    // every particle's stencil must lie inside the grid (MPMSimulator.cpp:210-222)
+    const GridP &g = s->g;
+    const float o[3] = {g.ox, g.oy, g.oz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            float x = xyz[3 * i + a];
+            float hi = o[a] + g.h * (float)(nn[a] - 1);
+            if (!(x >= o[a] && x <= hi))
+                return set_error(MPM_ERR_RANGE, "particle outside the grid [origin, origin+(N-1)h]");
+        }
+    rc = ensure_capacity(s, s->total + n);
+    if (rc) return rc;
+    GroupInfo gi;
+    gi.src = *mat;
+    gi.off = s->total;
+    gi.n = n;
+    double h3 = s->cfg.h * s->cfg.h * s->cfg.h;
+    gi.vol = particle_volume > 0 ? particle_volume : h3;   // V_p = occ h^3 / N_p = h^3 (MPMSimulator.cpp:263)
+    gi.mass = mat->rho * gi.vol;                            // MPMSimulator.cpp:264
+    gi.mat = make_matp(*mat, gi.mass, gi.vol);
+    // host staging in plane order
+    std::vector<float> host((size_t)n * PL_COUNT, 0.0f);
+    auto plane = [&](int p) { return host.data() + (size_t)p * n; };
+    for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            plane(PL_X + a)[i] = xyz[3 * i + a];
+            plane(PL_V + a)[i] = vel ? vel[3 * i + a] : (float)(v0 ? v0[a] : 0.0);
+        }
+        plane(PL_F + 0)[i] = 1.0f; plane(PL_F + 4)[i] = 1.0f; plane(PL_F + 8)[i] = 1.0f;   // F = I (:20-26)
+        plane(PL_S)[i] = 1.0f;                                                             // det Fp = 1, J = 1
+    }
+    for (int p = 0; p < PL_COUNT; ++p)
+        MPM_HIP(hipMemcpyAsync(s->P + (size_t)p * s->cap + gi.off, plane(p), (size_t)n * sizeof(float),
+                               hipMemcpyHostToDevice, s->stream));
+    std::vector<uint32_t> ids((size_t)n);
+    for (int64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+    MPM_HIP(hipMemcpyAsync(s->pid + gi.off, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    s->groups.push_back(gi);
+    s->total += n;
+    s->tau_valid = false;
+    s->sorted_valid = false;
+    if (group) *group = (int32_t)(s->groups.size() - 1);
+    return MPM_OK;
+}
+
+int mpm_add_object(mpm_sim *s, const mpm_object *obj, int32_t *group)
+{
+    if (!s || !obj) return set_error(MPM_ERR_INVALID, "null argument");
+    std::vector<float> xyz;
+    int rc = sample_object(s->cfg, *obj, xyz);
+    if (rc) return rc;
+    int64_t n = (int64_t)xyz.size() / 3;
+    if (n == 0) return set_error(MPM_ERR_RANGE, "object produced no particles");
+    double h3 = s->cfg.h * s->cfg.h * s->cfg.h;
+    double vol = s->cfg.volume_mode == MPM_VOLUME_PPC ? h3 / std::max(1, s->cfg.ppc) : h3;
+    return mpm_add_particles(s, &obj->material, n, xyz.data(), nullptr, obj->init_vel, vol, group);
+}
+
+int mpm_step_phase(mpm_sim *s, int32_t phase, double dt)
+{
+    if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
+    if (phase < MPM_PHASE_P2G || phase > MPM_PHASE_G2P) return set_error(MPM_ERR_INVALID, "unknown phase");
+    float fdt = (float)(dt > 0 ? dt : s->cfg.dt);
+    return run_phase(s, phase, fdt);
+}
+
+int mpm_step(mpm_sim *s, double dt)
+{
+    if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
+    double d = dt > 0 ? dt : s->cfg.dt;
+    float fdt = (float)d;
+    for (int ph = MPM_PHASE_P2G; ph <= MPM_PHASE_G2P; ++ph) {
+        int rc = run_phase(s, ph, fdt);
+        if (rc) return rc;
+    }
+    s->steps += 1;          // ++current_frame (MPMCUDA.cu:923)
+    s->time += d;           // current_time += adp_dt (MPMCUDA.cu:924)
+    return MPM_OK;
+}
+
+int mpm_run(mpm_sim *s, int32_t nsteps, double dt)
+{
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = mpm_step(s, dt);
+        if (rc) return rc;
+    }
+    return MPM_OK;
+}
+
+int64_t mpm_total_particles(const mpm_sim *s) { return s ? s->total : 0; }
+int32_t mpm_group_count(const mpm_sim *s) { return s ? (int32_t)s->groups.size() : 0; }
+int64_t mpm_group_size(const mpm_sim *s, int32_t g)
+{
+    if (!s || g < 0 || g >= (int32_t)s->groups.size()) return -1;
+    return s->groups[g].n;
+}
+
+int mpm_synchronize(mpm_sim *s)
+{
+    if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    return MPM_OK;
+}
+
+int mpm_read_particles(mpm_sim *s, int32_t group, int32_t field, float *out)
+{
+    if (!s || !out) return set_error(MPM_ERR_INVALID, "null argument");
+    if (group < 0 || group >= (int32_t)s->groups.size()) return set_error(MPM_ERR_INVALID, "bad group");
+    int w = width_of(field), p0 = plane_of(field);
+    if (w < 0) return set_error(MPM_ERR_INVALID, "unknown particle field");
+    const GroupInfo &gi = s->groups[group];
+    if (field == MPM_P_TAU && !s->tau_valid) {
+        // stress of the current state, as the next P2G will use it
+        for (size_t k = 0; k < s->groups.size(); ++k) {
+            int rc = check_launch(launch_compute_tau(s->stream, s->groups[k].mat, view_of(s, (int)k),
+                                                     s->cfg.svd_sweeps, s->counters), "compute_tau");
+            if (rc) return rc;
+        }
+        s->tau_valid = true;
+    }
+    int64_t n = gi.n;
+    std::vector<float> buf((size_t)n * w);
+    std::vector<uint32_t> ids((size_t)n);
+    for (int c = 0; c < w; ++c)
+        MPM_HIP(hipMemcpyAsync(buf.data() + (size_t)c * n, s->P + (size_t)(p0 + c) * s->cap + gi.off,
+                               (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipMemcpyAsync(ids.data(), s->pid + gi.off, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    for (int64_t j = 0; j < n; ++j) {
+        uint32_t i = ids[j];
+        if (i >= (uint32_t)n) return set_error(MPM_ERR_STATE, "particle id table corrupted");
+        for (int c = 0; c < w; ++c) out[(size_t)i * w + c] = buf[(size_t)c * n + j];
+    }
+    return MPM_OK;
+}
+
+int mpm_write_particles(mpm_sim *s, int32_t group, int32_t field, const float *in)
+{
+    if (!s || !in) return set_error(MPM_ERR_INVALID, "null argument");
+    if (group < 0 || group >= (int32_t)s->groups.size()) return set_error(MPM_ERR_INVALID, "bad group");
+    if (field == MPM_P_TAU) return set_error(MPM_ERR_INVALID, "TAU is derived state and read-only");
+    int w = width_of(field), p0 = plane_of(field);
+    if (w < 0) return set_error(MPM_ERR_INVALID, "unknown particle field");
+    const GroupInfo &gi = s->groups[group];
+    int64_t n = gi.n;
+    if (field == MPM_P_POS) {
+        const GridP &g = s->g;
+        const float o[3] = {g.ox, g.oy, g.oz};
+        const int nn[3] = {g.nx, g.ny, g.nz};
+        for (int64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                float x = in[3 * i + a];
+                if (!(x >= o[a] && x <= o[a] + g.h * (float)(nn[a] - 1)))
+                    return set_error(MPM_ERR_RANGE, "particle outside the grid");
+            }
+    }
+    std::vector<uint32_t> ids((size_t)n);
+    MPM_HIP(hipMemcpyAsync(ids.data(), s->pid + gi.off, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    std::vector<float> buf((size_t)n * w);
+    for (int64_t j = 0; j < n; ++j) {
+        uint32_t i = ids[j];
+        if (i >= (uint32_t)n) return set_error(MPM_ERR_STATE, "particle id table corrupted");
+        for (int c = 0; c < w; ++c) buf[(size_t)c * n + j] = in[(size_t)i * w + c];
+    }
+    for (int c = 0; c < w; ++c)
+        MPM_HIP(hipMemcpyAsync(s->P + (size_t)(p0 + c) * s->cap + gi.off, buf.data() + (size_t)c * n,
+                               (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    if (field == MPM_P_F || field == MPM_P_JP) s->tau_valid = false;
+    if (field == MPM_P_POS) s->sorted_valid = false;
+    return MPM_OK;
+}
+
+int mpm_get_positions(mpm_sim *s, float *out)
+{
+    if (!s || !out) return set_error(MPM_ERR_INVALID, "null argument");
+    int64_t off = 0;
+    for (size_t g = 0; g < s->groups.size(); ++g) {
+        int rc = mpm_read_particles(s, (int32_t)g, MPM_P_POS, out + 3 * off);
+        if (rc) return rc;
+        off += s->groups[g].n;
+    }
+    return MPM_OK;
+}
+
+int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
+{
+    if (!s || !out) return set_error(MPM_ERR_INVALID, "null argument");
+    if (field < MPM_G_MASS || field > MPM_G_VELOCITY) return set_error(MPM_ERR_INVALID, "unknown grid field");
+    const GridP &g = s->g;
+    size_t nslots = (size_t)s->nblocks * BLK_NODES;
+    std::vector<float4> mp(nslots), gv(nslots);
+    std::vector<uint32_t> cnt(CT_COUNT);
+    MPM_HIP(hipMemcpyAsync(mp.data(), s->grid_mp, nslots * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipMemcpyAsync(gv.data(), s->grid_v, nslots * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipMemcpyAsync(cnt.data(), s->counters, CT_COUNT * 4, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    uint32_t nact = cnt[CT_ACTIVE];
+    std::vector<uint32_t> lst(nact);
+    if (nact) {
+        MPM_HIP(hipMemcpy(lst.data(), s->list, (size_t)nact * 4, hipMemcpyDeviceToHost));
+    }
+    std::vector<char> active((size_t)s->nblocks, 0);
+    for (uint32_t k = 0; k < nact; ++k) if (lst[k] < (uint32_t)s->nblocks) active[lst[k]] = 1;
+    // grid_mp holds (m, p) between P2G and the grid update; grid_v after it
+    const bool after_p2g = s->last_phase == MPM_PHASE_P2G;
+    int w = field == MPM_G_MASS ? 1 : 3;
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                int64_t slot = node_slot(g, x, y, z);
+                int64_t blk = slot / BLK_NODES;
+                size_t o = ((size_t)z * g.ny + y) * g.nx + x;
+                float4 a = mp[slot];
+                float4 b = active[blk] ? gv[slot] : make_float4(0, 0, 0, 0);
+                if (field == MPM_G_MASS) out[o] = after_p2g ? a.x : b.w;
+                else if (field == MPM_G_MOMENTUM) { out[3 * o] = a.y; out[3 * o + 1] = a.z; out[3 * o + 2] = a.w; }
+                else { out[3 * o] = b.x; out[3 * o + 1] = b.y; out[3 * o + 2] = b.z; }
+            }
+    (void)w;
+    return MPM_OK;
+}
+
+int mpm_health(mpm_sim *s, uint32_t *bits)
+{
+    if (!s || !bits) return set_error(MPM_ERR_INVALID, "null argument");
+    uint32_t v = 0;
+    MPM_HIP(hipMemcpyAsync(&v, s->counters + CT_HEALTH, 4, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    int rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_HEALTH, 1), "zero");
+    if (rc) return rc;
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    *bits = v;
+    return MPM_OK;
+}
+
+int mpm_stats(mpm_sim *s, double st[8])
+{
+    if (!s || !st) return set_error(MPM_ERR_INVALID, "null argument");
+    uint32_t c[CT_COUNT];
+    MPM_HIP(hipMemcpyAsync(c, s->counters, sizeof c, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    st[0] = (double)s->steps;
+    st[1] = c[CT_ACTIVE];
+    st[2] = c[CT_FALLBACK];
+    st[3] = c[CT_REJECT];
+    st[4] = (double)s->sorts;
+    st[5] = s->time;
+    st[6] = c[CT_TILES];
+    st[7] = 0;
+    return MPM_OK;
+}
+
+int mpm_set_timing(mpm_sim *s, int32_t enable)
+{
+    if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
+    s->timing = enable != 0;
+    return MPM_OK;
+}
+
+int mpm_phase_times(mpm_sim *s, double ms[4], int64_t counts[4], int64_t *stats_nodes)
+{
+    if (!s || !ms || !counts) return set_error(MPM_ERR_INVALID, "null argument");
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < 4; ++k) { ms[k] = 0.0; counts[k] = 0; }
+    for (auto &t : s->timed) {
+        float e = 0.f;
+        MPM_HIP(hipEventElapsedTime(&e, t.a, t.b));
+        if (t.phase >= 0 && t.phase < 4) { ms[t.phase] += e; counts[t.phase] += 1; }
+        s->event_pool.push_back(t.a);
+        s->event_pool.push_back(t.b);
+    }
+    s->timed.clear();
+    if (stats_nodes) {
+        uint32_t c[CT_COUNT];
+        MPM_HIP(hipMemcpy(c, s->counters, sizeof c, hipMemcpyDeviceToHost));
+        *stats_nodes = c[CT_NODES];
+    }
+    return MPM_OK;
+}
+
+double mpm_current_time(const mpm_sim *s) { return s ? s->time : 0.0; }
+int64_t mpm_current_frame(const mpm_sim *s) { return s ? s->steps : 0; }
+
+int mpm_write_bgeo(const char *path, const float *xyz, int64_t n)
+{
+    if (!path || (n > 0 && !xyz) || n < 0) return set_error(MPM_ERR_INVALID, "bad argument");
+    return write_bgeo(path, xyz, n);
+}
+
+int mpm_set_warm_start_weights(mpm_sim *s, const float *w, int64_t count)
+{
+    if (!s || !w || count <= 0) return set_error(MPM_ERR_INVALID, "bad argument");
+    if (s->mlp_w) { (void)hipFree(s->mlp_w); s->mlp_w = nullptr; }
+    MPM_HIP(hipMalloc(&s->mlp_w, (size_t)count * 4));
+    MPM_HIP(hipMemcpy(s->mlp_w, w, (size_t)count * 4, hipMemcpyHostToDevice));
+    s->mlp_count = count;
+    return MPM_OK;
+}
+
+void mpm_destroy(mpm_sim *s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
+                    s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
+                    s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"
+
+namespace mpm {
+
+// Event pair around a phase when timing is on.
+static hipEvent_t take_event(mpm_sim *s)
+{
+    hipEvent_t e = nullptr;
+    if (!s->event_pool.empty()) {
+        e = s->event_pool.back();
+        s->event_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+struct PhaseTimer {
+    mpm_sim *s;
+    int id;
+    hipEvent_t a = nullptr;
+    PhaseTimer(mpm_sim *s_, int id_) : s(s_), id(id_)
+    {
+        if (s->timing && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
+    }
+    ~PhaseTimer()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(s);
+        if (b && hipEventRecord(b, s->stream) == hipSuccess) s->timed.push_back({id, a, b});
+        else { s->event_pool.push_back(a); if (b) s->event_pool.push_back(b); }
+    }
+};
+
+// One phase of MPMSimulator::step (MPMCUDA.cu:793-925).
+static int run_phase(mpm_sim *s, int phase, float dt)
+{
+    const bool fast = s->cfg.form == MPM_FORM_FAST;
+    const int thr = s->cfg.threads;
+    int rc = MPM_OK;
+    s->last_phase = phase;
+    if (phase == MPM_PHASE_P2G) {
+        rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 2), "zero");
+        if (rc) return rc;
+        if (fast) {
+            {
+                PhaseTimer t(s, 0);
+                rc = fast_prepare(s);
+            }
+            if (rc) return rc;
+            PhaseTimer t(s, 1);
+            for (size_t k = 0; k < s->groups.size() && !rc; ++k) rc = launch_p2g_tiles(s, (int)k, dt);
+        } else {
+            PhaseTimer t(s, 1);
+            for (size_t k = 0; k < s->groups.size() && !rc; ++k)
+                rc = check_launch(launch_p2g_atomic(s->stream, s->g, s->groups[k].mat, view_of(s, (int)k), dt,
+                                                    s->cfg.svd_sweeps, 0, s->grid_mp, s->flags, s->counters, thr),
+                                  "p2g");
+        }
+        return rc;
+    }
+    if (phase == MPM_PHASE_GRID) {
+        PhaseTimer t(s, 2);
+        rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
+        if (rc) return rc;
+        return check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,
+                                               (int)s->nblocks),
+                            "grid_update");
+    }
+    // G2P
+    PhaseTimer t(s, 3);
+    for (size_t k = 0; k < s->groups.size() && !rc; ++k) {
+        if (fast) rc = launch_g2p_fast(s, (int)k, dt);
+        else
+            rc = check_launch(launch_g2p(s->stream, s->g, s->groups[k].mat, view_of(s, (int)k), dt, s->grid_v, 0,
+                                         s->cfg.svd_sweeps, s->counters, thr),
+                              "g2p");
+    }
+    if (!rc) {
+        s->tau_valid = fast;
+        if (fast) s->steps_since_sort += 1;
+    }
+    return rc;
+}
+
+}  // namespace mpm

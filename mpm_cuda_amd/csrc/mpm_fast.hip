@@ -1,0 +1,535 @@
+// mpm_fast.hip -- FAST execution form: cell-block sort and LDS-blocked P2G.
+//
+// No counterpart in the reference (JerryYan97/MPM_CUDA scatters unsorted
+// particles with global FP64 atomics, MPMCUDA.cu:388-421).  Pieces:
+//   * exclusive scan (three-kernel, wave-shuffle based);
+//   * stable LSD radix sort of 32-bit keys, 8-bit digits, ballot-matched
+//     in-wave ranking;
+//   * per-group sort of particles by the 4x4x4 block of their base node,
+//     payload permutation of every SoA plane, and a tile list (block,
+//     first particle, count) built from per-block counts;
+//   * P2G over tiles: each workgroup scatters its tile's particles into an
+//     8^3-node LDS arena with LDS atomics and flushes each touched node once
+//     with global atomics.
+#include "mpm_kernels.h"
+#include "mpm_launch.h"
+
+namespace mpm {
+
+PartView view_of(mpm_sim *s, int g);
+
+constexpr int ARENA = 8;                       // arena nodes per axis: block (4) + 1 below + 3 above
+constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
+
+// ---------------------------------------------------------------------------
+// Scan helpers.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread across the workgroup (<= 1024 threads).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total, uint32_t *lds)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t w = (int)threadIdx.x < nw ? lds[threadIdx.x] : 0u;
+        uint32_t wi = wave_incl_scan(w);
+        if ((int)threadIdx.x < nw) lds[16 + threadIdx.x] = wi - w;
+        if ((int)threadIdx.x == nw - 1) lds[32] = wi;
+    }
+    __syncthreads();
+    uint32_t res = lds[16 + wid] + inc - v;
+    total = lds[32];
+    __syncthreads();
+    return res;
+}
+
+constexpr int SCAN_T = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_T * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const uint32_t *in, int n, uint32_t *partial)
+{
+    __shared__ uint32_t lds[40];
+    int base = blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        int i = base + k * SCAN_T + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan(s, tot, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_partials(uint32_t *partial, int np)
+{
+    __shared__ uint32_t lds[40];
+    uint32_t carry = 0;
+    for (int c = 0; c < np; c += 1024) {
+        int i = c + threadIdx.x;
+        uint32_t v = i < np ? partial[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan(v, tot, lds);
+        if (i < np) partial[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint32_t *out, int n,
+                                                       const uint32_t *partial, uint32_t *total_out)
+{
+    __shared__ uint32_t lds[40];
+    int i0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (i0 + k < n) ? in[i0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = partial[blockIdx.x] + block_excl_scan(s, tot, lds);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (i0 + k < n) {
+            out[i0 + k] = run;
+            if (total_out && i0 + k == n - 1) *total_out = run + v[k];
+        }
+        run += v[k];
+    }
+}
+
+// out = exclusive scan of in (in-place allowed); partial holds ceil(n / 2048) words.
+static int scan_excl(hipStream_t st, const uint32_t *in, uint32_t *out, int n, uint32_t *partial,
+                     uint32_t *total_out)
+{
+    int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nt == 0) return 0;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(SCAN_T), 0, st, in, n, partial);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partial, nt);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SCAN_T), 0, st, in, out, n, partial, total_out);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits.  A tile of RS_TILE keys is handled by
+// 4 waves, each owning a contiguous run of 1024 keys processed 64 at a time;
+// equal digits inside a wave are matched with 8 ballots, so ranks follow
+// input order and the sort is stable.
+constexpr int RS_T = 256, RS_WAVE_KEYS = 1024, RS_ITERS = RS_WAVE_KEYS / 64, RS_TILE = 4 * RS_WAVE_KEYS;
+
+__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *hist,
+                                                  int ntiles)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_TILE / RS_T; ++k) {
+        int i = base + k * RS_T + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                     uint32_t *vout, int n, int shift, const uint32_t *hist,
+                                                     int ntiles)
+{
+    __shared__ uint32_t cnt[4][256];
+    for (int k = 0; k < 4; ++k) cnt[k][threadIdx.x] = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    uint32_t key[RS_ITERS], val[RS_ITERS], rank[RS_ITERS];
+#pragma unroll
+    for (int it = 0; it < RS_ITERS; ++it) {
+        int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        bool valid = i < n;
+        uint32_t k = valid ? kin[i] : 0u;
+        uint32_t d = (k >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        uint32_t below = (uint32_t)__popcll(m & lt);
+        uint32_t prev = 0;
+        if (valid) prev = cnt[w][d];
+        if (valid && below == 0) cnt[w][d] = prev + (uint32_t)__popcll(m);
+        key[it] = k;
+        val[it] = valid ? vin[i] : 0u;
+        rank[it] = prev + below;
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0;
+        for (int k = 0; k < 4; ++k) {
+            uint32_t t = cnt[k][threadIdx.x];
+            cnt[k][threadIdx.x] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RS_ITERS; ++it) {
+        int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        if (i < n) {
+            uint32_t d = (key[it] >> shift) & 255u;
+            uint32_t dst = hist[(int64_t)d * ntiles + blockIdx.x] + cnt[w][d] + rank[it];
+            if (dst < (uint32_t)n) {      // always true for a consistent histogram
+                kout[dst] = key[it];
+                vout[dst] = val[it];
+            }
+        }
+    }
+}
+
+// Sorts (keys, vals) of length n by the low `bits` key bits.  Returns through
+// `in_alt` whether the result ended in the *_alt buffers.
+static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                      int n, int bits, uint32_t *hist, uint32_t *partial, bool &in_alt)
+{
+    in_alt = false;
+    if (n <= 1) return 0;
+    int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, hist, ntiles);
+        int rc = scan_excl(st, hist, hist, 256 * ntiles, partial, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_T), 0, st, ka, va, kb, vb, n, shift, hist, ntiles);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+        in_alt = !in_alt;
+    }
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Keys, permutation and tiles.
+__global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *vals, uint32_t *blk_count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pv.n) return;
+    int b[3];
+    const float o[3] = {g.ox, g.oy, g.oz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int v = base_index_f32(ld(pv, PL_X + a, i), o[a], g.inv_h);
+        b[a] = v > nn[a] - 3 ? nn[a] - 3 : v;
+    }
+    uint32_t key = ((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
+                   (uint32_t)(b[0] >> 2);
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+    atomicAdd(&blk_count[key], 1u);
+}
+
+__global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, int64_t cap, int64_t off, int n,
+                          const uint32_t *__restrict__ perm, const uint32_t *__restrict__ pid,
+                          uint32_t *__restrict__ pida, uint32_t plane_mask)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t src = perm[j];
+    if (src >= (uint32_t)n) src = (uint32_t)j;    // never taken for a valid permutation
+    for (int p = 0; p < PL_COUNT; ++p)
+        if ((plane_mask >> p) & 1u) Pa[(int64_t)p * cap + off + j] = P[(int64_t)p * cap + off + src];
+    pida[off + j] = pid[off + src];
+}
+
+__global__ void k_tiles_per_block(const uint32_t *cnt, uint32_t *ntile, int nblocks, int tile)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblocks) ntile[b] = (cnt[b] + (uint32_t)tile - 1u) / (uint32_t)tile;
+}
+
+__global__ void k_emit_tiles(const uint32_t *cnt, const uint32_t *blk_start, const uint32_t *tile_start,
+                             int nblocks, int tile, uint4 *desc, uint32_t desc_cap)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    uint32_t c = cnt[b];
+    uint32_t t0 = tile_start[b], s0 = blk_start[b];
+    for (uint32_t k = 0; k * (uint32_t)tile < c && t0 + k < desc_cap; ++k) {
+        uint32_t m = c - k * (uint32_t)tile;
+        desc[t0 + k] = make_uint4(s0 + k * (uint32_t)tile, m < (uint32_t)tile ? m : (uint32_t)tile, (uint32_t)b, 0u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2G over tiles with an LDS arena (persistent over the tile list).
+template <int MAT>
+__global__ void __launch_bounds__(256) k_p2g_tile(GridP g, MatP mat, PartView pv, const uint4 *__restrict__ desc,
+                                                  const uint32_t *__restrict__ tile_count, uint32_t desc_cap, float dt, float4 *grid,
+                                                  uint32_t *flags, uint32_t *counters)
+{
+    __shared__ float arena[4 * ARENA_NODES];
+    const uint32_t nt = min(*tile_count, desc_cap);
+    uint32_t bits = 0;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const uint4 d = desc[t];
+        for (int k = threadIdx.x; k < 4 * ARENA_NODES; k += blockDim.x) arena[k] = 0.f;
+        __syncthreads();
+        const int bx = (int)(d.z % (uint32_t)g.nbx);
+        const int by = (int)((d.z / (uint32_t)g.nbx) % (uint32_t)g.nby);
+        const int bz = (int)(d.z / ((uint32_t)g.nbx * (uint32_t)g.nby));
+        const int ax = bx * BLK - 1, ay = by * BLK - 1, az = bz * BLK - 1;
+        for (uint32_t p = threadIdx.x; p < d.y; p += blockDim.x) {
+            const int i = (int)(d.x + p);
+            float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+            Stencil s;
+            bits |= make_stencil(g, x, s);
+            Scatter sc;
+            load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+            const int lx = s.b[0] - ax, ly = s.b[1] - ay, lz = s.b[2] - az;
+            const bool inside = lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
+                                lz <= ARENA - 3;
+            if (inside) {
+#pragma unroll
+                for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+                    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+                        for (int ox = 0; ox < 3; ++ox) {
+                            float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                            int a = ((lz + oz) * ARENA + (ly + oy)) * ARENA + (lx + ox);
+                            atomicAdd(&arena[a], c.x);
+                            atomicAdd(&arena[ARENA_NODES + a], c.y);
+                            atomicAdd(&arena[2 * ARENA_NODES + a], c.z);
+                            atomicAdd(&arena[3 * ARENA_NODES + a], c.w);
+                        }
+            } else {
+                bits |= MPM_H_ARENA_SPILL;
+#pragma unroll
+                for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+                    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+                        for (int ox = 0; ox < 3; ++ox) {
+                            float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                            float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
+                            unsafeAtomicAdd(dst + 0, c.x);
+                            unsafeAtomicAdd(dst + 1, c.y);
+                            unsafeAtomicAdd(dst + 2, c.z);
+                            unsafeAtomicAdd(dst + 3, c.w);
+                        }
+                int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2, by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
+                int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
+                for (int zz = bz0; zz <= bz1; ++zz)
+                    for (int yy = by0; yy <= by1; ++yy)
+                        for (int xx = bx0; xx <= bx1; ++xx) flags[((int64_t)zz * g.nby + yy) * g.nbx + xx] = 1u;
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < ARENA_NODES; k += blockDim.x) {
+            float m = arena[k];
+            if (m != 0.f) {
+                int nx_ = ax + (k & 7), ny_ = ay + ((k >> 3) & 7), nz_ = az + (k >> 6);
+                if (nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz) {
+                    float *dst = (float *)(grid + node_slot(g, nx_, ny_, nz_));
+                    unsafeAtomicAdd(dst + 0, m);
+                    unsafeAtomicAdd(dst + 1, arena[ARENA_NODES + k]);
+                    unsafeAtomicAdd(dst + 2, arena[2 * ARENA_NODES + k]);
+                    unsafeAtomicAdd(dst + 3, arena[3 * ARENA_NODES + k]);
+                    flags[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (bits) atomicOr(counters + CT_HEALTH, bits);
+    if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+static inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+static int tile_size(const mpm_sim *s) { return s->cfg.tile > 0 ? s->cfg.tile : 512; }
+
+static int64_t tile_bound(const mpm_sim *s, int64_t n)
+{
+    return std::min<int64_t>(s->nblocks, n) + n / tile_size(s) + 1;
+}
+
+static int ensure_fast_buffers(mpm_sim *s)
+{
+    int64_t nmax = 0;
+    for (auto &gi : s->groups) nmax = std::max(nmax, gi.n);
+    if (!s->keys || s->sort_hist_len < nmax) {
+        void *ptrs[] = {s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan};
+        for (void *p : ptrs) if (p) (void)hipFree(p);
+        int64_t ntiles = (nmax + RS_TILE - 1) / RS_TILE;
+        int64_t hist_len = 256 * ntiles;
+        int64_t scan_words = std::max<int64_t>((hist_len + SCAN_TILE - 1) / SCAN_TILE,
+                                               (s->nblocks + SCAN_TILE - 1) / SCAN_TILE) + 16;
+        MPM_HIP(hipMalloc(&s->keys, nmax * 4));
+        MPM_HIP(hipMalloc(&s->keys_alt, nmax * 4));
+        MPM_HIP(hipMalloc(&s->vals, nmax * 4));
+        MPM_HIP(hipMalloc(&s->vals_alt, nmax * 4));
+        MPM_HIP(hipMalloc(&s->sort_hist, hist_len * 4));
+        MPM_HIP(hipMalloc(&s->sort_scan, scan_words * 4));
+        s->sort_hist_len = nmax;
+    }
+    if (!s->blk_count) {
+        MPM_HIP(hipMalloc(&s->blk_count, s->nblocks * 4));
+        MPM_HIP(hipMalloc(&s->blk_start, s->nblocks * 4));
+        MPM_HIP(hipMalloc(&s->scan_tmp, s->nblocks * 4));
+    }
+    int64_t need = 0;
+    s->group_tile_off.assign(s->groups.size(), 0);
+    for (size_t k = 0; k < s->groups.size(); ++k) {
+        s->group_tile_off[k] = need;
+        need += tile_bound(s, s->groups[k].n);
+    }
+    if (s->tile_cap < need) {
+        if (s->tile_desc) (void)hipFree(s->tile_desc);
+        MPM_HIP(hipMalloc(&s->tile_desc, need * sizeof(uint4)));
+        s->tile_cap = need;
+    }
+    if (!s->tile_counts) MPM_HIP(hipMalloc(&s->tile_counts, 256 * 4));
+    if (s->groups.size() > 256) return set_error(MPM_ERR_INVALID, "at most 256 particle groups");
+    return MPM_OK;
+}
+
+static int sort_all(mpm_sim *s)
+{
+    int rc = ensure_fast_buffers(s);
+    if (rc) return rc;
+    hipStream_t st = s->stream;
+    int bits = 1;
+    while ((1ll << bits) < s->nblocks) ++bits;
+    uint32_t plane_mask = (1u << PL_COUNT) - 1u;
+    if (!s->tau_valid) plane_mask &= ~(((1u << 6) - 1u) << PL_TAU);
+    const int T = tile_size(s);
+    for (size_t k = 0; k < s->groups.size(); ++k) {
+        PartView pv = view_of(s, (int)k);
+        int n = pv.n;
+        if ((rc = launch_zero_u32(st, s->blk_count, (int)s->nblocks))) return hip_fail((hipError_t)rc, "zero");
+        hipLaunchKernelGGL(k_block_keys, dim3(nblk(n, 256)), dim3(256), 0, st, s->g, pv, s->keys, s->vals,
+                           s->blk_count);
+        bool in_alt = false;
+        if ((rc = radix_sort(st, s->keys, s->vals, s->keys_alt, s->vals_alt, n, bits, s->sort_hist, s->sort_scan,
+                             in_alt)))
+            return hip_fail((hipError_t)rc, "radix_sort");
+        const uint32_t *perm = in_alt ? s->vals_alt : s->vals;
+        hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, s->P, s->P_alt, s->cap, pv.off, n, perm,
+                           s->pid, s->pid_alt, plane_mask);
+        // tiles: particle starts per block and tile starts per block
+        int nb = (int)s->nblocks;
+        if ((rc = scan_excl(st, s->blk_count, s->blk_start, nb, s->sort_scan, nullptr)))
+            return hip_fail((hipError_t)rc, "scan");
+        hipLaunchKernelGGL(k_tiles_per_block, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_count, s->scan_tmp, nb,
+                           T);
+        if ((rc = scan_excl(st, s->scan_tmp, s->scan_tmp, nb, s->sort_scan, s->tile_counts + k)))
+            return hip_fail((hipError_t)rc, "scan");
+        hipLaunchKernelGGL(k_emit_tiles, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_count, s->blk_start,
+                           s->scan_tmp, nb, T, (uint4 *)s->tile_desc + s->group_tile_off[k],
+                           (uint32_t)tile_bound(s, n));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "sort kernels");
+    }
+    std::swap(s->P, s->P_alt);
+    std::swap(s->pid, s->pid_alt);
+    s->sorted_valid = true;
+    s->steps_since_sort = 0;
+    s->sorts += 1;
+    return MPM_OK;
+}
+
+int fast_prepare(mpm_sim *s)
+{
+    int rc = MPM_OK;
+    if (!s->sorted_valid || s->steps_since_sort >= s->cfg.sort_every) {
+        rc = sort_all(s);
+        if (rc) return rc;
+    }
+    if (!s->tau_valid) {
+        for (size_t k = 0; k < s->groups.size(); ++k) {
+            rc = launch_compute_tau(s->stream, s->groups[k].mat, view_of(s, (int)k), s->cfg.svd_sweeps, s->counters);
+            if (rc) return hip_fail((hipError_t)rc, "compute_tau");
+        }
+        s->tau_valid = true;
+    }
+    return MPM_OK;
+}
+
+int launch_p2g_tiles(mpm_sim *s, int group, float dt)
+{
+    PartView pv = view_of(s, group);
+    if (pv.n <= 0) return MPM_OK;
+    const MatP &m = s->groups[group].mat;
+    int64_t bound = tile_bound(s, pv.n);
+    int wg = (int)std::min<int64_t>(bound, 2048);
+    const uint4 *desc = (const uint4 *)s->tile_desc + s->group_tile_off[group];
+    const uint32_t *tc = s->tile_counts + group;
+    const uint32_t cap = (uint32_t)bound;
+    dim3 gr(wg), bl(256);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_tile<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, dt, s->grid_mp, s->flags, s->counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_tile<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, dt, s->grid_mp, s->flags, s->counters); break;
+    default: hipLaunchKernelGGL(k_p2g_tile<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, dt, s->grid_mp, s->flags, s->counters); break;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MPM_OK : hip_fail(e, "p2g_tile");
+}
+
+// Stand-alone sort for the probe entry point (own temporaries).
+int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, uint32_t *perm_h)
+{
+    if (n <= 0) return MPM_OK;
+    if (n > (1ll << 30)) return set_error(MPM_ERR_INVALID, "too many keys");
+    if (bits < 1 || bits > 32) return set_error(MPM_ERR_INVALID, "bits out of range [1,32]");
+    int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    int64_t hist_len = 256 * ntiles;
+    int64_t scan_words = (hist_len + SCAN_TILE - 1) / SCAN_TILE + 16;
+    uint32_t *k = nullptr, *ka = nullptr, *v = nullptr, *va = nullptr, *hist = nullptr, *scan = nullptr;
+    hipStream_t st = nullptr;
+    int rc = MPM_OK;
+    auto chk = [&](hipError_t e, const char *w) { if (!rc && e != hipSuccess) rc = hip_fail(e, w); };
+    chk(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    chk(hipMalloc(&k, n * 4), "malloc"); chk(hipMalloc(&ka, n * 4), "malloc");
+    chk(hipMalloc(&v, n * 4), "malloc"); chk(hipMalloc(&va, n * 4), "malloc");
+    chk(hipMalloc(&hist, hist_len * 4), "malloc"); chk(hipMalloc(&scan, scan_words * 4), "malloc");
+    std::vector<uint32_t> iota((size_t)n);
+    for (int64_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
+    chk(hipMemcpyAsync(k, keys_h, n * 4, hipMemcpyHostToDevice, st), "h2d");
+    chk(hipMemcpyAsync(v, iota.data(), n * 4, hipMemcpyHostToDevice, st), "h2d");
+    bool in_alt = false;
+    if (!rc) {
+        int e = radix_sort(st, k, v, ka, va, (int)n, bits, hist, scan, in_alt);
+        if (e) rc = hip_fail((hipError_t)e, "radix_sort");
+    }
+    chk(hipMemcpyAsync(sorted_h, in_alt ? ka : k, n * 4, hipMemcpyDeviceToHost, st), "d2h");
+    chk(hipMemcpyAsync(perm_h, in_alt ? va : v, n * 4, hipMemcpyDeviceToHost, st), "d2h");
+    chk(hipStreamSynchronize(st), "sync");
+    void *ptrs[] = {k, ka, v, va, hist, scan};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int launch_g2p_fast(mpm_sim *s, int group, float dt)
+{
+    int rc = launch_g2p(s->stream, s->g, s->groups[group].mat, view_of(s, group), dt, s->grid_v, 1,
+                        s->cfg.svd_sweeps, s->counters, s->cfg.threads);
+    return rc ? hip_fail((hipError_t)rc, "g2p_fast") : MPM_OK;
+}
+
+}  // namespace mpm

@@ -1,0 +1,135 @@
+// mpm_internal.h -- types shared by the HIP translation units of libmpm_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mpm/mpm.h"
+#include "mpm_math.h"
+
+namespace mpm {
+
+// Particle storage: one FP32 plane per scalar component (structure of
+// arrays).  Plane p of particle i lives at P[p * cap + i].  Groups occupy
+// contiguous index ranges [off, off + n) in every plane.
+enum Plane : int {
+    PL_X = 0,     // 3 planes: position
+    PL_V = 3,     // 3: velocity
+    PL_C = 6,     // 9: affine velocity C, row-major
+    PL_F = 15,    // 9: (elastic) deformation gradient, row-major
+    PL_S = 24,    // 1: snow J_p = det F_p, water J
+    PL_TAU = 25,  // 6: Kirchhoff stress (xx,yy,zz,xy,xz,yz) for the next P2G
+    PL_COUNT = 31
+};
+
+// Grid nodes are stored block-major: 4x4x4-node blocks, block index
+// (bz * nby + by) * nbx + bx, node slot ((z&3)*4 + (y&3))*4 + (x&3); one
+// float4 per node.  grid_mp holds (mass, momentum) during P2G; grid_v holds
+// (velocity, mass) after the grid update.
+constexpr int BLK = 4;
+constexpr int BLK_NODES = 64;
+
+struct GridP {
+    int nx, ny, nz;
+    int nbx, nby, nbz;
+    float h, inv_h;
+    float ox, oy, oz;
+    float gx, gy, gz;
+    int pad, bc_mode;
+};
+
+struct MatP {
+    int type, water_c_mode;
+    float mass, vol, mu, lam;
+    float theta_c, theta_s, xi;
+    float eos_k, eos_gamma;
+};
+
+// Per-launch view of one group's particles.
+struct PartView {
+    float *P;
+    uint32_t *pid;
+    int64_t cap;
+    int64_t off;
+    int n;
+};
+
+// Device counters (one uint32 array).
+enum Counter : int {
+    CT_ACTIVE = 0,      // active grid blocks of the current step   } both reset at
+    CT_NODES = 1,       // grid nodes with mass in the grid update   } the start of P2G
+    CT_HEALTH = 2,      // MPM_H_* bits
+    CT_FALLBACK = 3,    // FP64 SVD fallbacks
+    CT_REJECT = 4,      // warm-start rejections
+    CT_TILES = 5,       // P2G tiles (FAST form)
+    CT_COUNT = 16
+};
+
+__host__ __device__ inline int64_t node_slot(const GridP &g, int x, int y, int z)
+{
+    int64_t b = ((int64_t)(z >> 2) * g.nby + (y >> 2)) * g.nbx + (x >> 2);
+    return b * BLK_NODES + (((z & 3) * BLK + (y & 3)) * BLK + (x & 3));
+}
+
+struct GroupInfo {
+    mpm_material src;
+    MatP mat;
+    int64_t off = 0, n = 0;
+    double mass = 0, vol = 0;
+};
+
+}  // namespace mpm
+
+struct mpm_sim {
+    mpm_config cfg;
+    mpm::GridP g;
+    hipStream_t stream = nullptr;
+    int64_t nblocks = 0;
+    float4 *grid_mp = nullptr, *grid_v = nullptr;
+    uint32_t *flags = nullptr, *list = nullptr, *counters = nullptr;
+    // particles
+    float *P = nullptr, *P_alt = nullptr;
+    uint32_t *pid = nullptr, *pid_alt = nullptr;
+    int64_t cap = 0, total = 0;
+    std::vector<mpm::GroupInfo> groups;
+    bool tau_valid = false;
+    // FAST form: sort and tile buffers
+    uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
+    uint32_t *sort_hist = nullptr, *sort_scan = nullptr;
+    int64_t sort_hist_len = 0;
+    uint32_t *blk_count = nullptr, *blk_start = nullptr, *tile_desc = nullptr, *scan_tmp = nullptr;
+    int64_t tile_cap = 0;
+    std::vector<int64_t> group_tile_off;    // per group: first tile slot
+    uint32_t *tile_counts = nullptr;        // per group: tiles built by the last sort
+    bool sorted_valid = false;
+    int64_t steps_since_sort = 0;
+    int last_phase = -1;
+    // warm-start MLP weights (device)
+    float *mlp_w = nullptr;
+    int64_t mlp_count = 0;
+    // phase timing (HIP events on `stream`)
+    bool timing = false;
+    struct Timed { int phase; hipEvent_t a, b; };
+    std::vector<Timed> timed;
+    std::vector<hipEvent_t> event_pool;
+    // host bookkeeping
+    int64_t steps = 0;
+    int64_t sorts = 0;
+    double time = 0;
+    uint32_t health_host = 0;
+};
+
+namespace mpm {
+int set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+}  // namespace mpm
+
+#define MPM_HIP(call)                                                     \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return mpm::hip_fail(e_, #call);            \
+    } while (0)

@@ -1,0 +1,148 @@
+// mpm_kernels.h -- per-particle device routines shared by the P2G / G2P
+// kernels of both execution forms.
+#pragma once
+
+#include "mpm_internal.h"
+
+namespace mpm {
+
+constexpr float kFallbackRatio = 1e-4f;   // sigma_min / sigma_max below this -> FP64 SVD
+
+__device__ __forceinline__ float ld(const PartView &pv, int plane, int i)
+{
+    return pv.P[(int64_t)plane * pv.cap + pv.off + i];
+}
+__device__ __forceinline__ void st(const PartView &pv, int plane, int i, float v)
+{
+    pv.P[(int64_t)plane * pv.cap + pv.off + i] = v;
+}
+
+// Stencil of one particle: base node (clamped so the 3x3x3 support stays in
+// the grid) and per-axis B-spline weights / derivatives.
+struct Stencil {
+    int b[3];
+    float fx[3];
+    float w[3][3], dw[3][3];
+};
+
+__device__ __forceinline__ uint32_t make_stencil(const GridP &g, const float x[3], Stencil &s)
+{
+    uint32_t bits = 0;
+    const float o[3] = {g.ox, g.oy, g.oz};
+    const int n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int b = base_index_f32(x[a], o[a], g.inv_h);
+        if (b > n[a] - 3) { b = n[a] - 3; bits |= MPM_H_STENCIL_OOB; }
+        s.b[a] = b;
+        // relative coordinate, one rounding: (x - o) * inv_h - b
+        s.fx[a] = fmaf(x[a] - o[a], g.inv_h, -(float)b);
+        bspline_axis<float>(s.fx[a], s.w[a], s.dw[a]);
+    }
+    return bits;
+}
+
+// FP32 SVD with the FP64 fallback for ill-conditioned / inverted F.
+__device__ __forceinline__ void svd_checked(const float F[9], float U[9], float s[3], float V[9],
+                                            int sweeps, uint32_t *counters)
+{
+    svd3<float>(F, U, s, V, sweeps);
+    if (!(s[2] > kFallbackRatio * s[0])) {
+        double Fd[9], Ud[9], sd[3], Vd[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fd[k] = F[k];
+        svd3<double>(Fd, Ud, sd, Vd, 8);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { U[k] = (float)Ud[k]; V[k] = (float)Vd[k]; }
+        s[0] = (float)sd[0]; s[1] = (float)sd[1]; s[2] = (float)sd[2];
+        atomicAdd(counters + CT_FALLBACK, 1u);
+    }
+}
+
+// Kirchhoff stress of an elastic / snow particle from its deformation
+// gradient (reference P2G, MPMCUDA.cu:329-349).
+template <int MAT>
+__device__ __forceinline__ void stress_from_F(const MatP &m, const float F[9], float Jp, int sweeps,
+                                              float tau6[6], uint32_t *counters)
+{
+    float U[9], s[3], V[9];
+    svd_checked(F, U, s, V, sweeps, counters);
+    float mu = m.mu, lam = m.lam;
+    if (MAT == MPM_SNOW) {
+        float hc = expf(m.xi * (1.0f - Jp));           // MPMCUDA.cu:345
+        mu *= hc; lam *= hc;
+    }
+    fcr_tau_from_svd<float>(U, s, mu, lam, tau6);
+}
+
+// The per-particle quantities P2G scatters: node momentum
+//   p_i += w (a + B (x_i - x_p)) + G grad w_ip ,   node mass m_i += w m_p,
+// with a = m_p v_p, B = m_p C_p (MPMCUDA.cu:391-401) and G the dt-scaled
+// force matrix: -dt V_p tau (MPMCUDA.cu:409-411) or dt V_p p J I for water
+// (MPMCUDA.cu:412-414).
+struct Scatter {
+    float m;
+    float a[3];
+    float B[9];
+    float G[9];
+};
+
+template <int MAT>
+__device__ __forceinline__ void load_scatter(const PartView &pv, int i, const MatP &mat, float dt,
+                                             bool tau_stored, int sweeps, Scatter &sc,
+                                             uint32_t *counters)
+{
+    sc.m = mat.mass;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) sc.a[r] = mat.mass * ld(pv, PL_V + r, i);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sc.B[k] = mat.mass * ld(pv, PL_C + k, i);
+    if (MAT == MPM_WATER) {
+        float J = ld(pv, PL_S, i);
+        float pr = mat.eos_k * (powf(J, -mat.eos_gamma) - 1.0f);   // MPMCUDA.cu:353
+        float c = dt * mat.vol * pr * J;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sc.G[k] = (k % 4 == 0) ? c : 0.0f;
+    } else {
+        float t[6];
+        if (tau_stored) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t[k] = ld(pv, PL_TAU + k, i);
+        } else {
+            float F[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+            float Jp = (MAT == MPM_SNOW) ? ld(pv, PL_S, i) : 1.0f;
+            stress_from_F<MAT>(mat, F, Jp, sweeps, t, counters);
+        }
+        float c = -dt * mat.vol;
+        sc.G[0] = c * t[0]; sc.G[4] = c * t[1]; sc.G[8] = c * t[2];
+        sc.G[1] = sc.G[3] = c * t[3];
+        sc.G[2] = sc.G[6] = c * t[4];
+        sc.G[5] = sc.G[7] = c * t[5];
+    }
+}
+
+// Contribution of one particle to stencil node (ox, oy, oz): out = (m, p).
+__device__ __forceinline__ float4 node_contrib(const GridP &g, const Stencil &s, const Scatter &sc,
+                                               int ox, int oy, int oz)
+{
+    float wx = s.w[0][ox], wy = s.w[1][oy], wz = s.w[2][oz];
+    float w = wx * wy * wz;
+    float gr[3] = {g.inv_h * s.dw[0][ox] * wy * wz, g.inv_h * wx * s.dw[1][oy] * wz,
+                   g.inv_h * wx * wy * s.dw[2][oz]};
+    float d[3] = {((float)ox - s.fx[0]) * g.h, ((float)oy - s.fx[1]) * g.h, ((float)oz - s.fx[2]) * g.h};
+    float4 r;
+    r.x = w * sc.m;
+    float p[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float bd = sc.B[3 * c] * d[0] + sc.B[3 * c + 1] * d[1] + sc.B[3 * c + 2] * d[2];
+        float gg = sc.G[3 * c] * gr[0] + sc.G[3 * c + 1] * gr[1] + sc.G[3 * c + 2] * gr[2];
+        p[c] = w * (sc.a[c] + bd) + gg;
+    }
+    r.y = p[0]; r.z = p[1]; r.w = p[2];
+    return r;
+}
+
+}  // namespace mpm

@@ -1,0 +1,28 @@
+// mpm_launch.h -- host-side launchers of the step kernels.
+#pragma once
+
+#include "mpm_internal.h"
+
+namespace mpm {
+
+int launch_zero_u32(hipStream_t st, uint32_t *p, int n);
+int launch_p2g_atomic(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+                      int sweeps, int tau_stored, float4 *grid, uint32_t *flags, uint32_t *counters,
+                      int threads);
+int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters);
+int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
+                       uint32_t *counters, float4 *gmp, float4 *gv, int nblocks);
+int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+               const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads);
+int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);
+
+// FAST form (mpm_fast.hip)
+int fast_prepare(mpm_sim *s);                 // (re)sort particles and build P2G tiles when due
+int launch_p2g_tiles(mpm_sim *s, int group, float dt);
+int launch_g2p_fast(mpm_sim *s, int group, float dt);
+
+// Scene sampling and files (scene.cpp)
+int sample_object(const mpm_config &cfg, const mpm_object &obj, std::vector<float> &xyz);
+int write_bgeo(const char *path, const float *xyz, int64_t n);
+
+}  // namespace mpm

@@ -1,0 +1,224 @@
+// mpm_math.h -- per-particle math of the MPM step, written once for FP32
+// (the hot path) and FP64 (the SVD fallback).  Compiles as HIP device code
+// and, with MPM_HD empty, as plain host C++ for CPU studies of the FP32
+// algorithms.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPM_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#define MPM_HD inline
+#endif
+
+namespace mpm {
+
+template <class T> MPM_HD T m_sqrt(T x) { return sqrt(x); }
+template <> MPM_HD float m_sqrt<float>(float x) { return sqrtf(x); }
+template <class T> MPM_HD T m_abs(T x) { return x < T(0) ? -x : x; }
+
+// ---------------------------------------------------------------------------
+// Quadratic B-spline N and N' (reference MPMCUDA.cu:267-290, branchy |x| form
+// with the same half-open intervals), evaluated at the three stencil offsets
+// of one axis.  fx is the particle coordinate relative to the base node in
+// cell units, d_o = fx - o for o = 0, 1, 2.
+template <class T>
+MPM_HD void bspline_axis(T fx, T w[3], T dw[3])
+{
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        T d = fx - T(o);
+        T a = m_abs(d);
+        T inner = T(0.75) - a * a;
+        T outer = T(0.5) * (T(1.5) - a) * (T(1.5) - a);
+        w[o] = a < T(0.5) ? inner : (a < T(1.5) ? outer : T(0));
+        T g = T(0);
+        if (d > T(-0.5) && d < T(0.5)) g = T(-2) * d;
+        else if (d >= T(0.5) && d < T(1.5)) g = d - T(1.5);
+        else if (d > T(-1.5) && d <= T(-0.5)) g = T(1.5) + d;
+        dw[o] = g;
+    }
+}
+
+// Base node index on FP32 positions: t = (x - o) * inv_h - 0.5f with every
+// operation rounded to FP32 (no contraction), base = max(0, floor(t)).  The
+// oracle's orc_base_f32 is the same formula; the reference form is
+// max(0, int((x - o - h/2)/h)) in FP64 (MPMCUDA.cu:356-358).
+MPM_HD int base_index_f32(float x, float o, float inv_h)
+{
+#pragma clang fp contract(off)
+    float d = x - o;
+    float s = d * inv_h;
+    float t = s - 0.5f;
+    int b = (int)floorf(t);
+    return b < 0 ? 0 : b;
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 helpers, row-major A[3*r+c].
+template <class T> MPM_HD T det3(const T A[9])
+{
+    return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+           A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+template <class T> MPM_HD void matmul3(const T A[9], const T B[9], T C[9])
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+
+// Rotation matrix of a unit quaternion (w, x, y, z).
+template <class T> MPM_HD void quat_to_mat(const T q[4], T R[9])
+{
+    T w = q[0], x = q[1], y = q[2], z = q[3];
+    R[0] = T(1) - T(2) * (y * y + z * z); R[1] = T(2) * (x * y - w * z); R[2] = T(2) * (x * z + w * y);
+    R[3] = T(2) * (x * y + w * z); R[4] = T(1) - T(2) * (x * x + z * z); R[5] = T(2) * (y * z - w * x);
+    R[6] = T(2) * (x * z - w * y); R[7] = T(2) * (y * z + w * x); R[8] = T(1) - T(2) * (x * x + y * y);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 SVD  F = U diag(s) V^T,  U, V in SO(3),  s0 >= s1 >= |s2|,  sign(s2) =
+// sign(det F) -- the conventions of the reference's SVD (McAdams et al. 2011,
+// used through PolarSVD, MPMCUDA.cu:87-134).  Method (own): one-sided
+// Jacobi rotations on the columns of B = F V (starting from V = V0 when a
+// warm start is given), a fixed number of cyclic sweeps, columns sorted by
+// norm with a sign flip per swap (keeps det V = +1), then a Givens QR of B
+// gives U and the signed singular values.  Returns the largest remaining
+// column non-orthogonality |b_p . b_q| / (|b_p| |b_q|) before sorting.
+template <class T>
+MPM_HD void jacobi_rotate(T B[9], T W[9], int p, int q)
+{
+    T a = B[p] * B[p] + B[3 + p] * B[3 + p] + B[6 + p] * B[6 + p];
+    T b = B[q] * B[q] + B[3 + q] * B[3 + q] + B[6 + q] * B[6 + q];
+    T g = B[p] * B[q] + B[3 + p] * B[3 + q] + B[6 + p] * B[6 + q];
+    T zeta = (b - a) / (T(2) * g);
+    T t = (zeta >= T(0) ? T(1) : T(-1)) / (m_abs(zeta) + m_sqrt(T(1) + zeta * zeta));
+    // g == 0 (already orthogonal, or 0/0) and overflowing zeta: no rotation
+    if (!(g != T(0)) || !(t == t)) t = T(0);
+    T c = T(1) / m_sqrt(T(1) + t * t), s = c * t;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        T bp = B[3 * r + p], bq = B[3 * r + q];
+        B[3 * r + p] = c * bp - s * bq;
+        B[3 * r + q] = s * bp + c * bq;
+        T wp = W[3 * r + p], wq = W[3 * r + q];
+        W[3 * r + p] = c * wp - s * wq;
+        W[3 * r + q] = s * wp + c * wq;
+    }
+}
+
+template <class T>
+MPM_HD T col_offdiag(const T B[9])
+{
+    T n0 = B[0] * B[0] + B[3] * B[3] + B[6] * B[6];
+    T n1 = B[1] * B[1] + B[4] * B[4] + B[7] * B[7];
+    T n2 = B[2] * B[2] + B[5] * B[5] + B[8] * B[8];
+    T g01 = B[0] * B[1] + B[3] * B[4] + B[6] * B[7];
+    T g02 = B[0] * B[2] + B[3] * B[5] + B[6] * B[8];
+    T g12 = B[1] * B[2] + B[4] * B[5] + B[7] * B[8];
+    T r01 = g01 * g01 / (n0 * n1 + T(1e-30));
+    T r02 = g02 * g02 / (n0 * n2 + T(1e-30));
+    T r12 = g12 * g12 / (n1 * n2 + T(1e-30));
+    T r = r01 > r02 ? r01 : r02;
+    r = r > r12 ? r : r12;
+    return m_sqrt(r);
+}
+
+template <class T>
+MPM_HD void swap_cols_neg(T B[9], T W[9], int j)
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        T tb = B[3 * r + j]; B[3 * r + j] = -B[3 * r + j + 1]; B[3 * r + j + 1] = tb;
+        T tw = W[3 * r + j]; W[3 * r + j] = -W[3 * r + j + 1]; W[3 * r + j + 1] = tw;
+    }
+}
+
+template <class T>
+MPM_HD void givens_rows(T R[9], T Q[9], int i, int j, int k)
+{
+    T a = R[3 * i + k], b = R[3 * j + k];
+    T rr = m_sqrt(a * a + b * b);
+    T c = T(1), s = T(0);
+    if (rr > T(0)) { c = a / rr; s = b / rr; }
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        T ri = R[3 * i + col], rj = R[3 * j + col];
+        R[3 * i + col] = c * ri + s * rj;
+        R[3 * j + col] = -s * ri + c * rj;
+        T qi = Q[3 * col + i], qj = Q[3 * col + j];
+        Q[3 * col + i] = c * qi + s * qj;
+        Q[3 * col + j] = -s * qi + c * qj;
+    }
+}
+
+// Finish an SVD from B = F W with (nearly) orthogonal columns.
+template <class T>
+MPM_HD void svd3_finish(T B[9], T W[9], T U[9], T s[3], T V[9])
+{
+    T n0 = B[0] * B[0] + B[3] * B[3] + B[6] * B[6];
+    T n1 = B[1] * B[1] + B[4] * B[4] + B[7] * B[7];
+    T n2 = B[2] * B[2] + B[5] * B[5] + B[8] * B[8];
+    if (n0 < n1) { swap_cols_neg(B, W, 0); T t = n0; n0 = n1; n1 = t; }
+    if (n1 < n2) { swap_cols_neg(B, W, 1); T t = n1; n1 = n2; n2 = t; }
+    if (n0 < n1) { swap_cols_neg(B, W, 0); }
+    T Q[9] = {T(1), T(0), T(0), T(0), T(1), T(0), T(0), T(0), T(1)};
+    givens_rows(B, Q, 0, 1, 0);
+    givens_rows(B, Q, 0, 2, 0);
+    givens_rows(B, Q, 1, 2, 1);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { U[k] = Q[k]; V[k] = W[k]; }
+    s[0] = B[0]; s[1] = B[4]; s[2] = B[8];
+}
+
+template <class T>
+MPM_HD T svd3(const T F[9], T U[9], T s[3], T V[9], int sweeps, const T *V0 = nullptr)
+{
+    T B[9], W[9];
+    if (V0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) W[k] = V0[k];
+        matmul3(F, W, B);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { B[k] = F[k]; W[k] = (k % 4 == 0) ? T(1) : T(0); }
+    }
+    for (int it = 0; it < sweeps; ++it) {
+        jacobi_rotate(B, W, 0, 1);
+        jacobi_rotate(B, W, 0, 2);
+        jacobi_rotate(B, W, 1, 2);
+    }
+    T off = col_offdiag(B);
+    svd3_finish(B, W, U, s, V);
+    return off;
+}
+
+// Kirchhoff stress of the fixed-corotated model from an SVD of F:
+// tau = 2 mu (F - R) F^T + lam J (J - 1) I with R = U V^T (MPMCUDA.cu:229-264)
+// equals U diag(2 mu (s_i - 1) s_i + lam J (J - 1)) U^T.  Output is the
+// symmetric tensor as (xx, yy, zz, xy, xz, yz).
+template <class T>
+MPM_HD void fcr_tau_from_svd(const T U[9], const T s[3], T mu, T lam, T tau6[6])
+{
+    T J = s[0] * s[1] * s[2];
+    T l = lam * J * (J - T(1));
+    T d0 = T(2) * mu * (s[0] - T(1)) * s[0] + l;
+    T d1 = T(2) * mu * (s[1] - T(1)) * s[1] + l;
+    T d2 = T(2) * mu * (s[2] - T(1)) * s[2] + l;
+    // tau_rc = sum_k U_rk d_k U_ck
+    tau6[0] = U[0] * U[0] * d0 + U[1] * U[1] * d1 + U[2] * U[2] * d2;
+    tau6[1] = U[3] * U[3] * d0 + U[4] * U[4] * d1 + U[5] * U[5] * d2;
+    tau6[2] = U[6] * U[6] * d0 + U[7] * U[7] * d1 + U[8] * U[8] * d2;
+    tau6[3] = U[0] * U[3] * d0 + U[1] * U[4] * d1 + U[2] * U[5] * d2;
+    tau6[4] = U[0] * U[6] * d0 + U[1] * U[7] * d1 + U[2] * U[8] * d2;
+    tau6[5] = U[3] * U[6] * d0 + U[4] * U[7] * d1 + U[5] * U[8] * d2;
+}
+
+}  // namespace mpm

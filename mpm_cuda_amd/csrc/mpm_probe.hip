@@ -1,0 +1,113 @@
+// mpm_probe.hip -- single-stage device probes and host-only sampling entry
+// points of the C ABI (no reference counterpart; used by the parity tests).
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mpm_kernels.h"
+#include "mpm_launch.h"
+
+namespace mpm {
+
+int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, uint32_t *perm_h);
+
+__global__ void k_probe_base(const float *x, int n, float origin, float inv_h, int32_t *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = base_index_f32(x[i], origin, inv_h);
+}
+
+__global__ void k_probe_svd(const float *F, int n, int sweeps, float *U, float *s, float *V, uint32_t *counters)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float f[9], u[9], sg[3], v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k] = F[9 * i + k];
+    svd_checked(f, u, sg, v, sweeps, counters);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { U[9 * i + k] = u[k]; V[9 * i + k] = v[k]; }
+    s[3 * i] = sg[0]; s[3 * i + 1] = sg[1]; s[3 * i + 2] = sg[2];
+}
+
+// RAII set of device temporaries for the probes.
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs() { for (void *q : p) if (q) (void)hipFree(q); }
+    template <class T> hipError_t alloc(T **out, size_t bytes)
+    {
+        hipError_t e = hipMalloc((void **)out, bytes ? bytes : 4);
+        if (e == hipSuccess) p.push_back(*out);
+        return e;
+    }
+};
+
+}  // namespace mpm
+
+using namespace mpm;
+
+extern "C" {
+
+int mpm_sample_object(const mpm_config *cfg, const mpm_object *obj, float *xyz, int64_t *n)
+{
+    if (!cfg || !obj || !n) return set_error(MPM_ERR_INVALID, "null argument");
+    std::vector<float> pts;
+    int rc = sample_object(*cfg, *obj, pts);
+    if (rc) return rc;
+    int64_t cnt = (int64_t)pts.size() / 3;
+    if (xyz) memcpy(xyz, pts.data(), (size_t)std::min(*n, cnt) * 3 * sizeof(float));
+    *n = cnt;
+    return MPM_OK;
+}
+
+int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, int32_t *out)
+{
+    if ((!x || !out) && n > 0) return set_error(MPM_ERR_INVALID, "null argument");
+    if (n <= 0) return MPM_OK;
+    if (n > (1ll << 30)) return set_error(MPM_ERR_INVALID, "too many values");
+    DevBufs b;
+    float *dx = nullptr;
+    int32_t *dout = nullptr;
+    MPM_HIP(b.alloc(&dx, n * 4));
+    MPM_HIP(b.alloc(&dout, n * 4));
+    MPM_HIP(hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_base, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx, (int)n, origin, inv_h,
+                       dout);
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_probe_svd(const float *F, int64_t n, int32_t sweeps, float *U, float *s, float *V)
+{
+    if (n > 0 && (!F || !U || !s || !V)) return set_error(MPM_ERR_INVALID, "null argument");
+    if (n <= 0) return MPM_OK;
+    if (sweeps < 0 || sweeps > 16) return set_error(MPM_ERR_INVALID, "sweeps out of range [0,16]");
+    if (n > (1ll << 26)) return set_error(MPM_ERR_INVALID, "too many matrices");
+    DevBufs b;
+    float *dF = nullptr, *dU = nullptr, *ds = nullptr, *dV = nullptr;
+    uint32_t *cnt = nullptr;
+    MPM_HIP(b.alloc(&dF, n * 36));
+    MPM_HIP(b.alloc(&dU, n * 36));
+    MPM_HIP(b.alloc(&dV, n * 36));
+    MPM_HIP(b.alloc(&ds, n * 12));
+    MPM_HIP(b.alloc(&cnt, CT_COUNT * 4));
+    MPM_HIP(hipMemset(cnt, 0, CT_COUNT * 4));
+    MPM_HIP(hipMemcpy(dF, F, n * 36, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_svd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dF, (int)n, sweeps, dU, ds,
+                       dV, cnt);
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipMemcpy(U, dU, n * 36, hipMemcpyDeviceToHost));
+    MPM_HIP(hipMemcpy(V, dV, n * 36, hipMemcpyDeviceToHost));
+    MPM_HIP(hipMemcpy(s, ds, n * 12, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_probe_sort(const uint32_t *keys, int64_t n, int32_t bits, uint32_t *sorted, uint32_t *perm)
+{
+    if (n > 0 && (!keys || !sorted || !perm)) return set_error(MPM_ERR_INVALID, "null argument");
+    return probe_sort(keys, n, bits, sorted, perm);
+}
+
+}  // extern "C"

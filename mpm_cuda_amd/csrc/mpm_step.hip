@@ -1,0 +1,332 @@
+// mpm_step.hip -- HIP kernels of the explicit MPM step (gfx950).
+//
+// Reference: JerryYan97/MPM_CUDA src/simulator/MPMCUDA.cu -- P2G :310-433,
+// VelUpdate :435-472, InterpolateAndMove :474-741, step :793-925.  Particles
+// are FP32 structure-of-arrays; the grid is block-major (4^3 nodes per block)
+// and only blocks touched by the current P2G are updated and read.
+#include "mpm_kernels.h"
+#include "mpm_launch.h"
+
+namespace mpm {
+
+// ---------------------------------------------------------------------------
+// Small utilities.
+__global__ void k_zero_u32(uint32_t *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__device__ __forceinline__ void mark_blocks(const GridP &g, const Stencil &s, uint32_t *flags)
+{
+    int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2;
+    int by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
+    int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
+    for (int bz = bz0; bz <= bz1; ++bz)
+        for (int by = by0; by <= by1; ++by)
+            for (int bx = bx0; bx <= bx1; ++bx)
+                flags[((int64_t)bz * g.nby + by) * g.nbx + bx] = 1u;
+}
+
+// ---------------------------------------------------------------------------
+// PLAIN form P2G: one thread per particle, 27 nodes x 4 FP32 global atomics,
+// the stress SVD evaluated here (MPMCUDA.cu:310-433).
+template <int MAT>
+__global__ void __launch_bounds__(256) k_p2g_atomic(GridP g, MatP mat, PartView pv, float dt,
+                                                    int sweeps, int tau_stored, float4 *grid,
+                                                    uint32_t *flags, uint32_t *counters)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pv.n) return;
+    float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+    Stencil s;
+    uint32_t bits = make_stencil(g, x, s);
+    Scatter sc;
+    load_scatter<MAT>(pv, i, mat, dt, tau_stored != 0, sweeps, sc, counters);
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
+                unsafeAtomicAdd(dst + 0, c.x);
+                unsafeAtomicAdd(dst + 1, c.y);
+                unsafeAtomicAdd(dst + 2, c.z);
+                unsafeAtomicAdd(dst + 3, c.w);
+            }
+    mark_blocks(g, s, flags);
+    if (bits) atomicOr(counters + CT_HEALTH, bits);
+}
+
+// ---------------------------------------------------------------------------
+// Active-block list: compacts the block flags written by P2G (and clears them).
+__global__ void k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    bool f = b < nblocks && flags[b] != 0u;
+    if (f) flags[b] = 0u;
+    uint64_t mask = __ballot(f);
+    if (mask == 0) return;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counters + CT_ACTIVE, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    if (f) list[base + __popcll(below)] = (uint32_t)b;
+}
+
+// ---------------------------------------------------------------------------
+// Grid velocity update over active blocks (MPMCUDA.cu:435-472): v = p / m,
+// gravity, walls.  Reads grid_mp, writes grid_v and clears grid_mp for the
+// next P2G.  One wave per block, persistent over the active list.
+__global__ void __launch_bounds__(256) k_grid_update(GridP g, float dt, const uint32_t *list,
+                                                     uint32_t *counters, float4 *gmp,
+                                                     float4 *gv)
+{
+    const uint32_t nact = counters[CT_ACTIVE];
+    const int lane = threadIdx.x & 63;
+    for (uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6); k < nact; k += gridDim.x * 4) {
+        uint32_t b = list[k];
+        int64_t idx = (int64_t)b * BLK_NODES + lane;
+        float4 a = gmp[idx];
+        gmp[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!(fabsf(a.x) < 2.220446049250313e-16f)) {          // MPMCUDA.cu:444
+            int bx = (int)(b % (uint32_t)g.nbx);
+            int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
+            int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
+            int ix = bx * BLK + (lane & 3), iy = by * BLK + ((lane >> 2) & 3), iz = bz * BLK + (lane >> 4);
+            float vx = a.y / a.x + g.gx * dt;                  // MPMCUDA.cu:452-457
+            float vy = a.z / a.x + g.gy * dt;
+            float vz = a.w / a.x + g.gz * dt;
+            bool wx = ix <= g.pad || ix >= g.nx - g.pad;       // MPMCUDA.cu:463-469
+            bool wy = iy <= g.pad || iy >= g.ny - g.pad;
+            bool wz = iz <= g.pad || iz >= g.nz - g.pad;
+            if (g.bc_mode == MPM_BC_STICKY) {
+                if (wx || wy || wz) vx = vy = vz = 0.f;
+            } else {
+                if (wx) vx = 0.f;
+                if (wy) vy = 0.f;
+                if (wz) vz = 0.f;
+            }
+            out = make_float4(vx, vy, vz, a.x);
+        }
+        gv[idx] = out;
+        uint64_t has_mass = __ballot(out.w != 0.f);
+        if (lane == 0 && has_mass) atomicAdd(counters + CT_NODES, (uint32_t)__popcll(has_mass));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// G2P, deformation update and advection for one particle
+// (MPMCUDA.cu:474-741).  With write_tau the Kirchhoff stress of the updated
+// state is stored for the next P2G, so a step needs one SVD.
+template <int MAT>
+__device__ __forceinline__ uint32_t g2p_particle(const GridP &g, const MatP &mat, const PartView &pv,
+                                                 int i, float dt, const float4 *__restrict__ gv,
+                                                 bool write_tau, int sweeps, uint32_t *counters)
+{
+    float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+    Stencil s;
+    uint32_t bits = make_stencil(g, x, s);
+    float v[3] = {0.f, 0.f, 0.f}, C[9], L[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { C[k] = 0.f; L[k] = 0.f; }
+    const float c4 = 4.0f * g.inv_h;       // 4 w / h^2 (x_i - x_p), with x_i - x_p = (o - fx) h
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                float4 nv = gv[node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz)];
+                float wx = s.w[0][ox], wy = s.w[1][oy], wz = s.w[2][oz];
+                float w = wx * wy * wz;
+                float gr[3] = {g.inv_h * s.dw[0][ox] * wy * wz, g.inv_h * wx * s.dw[1][oy] * wz,
+                               g.inv_h * wx * wy * s.dw[2][oz]};
+                float dd[3] = {(float)ox - s.fx[0], (float)oy - s.fx[1], (float)oz - s.fx[2]};
+                float vi[3] = {nv.x, nv.y, nv.z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    v[r] += w * vi[r];                                   // :516-518
+                    float cw = c4 * w * vi[r];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        C[3 * r + c] += cw * dd[c];                      // :520-532
+                        L[3 * r + c] += vi[r] * gr[c];                   // :561-585
+                    }
+                }
+            }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) st(pv, PL_V + r, i, v[r]);
+    if (MAT == MPM_WATER && mat.water_c_mode == MPM_WATER_C_GRADV) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) st(pv, PL_C + k, i, L[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) st(pv, PL_C + k, i, C[k]);
+    }
+    float A[9];                                                          // I + dt grad v
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A[k] = dt * L[k] + ((k % 4 == 0) ? 1.0f : 0.0f);
+    if (MAT == MPM_JELLO) {                                              // :596-619
+        float F[9], Fn[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+        matmul3<float>(A, F, Fn);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fn[k]);
+        if (!(det3<float>(Fn) > 0.f)) bits |= MPM_H_DET_NONPOS;
+        if (write_tau) {
+            float t[6];
+            stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, t, counters);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, t[k]);
+        }
+    } else if (MAT == MPM_SNOW) {                                        // :620-709
+        float F[9], Ft[9], U[9], sg[3], V[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+        matmul3<float>(A, F, Ft);
+        svd_checked(Ft, U, sg, V, sweeps, counters);
+        float ratio = 1.0f;
+        float sc[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sc[k] = fminf(fmaxf(sg[k], 1.0f - mat.theta_c), 1.0f + mat.theta_s);   // :658-660
+            ratio *= sg[k] / sc[k];
+        }
+        float Fe[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                Fe[3 * r + c] = U[3 * r] * sc[0] * V[3 * c] + U[3 * r + 1] * sc[1] * V[3 * c + 1] +
+                                U[3 * r + 2] * sc[2] * V[3 * c + 2];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fe[k]);
+        // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
+        float Jp = ld(pv, PL_S, i) * ratio;
+        st(pv, PL_S, i, Jp);
+        if (!(Jp > 0.f)) bits |= MPM_H_DET_NONPOS;
+        if (write_tau) {
+            float hc = expf(mat.xi * (1.0f - Jp));
+            float t[6];
+            fcr_tau_from_svd<float>(U, sc, mat.mu * hc, mat.lam * hc, t);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, t[k]);
+        }
+    } else {                                                             // :710-713
+        float J = ld(pv, PL_S, i);
+        st(pv, PL_S, i, (1.0f + dt * (L[0] + L[4] + L[8])) * J);
+    }
+    const float lo[3] = {g.ox, g.oy, g.oz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float xn = x[a] + dt * v[a];                                     // :717-719
+        st(pv, PL_X + a, i, xn);
+        float hi = lo[a] + g.h * (float)(nn[a] - 1);
+        if (!(xn >= lo[a] && xn <= hi)) bits |= MPM_H_LEFT_DOMAIN;       // :722-739
+        if (!isfinite(xn) || !isfinite(v[a])) bits |= MPM_H_NONFINITE;
+    }
+    return bits;
+}
+
+template <int MAT>
+__global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, float dt,
+                                             const float4 *__restrict__ gv, int write_tau,
+                                             int sweeps, uint32_t *counters)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pv.n) return;
+    uint32_t bits = g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters);
+    if (bits) atomicOr(counters + CT_HEALTH, bits);
+}
+
+// Kirchhoff stress of the current state, for the first P2G of the FAST form
+// (and after the host rewrites F or J_p).
+template <int MAT>
+__global__ void k_compute_tau(MatP mat, PartView pv, int sweeps, uint32_t *counters)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pv.n) return;
+    float F[9], t[6];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+    float Jp = (MAT == MPM_SNOW) ? ld(pv, PL_S, i) : 1.0f;
+    stress_from_F<MAT>(mat, F, Jp, sweeps, t, counters);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, t[k]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.
+static inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+int launch_zero_u32(hipStream_t st, uint32_t *p, int n)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_zero_u32, dim3(nblk(n, 256)), dim3(256), 0, st, p, n);
+    return (int)hipGetLastError();
+}
+
+int launch_p2g_atomic(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+                      int sweeps, int tau_stored, float4 *grid, uint32_t *flags, uint32_t *counters,
+                      int threads)
+{
+    if (pv.n <= 0) return 0;
+    dim3 gr(nblk(pv.n, threads)), bl(threads);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_atomic<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, sweeps, tau_stored, grid, flags, counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_atomic<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, sweeps, tau_stored, grid, flags, counters); break;
+    default: hipLaunchKernelGGL(k_p2g_atomic<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, sweeps, tau_stored, grid, flags, counters); break;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters)
+{
+    hipLaunchKernelGGL(k_compact_blocks, dim3(nblk(nblocks, 256)), dim3(256), 0, st, flags, nblocks, list,
+                       counters);
+    return (int)hipGetLastError();
+}
+
+int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
+                       uint32_t *counters, float4 *gmp, float4 *gv, int nblocks)
+{
+    int wg = nblk(nblocks, 4);
+    if (wg > 8192) wg = 8192;
+    if (wg < 1) wg = 1;
+    hipLaunchKernelGGL(k_grid_update, dim3(wg), dim3(256), 0, st, g, dt, list, counters, gmp, gv);
+    return (int)hipGetLastError();
+}
+
+int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+               const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads)
+{
+    if (pv.n <= 0) return 0;
+    dim3 gr(nblk(pv.n, threads)), bl(threads);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_g2p<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_g2p<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
+    default: hipLaunchKernelGGL(k_g2p<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters)
+{
+    if (pv.n <= 0 || m.type == MPM_WATER) return 0;
+    dim3 gr(nblk(pv.n, 256)), bl(256);
+    if (m.type == MPM_JELLO)
+        hipLaunchKernelGGL(k_compute_tau<MPM_JELLO>, gr, bl, 0, st, m, pv, sweeps, counters);
+    else
+        hipLaunchKernelGGL(k_compute_tau<MPM_SNOW>, gr, bl, 0, st, m, pv, sweeps, counters);
+    return (int)hipGetLastError();
+}
+
+}  // namespace mpm

@@ -1,0 +1,190 @@
+"""HIP path vs the FP64 oracle on identical FP32 initial states (GPU only).
+
+Bars: grid fields after P2G and after the grid update within 1e-5 normwise
+relative error; particle fields after one step within 1e-5 (x, v, C, F,
+J_p / J); integer work (base indices, sort) bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mpm_cuda_amd import capi
+from tests.scenes import Pair, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_GRID = 1e-5
+TOL_PART = 1e-5
+
+MATS = [capi.JELLO, capi.SNOW, capi.WATER]
+FORMS = [capi.FORM_PLAIN, capi.FORM_FAST]
+MAT_ID = {capi.JELLO: "jello", capi.SNOW: "snow", capi.WATER: "water"}
+FORM_ID = {capi.FORM_PLAIN: "plain", capi.FORM_FAST: "fast"}
+
+
+def _ids(p):
+    return f"{MAT_ID[p[0]]}-{FORM_ID[p[1]]}"
+
+
+CASES = [(m, f) for m in MATS for f in FORMS]
+
+
+def test_base_index_bitexact(gpu):
+    """Device base-node index == the oracle's FP32 formula, bit for bit,
+    including values a few ulps around every cell threshold."""
+    rng = np.random.default_rng(7)
+    n_grid = 192
+    inv_h = np.float32(n_grid)
+    x = rng.random(200000).astype(np.float32)
+    thr = ((np.arange(1, n_grid - 1) + 0.5) / n_grid).astype(np.float32)
+    near = [thr]
+    for k in range(1, 6):
+        near.append(np.nextafter(near[-1], np.float32(2)))
+    down = [thr]
+    for k in range(1, 6):
+        down.append(np.nextafter(down[-1], np.float32(-1)))
+    x = np.concatenate([x, *near, *down, np.float32([0.0, 1e-8, 0.5 / n_grid])])
+    got = capi.probe_base_index(x, 0.0, inv_h)
+    want = np.array([O.base_f32(v, 0.0, inv_h) for v in x], np.int32)
+    assert np.array_equal(got, want)
+    # disagreement with the reference's FP64 formula is confined to thresholds
+    ref = np.array([O.base_ref(float(v), 0.0, 1.0 / n_grid) for v in x], np.int32)
+    diff = np.nonzero(ref != got)[0]
+    assert np.all(np.abs(ref[diff] - got[diff]) == 1)
+    frac_random = np.mean(ref[:200000] != got[:200000])
+    assert frac_random < 1e-4, frac_random
+
+
+def test_radix_sort_stable(gpu):
+    rng = np.random.default_rng(3)
+    for n, bits in [(1, 8), (1000, 12), (123457, 17), (300000, 24), (70000, 32)]:
+        keys = rng.integers(0, 2 ** min(bits, 31), size=n, dtype=np.uint64).astype(np.uint32)
+        if bits == 32:
+            keys = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+        keys[: n // 3] = keys[0]   # many duplicates: stability matters
+        srt, perm = capi.probe_sort(keys, bits)
+        ref = np.argsort(keys, kind="stable")
+        assert np.array_equal(perm, ref.astype(np.uint32)), (n, bits)
+        assert np.array_equal(srt, keys[ref])
+
+
+def test_svd_vs_lapack_fixtures(gpu):
+    d = np.load("tests/golden/svd_fixtures.npz")
+    F, S, R, kind = d["F"], d["S"], d["R"], d["kind"]
+    U, s, V = capi.probe_svd(F.astype(np.float32), sweeps=4)
+    U = U.astype(np.float64); V = V.astype(np.float64); s = s.astype(np.float64)
+    for i in range(len(F)):
+        if kind[i] == "zero":
+            continue
+        scale = abs(S[i][0])
+        assert abs(np.linalg.det(U[i]) - 1) < 1e-5 and abs(np.linalg.det(V[i]) - 1) < 1e-5, (i, kind[i])
+        assert np.abs(s[i] - S[i]).max() <= 2e-6 * scale + 1e-7, (i, kind[i], s[i], S[i])
+        rec = U[i] @ np.diag(s[i]) @ V[i].T
+        assert np.abs(rec - F[i]).max() <= 3e-6 * scale, (i, kind[i])
+        cond = abs(S[i][2]) / scale
+        if kind[i] not in ("repeated", "isotropic", "identity", "diagonal", "rotation") and cond > 1e-3:
+            Rg = U[i] @ V[i].T
+            assert np.abs(Rg - R[i]).max() < 1e-4 / cond, (i, kind[i])
+
+
+def _p2g_compare(p: Pair):
+    p.gpu.phase(capi.PHASE_P2G, p.dt)
+    p.orc.clear_grid()
+    flags = p.orc.p2g(p.dt)
+    assert flags == 0
+    m_g = p.gpu.grid(capi.G_MASS)
+    mom_g = p.gpu.grid(capi.G_MOMENTUM)
+    m_o = p.orc.grid_mass()
+    mom_o = p.orc.grid_vec()
+    return rel(m_g, m_o), rel(mom_g, mom_o)
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_p2g_grid_parity(gpu, case):
+    mat, form = case
+    p = Pair(type_=mat, form=form, seed=11)
+    try:
+        em, ep = _p2g_compare(p)
+        assert em < TOL_GRID and ep < TOL_GRID, (em, ep)
+        # mass conservation
+        assert abs(p.gpu.grid(capi.G_MASS).astype(np.float64).sum() - p.orc.groups[0].mat.mass * p.n) \
+            < 1e-5 * p.orc.groups[0].mat.mass * p.n
+        p.gpu.phase(capi.PHASE_GRID, p.dt)
+        p.orc.grid_update(p.dt)
+        vg = p.gpu.grid(capi.G_VELOCITY).astype(np.float64)
+        vo = p.orc.grid_vec()
+        mo = p.orc.grid_mass()[..., None]
+        # mass-weighted (tiny-mass nodes amplify FP32 error in v = p/m)
+        ev = rel(vg * mo, vo * mo)
+        assert ev < TOL_GRID, ev
+    finally:
+        p.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_one_step_particle_parity(gpu, case):
+    mat, form = case
+    p = Pair(type_=mat, form=form, seed=21)
+    try:
+        p.gpu.step(p.dt)
+        assert p.orc.step(p.dt) == 0
+        g, o = p.gpu_particles(), p.orc_particles()
+        for k in ("x", "v", "C", "F", "S"):
+            if mat == capi.WATER and k == "F":
+                continue
+            e = rel(g[k], o[k]) if k not in ("F", "S") else rel(g[k] - (np.eye(3) if k == "F" else 1.0),
+                                                              o[k] - (np.eye(3) if k == "F" else 1.0))
+            assert e < (TOL_PART if k != "S" else 1e-4), (k, e)
+        assert p.gpu.health() & ~capi.H_ARENA_SPILL == 0
+    finally:
+        p.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_rollout_parity(gpu, case):
+    """50 steps from the rest state with an initial velocity; tolerances grow
+    with FP32 error accumulation."""
+    mat, form = case
+    p = Pair(type_=mat, form=form, seed=5, randomize=False)
+    try:
+        for _ in range(50):
+            p.gpu.step(p.dt)
+            assert p.orc.step(p.dt) == 0
+        g, o = p.gpu_particles(), p.orc_particles()
+        assert rel(g["x"], o["x"]) < 1e-6
+        assert rel(g["v"], o["v"]) < 2e-4
+        if mat != capi.WATER:
+            assert rel(g["F"] - np.eye(3), o["F"] - np.eye(3)) < 2e-3
+        assert p.gpu.health() & ~capi.H_ARENA_SPILL == 0
+    finally:
+        p.close()
+
+
+def test_fast_matches_plain(gpu):
+    """The two execution forms agree with each other after 20 snow steps."""
+    a = Pair(type_=capi.SNOW, form=capi.FORM_PLAIN, seed=9)
+    b = Pair(type_=capi.SNOW, form=capi.FORM_FAST, seed=9, sort_every=3)
+    try:
+        for _ in range(20):
+            a.gpu.step(a.dt)
+            b.gpu.step(b.dt)
+        ga, gb = a.gpu_particles(), b.gpu_particles()
+        for k in ("x", "v", "C"):
+            assert rel(gb[k], ga[k]) < 1e-4, k
+        st = b.gpu.stats()
+        assert st["sorts"] >= 6
+    finally:
+        a.close()
+        b.close()
+
+
+def test_positions_roundtrip_order(gpu):
+    """Reads return particles in their original order after re-sorting."""
+    p = Pair(type_=capi.JELLO, form=capi.FORM_FAST, seed=2, sort_every=1, randomize=False)
+    try:
+        x0 = p.gpu.read(0, capi.P_POS).copy()
+        p.gpu.step(1e-9)
+        x1 = p.gpu.read(0, capi.P_POS)
+        assert np.abs(x1 - x0).max() < 1e-6
+    finally:
+        p.close()
