@@ -69,6 +69,12 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}")
     os.replace(tmp, LIB)
+    # command-line driver over the C++ mirror header, linked against the library
+    cli = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "mpm_run.cpp"),
+           "-o", os.path.join(HERE, "mpm_run"), f"-L{HERE}", "-lmpm_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cli, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mpm_run build failed:\n{r.stdout}")
     return LIB
 
 

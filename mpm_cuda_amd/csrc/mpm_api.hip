@@ -434,6 +434,32 @@ int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
     for (uint32_t k = 0; k < nact; ++k) if (lst[k] < (uint32_t)s->nblocks) active[lst[k]] = 1;
     // grid_mp holds (m, p) between P2G and the grid update; grid_v after it
     const bool after_p2g = s->last_phase == MPM_PHASE_P2G;
+    if (after_p2g && s->cfg.form == MPM_FORM_FAST && s->blk_tile) {
+        // FAST form: add the staged block arenas of the last P2G
+        std::vector<uint32_t> tcount(s->groups.size());
+        MPM_HIP(hipMemcpy(tcount.data(), s->tile_counts, tcount.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < s->groups.size(); ++k) {
+            uint32_t nt = std::min<uint32_t>(tcount[k], (uint32_t)std::min<int64_t>(s->nblocks, s->groups[k].n));
+            if (!nt) continue;
+            std::vector<uint4> desc(nt);
+            MPM_HIP(hipMemcpy(desc.data(), (const uint4 *)s->tile_desc + s->group_tile_off[k], nt * sizeof(uint4),
+                              hipMemcpyDeviceToHost));
+            std::vector<float4> arena(ARENA_NODES);
+            for (uint32_t t = 0; t < nt; ++t) {
+                if (desc[t].w >= (uint32_t)s->staging_cap) continue;
+                MPM_HIP(hipMemcpy(arena.data(), s->staging + (int64_t)desc[t].w * ARENA_NODES,
+                                  ARENA_NODES * sizeof(float4), hipMemcpyDeviceToHost));
+                const int b = (int)desc[t].z;
+                const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+                for (int q = 0; q < ARENA_NODES; ++q) {
+                    const int x = bx * BLK - 1 + (q & 7), y = by * BLK - 1 + ((q >> 3) & 7), z = bz * BLK - 1 + (q >> 6);
+                    if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                    float4 &dst = mp[node_slot(g, x, y, z)];
+                    dst.x += arena[q].x; dst.y += arena[q].y; dst.z += arena[q].z; dst.w += arena[q].w;
+                }
+            }
+        }
+    }
     int w = field == MPM_G_MASS ? 1 : 3;
     for (int z = 0; z < g.nz; ++z)
         for (int y = 0; y < g.ny; ++y)
@@ -534,7 +560,8 @@ void mpm_destroy(mpm_sim *s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
                     s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
-                    s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w};
+                    s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
+                    s->cellkey, s->staging, s->blk_tile};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
@@ -607,8 +634,11 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         PhaseTimer t(s, 2);
         rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
         if (rc) return rc;
+        const bool staged = fast && s->blk_tile;
         return check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,
-                                               (int)s->nblocks),
+                                               (int)s->nblocks, staged ? s->staging : nullptr,
+                                               staged ? s->blk_tile : nullptr, (int)s->groups.size(),
+                                               (uint32_t)s->staging_cap),
                             "grid_update");
     }
     // G2P

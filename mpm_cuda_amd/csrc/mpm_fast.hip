@@ -5,21 +5,21 @@
 //   * exclusive scan (three-kernel, wave-shuffle based);
 //   * stable LSD radix sort of 32-bit keys, 8-bit digits, ballot-matched
 //     in-wave ranking;
-//   * per-group sort of particles by the 4x4x4 block of their base node,
-//     payload permutation of every SoA plane, and a tile list (block,
-//     first particle, count) built from per-block counts;
-//   * P2G over tiles: each workgroup scatters its tile's particles into an
-//     8^3-node LDS arena with LDS atomics and flushes each touched node once
-//     with global atomics.
+//   * per-group sort of particles by (4x4x4 block, cell) of their base
+//     node, payload permutation of every SoA plane, and one tile (block,
+//     first particle, count, staging slot) per non-empty block;
+//   * P2G over tiles without atomics: one wave per block scatters into an
+//     8^3-node LDS arena by plain read-modify-write, lane l owning cell l,
+//     and writes the arena to a staging slot that the grid update gathers.
+// (On gfx950 an LDS float atomic runs ~20x slower than a plain LDS
+// read-modify-write, and global float atomics at ~77 G/s would dominate a
+// per-node flush.)
 #include "mpm_kernels.h"
 #include "mpm_launch.h"
 
 namespace mpm {
 
 PartView view_of(mpm_sim *s, int g);
-
-constexpr int ARENA = 8;                       // arena nodes per axis: block (4) + 1 below + 3 above
-constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
 
 // ---------------------------------------------------------------------------
 // Scan helpers.
@@ -221,11 +221,15 @@ static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *
 
 // ---------------------------------------------------------------------------
 // Keys, permutation and tiles.
-__global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *vals, uint32_t *blk_count)
+//
+// Sort key: (linear index of the 4^3-node block holding the particle's base
+// node) << 6 | (cell of the base inside that block).  A stable sort leaves
+// every block's particles grouped by cell, which P2G uses to give each lane
+// of a wave its own cell.
+constexpr int KEY_LOW = 6;
+
+__device__ __forceinline__ void base_of(const GridP &g, const PartView &pv, int i, int b[3])
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pv.n) return;
-    int b[3];
     const float o[3] = {g.ox, g.oy, g.oz};
     const int nn[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
@@ -233,16 +237,26 @@ __global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *val
         int v = base_index_f32(ld(pv, PL_X + a, i), o[a], g.inv_h);
         b[a] = v > nn[a] - 3 ? nn[a] - 3 : v;
     }
-    uint32_t key = ((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
-                   (uint32_t)(b[0] >> 2);
-    keys[i] = key;
-    vals[i] = (uint32_t)i;
-    atomicAdd(&blk_count[key], 1u);
 }
 
+__global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *vals)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pv.n) return;
+    int b[3];
+    base_of(g, pv, i, b);
+    uint32_t blk = ((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
+                   (uint32_t)(b[0] >> 2);
+    uint32_t cell = (uint32_t)(((b[2] & 3) * BLK + (b[1] & 3)) * BLK + (b[0] & 3));
+    keys[i] = (blk << KEY_LOW) | cell;
+    vals[i] = (uint32_t)i;
+}
+
+// Gathers every permuted SoA plane, the particle ids and the sort-time cell.
 __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, int64_t cap, int64_t off, int n,
-                          const uint32_t *__restrict__ perm, const uint32_t *__restrict__ pid,
-                          uint32_t *__restrict__ pida, uint32_t plane_mask)
+                          const uint32_t *__restrict__ perm, const uint32_t *__restrict__ skeys,
+                          const uint32_t *__restrict__ pid, uint32_t *__restrict__ pida, uint8_t *__restrict__ cellkey,
+                          uint32_t plane_mask)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -251,104 +265,194 @@ __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, i
     for (int p = 0; p < PL_COUNT; ++p)
         if ((plane_mask >> p) & 1u) Pa[(int64_t)p * cap + off + j] = P[(int64_t)p * cap + off + src];
     pida[off + j] = pid[off + src];
+    cellkey[off + j] = (uint8_t)(skeys[j] & ((1u << KEY_LOW) - 1u));
 }
 
-__global__ void k_tiles_per_block(const uint32_t *cnt, uint32_t *ntile, int nblocks, int tile)
+// First and one-past-last sorted position of every block (untouched blocks
+// keep the zeros written before).
+__global__ void k_block_runs(const uint32_t *__restrict__ skeys, int n, uint32_t *blk_start, uint32_t *blk_end)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = skeys[i] >> KEY_LOW;
+    if (i == 0 || (skeys[i - 1] >> KEY_LOW) != b) blk_start[b] = (uint32_t)i;
+    if (i == n - 1 || (skeys[i + 1] >> KEY_LOW) != b) blk_end[b] = (uint32_t)(i + 1);
+}
+
+__global__ void k_tile_flags(const uint32_t *blk_start, const uint32_t *blk_end, uint32_t *has, int nblocks)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nblocks) ntile[b] = (cnt[b] + (uint32_t)tile - 1u) / (uint32_t)tile;
+    if (b < nblocks) has[b] = blk_end[b] > blk_start[b] ? 1u : 0u;
 }
 
-__global__ void k_emit_tiles(const uint32_t *cnt, const uint32_t *blk_start, const uint32_t *tile_start,
-                             int nblocks, int tile, uint4 *desc, uint32_t desc_cap)
+// One tile per non-empty block: (first particle, count, block, tile id).
+// blk_tile maps the block to its global tile id (staging slot).
+__global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end, const uint32_t *tile_index,
+                             int nblocks, uint4 *desc, uint32_t desc_cap, int32_t *blk_tile, uint32_t tile_base)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblocks) return;
-    uint32_t c = cnt[b];
-    uint32_t t0 = tile_start[b], s0 = blk_start[b];
-    for (uint32_t k = 0; k * (uint32_t)tile < c && t0 + k < desc_cap; ++k) {
-        uint32_t m = c - k * (uint32_t)tile;
-        desc[t0 + k] = make_uint4(s0 + k * (uint32_t)tile, m < (uint32_t)tile ? m : (uint32_t)tile, (uint32_t)b, 0u);
+    uint32_t c = blk_end[b] - blk_start[b];
+    uint32_t t = tile_index[b];
+    if (c > 0 && t < desc_cap) {
+        desc[t] = make_uint4(blk_start[b], c, (uint32_t)b, tile_base + t);
+        blk_tile[b] = (int32_t)(tile_base + t);
+    } else {
+        blk_tile[b] = -1;
     }
 }
 
 // ---------------------------------------------------------------------------
-// P2G over tiles with an LDS arena (persistent over the tile list).
-template <int MAT>
-__global__ void __launch_bounds__(256) k_p2g_tile(GridP g, MatP mat, PartView pv, const uint4 *__restrict__ desc,
-                                                  const uint32_t *__restrict__ tile_count, uint32_t desc_cap, float dt, float4 *grid,
-                                                  uint32_t *flags, uint32_t *counters)
+// P2G over blocks without atomics.
+//
+// One wave owns one block's particles.  Lane l takes cell l of the block;
+// in round r it scatters the r-th particle of that cell, so the lanes of a
+// round normally have 64 different base nodes and the 27 stencil updates of
+// one instruction touch 64 different LDS slots: a plain read-modify-write is
+// race-free.  Updates of different stencil offsets can alias across lanes,
+// so they are issued in program order (compiler barrier between offsets; LDS
+// executes a wave's operations in order).  Particles that drifted since the
+// sort may share a base: lanes are matched on their base and same-base lanes
+// take turns.  The arena (8^3 nodes: the block's 4 + 1 below + 3 above per
+// axis) is written to the tile's staging slot with plain stores; the grid
+// update gathers the staging slots overlapping each node block.  Particles
+// outside the arena fall back to global atomics on grid_mp.
+constexpr int A_SY = 12, A_SZ = 100;          // LDS slot strides (float4 slots): spread 8-lane groups
+constexpr int A_SLOTS = 7 + 7 * A_SY + 7 * A_SZ + 1;
+
+__device__ __forceinline__ int a_slot(int x, int y, int z) { return x + A_SY * y + A_SZ * z; }
+
+__device__ __forceinline__ uint64_t match_key9(uint32_t key)
 {
-    __shared__ float arena[4 * ARENA_NODES];
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        bool bit = (key >> b) & 1u;
+        uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        uint32_t o = __shfl_xor(v, d);
+        v = v > o ? v : o;
+    }
+    return v;
+}
+
+template <int MAT>
+__global__ void __launch_bounds__(64) k_p2g_block(GridP g, MatP mat, PartView pv, const uint8_t *__restrict__ cellkey,
+                                                  const uint4 *__restrict__ desc, const uint32_t *__restrict__ tile_count,
+                                                  uint32_t desc_cap, uint32_t staging_cap, float dt,
+                                                  float4 *__restrict__ staging, float4 *grid_spill, uint32_t *flags,
+                                                  uint32_t *counters)
+{
+    __shared__ float4 arena[A_SLOTS];
+    __shared__ uint32_t cstart[64], cend[64];
+    const int lane = threadIdx.x;
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const uint4 d = desc[t];
-        for (int k = threadIdx.x; k < 4 * ARENA_NODES; k += blockDim.x) arena[k] = 0.f;
+        for (int k = lane; k < A_SLOTS; k += 64) arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        cstart[lane] = 0u;
+        cend[lane] = 0u;
         __syncthreads();
+        // cell runs of this block (particles are sorted by cell)
+        for (uint32_t p = lane; p < d.y; p += 64) {
+            uint32_t c = cellkey[pv.off + d.x + p];
+            uint32_t cp = p > 0 ? cellkey[pv.off + d.x + p - 1] : 0xffu;
+            uint32_t cn = p + 1 < d.y ? cellkey[pv.off + d.x + p + 1] : 0xffu;
+            if (c != cp) cstart[c & 63] = p;
+            if (c != cn) cend[c & 63] = p + 1;
+        }
+        __syncthreads();
+        const uint32_t my0 = min(cstart[lane], d.y);
+        const uint32_t my1 = min(cend[lane], d.y);
+        const uint32_t myn = my1 > my0 ? my1 - my0 : 0u;
+        const uint32_t rounds = wave_max_u32(myn);
         const int bx = (int)(d.z % (uint32_t)g.nbx);
         const int by = (int)((d.z / (uint32_t)g.nbx) % (uint32_t)g.nby);
         const int bz = (int)(d.z / ((uint32_t)g.nbx * (uint32_t)g.nby));
         const int ax = bx * BLK - 1, ay = by * BLK - 1, az = bz * BLK - 1;
-        for (uint32_t p = threadIdx.x; p < d.y; p += blockDim.x) {
-            const int i = (int)(d.x + p);
-            float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+        const bool staged = d.w < staging_cap;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const bool active = r < myn;
             Stencil s;
-            bits |= make_stencil(g, x, s);
             Scatter sc;
-            load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
-            const int lx = s.b[0] - ax, ly = s.b[1] - ay, lz = s.b[2] - az;
-            const bool inside = lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
-                                lz <= ARENA - 3;
-            if (inside) {
+            bool inside = false;
+            uint32_t key = 511u;
+            if (active) {
+                const int i = (int)(d.x + my0 + r);
+                float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+                bits |= make_stencil(g, x, s);
+                load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+                const int lx = s.b[0] - ax, ly = s.b[1] - ay, lz = s.b[2] - az;
+                inside = staged && lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
+                         lz <= ARENA - 3;
+                if (inside) key = (uint32_t)(lx + 6 * (ly + 6 * lz));
+                else {
+                    bits |= MPM_H_ARENA_SPILL;
 #pragma unroll
-                for (int oz = 0; oz < 3; ++oz)
+                    for (int oz = 0; oz < 3; ++oz)
 #pragma unroll
-                    for (int oy = 0; oy < 3; ++oy)
+                        for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
-                        for (int ox = 0; ox < 3; ++ox) {
-                            float4 c = node_contrib(g, s, sc, ox, oy, oz);
-                            int a = ((lz + oz) * ARENA + (ly + oy)) * ARENA + (lx + ox);
-                            atomicAdd(&arena[a], c.x);
-                            atomicAdd(&arena[ARENA_NODES + a], c.y);
-                            atomicAdd(&arena[2 * ARENA_NODES + a], c.z);
-                            atomicAdd(&arena[3 * ARENA_NODES + a], c.w);
-                        }
-            } else {
-                bits |= MPM_H_ARENA_SPILL;
+                            for (int ox = 0; ox < 3; ++ox) {
+                                float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                                float *dst = (float *)(grid_spill + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
+                                unsafeAtomicAdd(dst + 0, c.x);
+                                unsafeAtomicAdd(dst + 1, c.y);
+                                unsafeAtomicAdd(dst + 2, c.z);
+                                unsafeAtomicAdd(dst + 3, c.w);
+                            }
+                    int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2, by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
+                    int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
+                    for (int zz = bz0; zz <= bz1; ++zz)
+                        for (int yy = by0; yy <= by1; ++yy)
+                            for (int xx = bx0; xx <= bx1; ++xx)
+                                flags[((int64_t)zz * g.nby + yy) * g.nbx + xx] = 1u;
+                }
+            }
+            uint64_t pending = __ballot(inside);
+            const uint64_t same = match_key9(key);
+            while (pending) {
+                const uint64_t grp = same & pending;
+                const bool winner = inside && ((pending >> lane) & 1ull) && (__ffsll((unsigned long long)grp) - 1 == lane);
+                if (winner) {
+                    const int lx = s.b[0] - ax, ly = s.b[1] - ay, lz = s.b[2] - az;
 #pragma unroll
-                for (int oz = 0; oz < 3; ++oz)
+                    for (int oz = 0; oz < 3; ++oz)
 #pragma unroll
-                    for (int oy = 0; oy < 3; ++oy)
+                        for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
-                        for (int ox = 0; ox < 3; ++ox) {
-                            float4 c = node_contrib(g, s, sc, ox, oy, oz);
-                            float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
-                            unsafeAtomicAdd(dst + 0, c.x);
-                            unsafeAtomicAdd(dst + 1, c.y);
-                            unsafeAtomicAdd(dst + 2, c.z);
-                            unsafeAtomicAdd(dst + 3, c.w);
-                        }
-                int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2, by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
-                int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
-                for (int zz = bz0; zz <= bz1; ++zz)
-                    for (int yy = by0; yy <= by1; ++yy)
-                        for (int xx = bx0; xx <= bx1; ++xx) flags[((int64_t)zz * g.nby + yy) * g.nbx + xx] = 1u;
+                            for (int ox = 0; ox < 3; ++ox) {
+                                float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                                const int a = a_slot(lx + ox, ly + oy, lz + oz);
+                                float4 v = arena[a];
+                                v.x += c.x; v.y += c.y; v.z += c.z; v.w += c.w;
+                                arena[a] = v;
+                                __asm__ volatile("" ::: "memory");
+                            }
+                }
+                pending &= ~__ballot(winner);
             }
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < ARENA_NODES; k += blockDim.x) {
-            float m = arena[k];
-            if (m != 0.f) {
-                int nx_ = ax + (k & 7), ny_ = ay + ((k >> 3) & 7), nz_ = az + (k >> 6);
-                if (nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz) {
-                    float *dst = (float *)(grid + node_slot(g, nx_, ny_, nz_));
-                    unsafeAtomicAdd(dst + 0, m);
-                    unsafeAtomicAdd(dst + 1, arena[ARENA_NODES + k]);
-                    unsafeAtomicAdd(dst + 2, arena[2 * ARENA_NODES + k]);
-                    unsafeAtomicAdd(dst + 3, arena[3 * ARENA_NODES + k]);
+        // write the arena back: staging slot d.w, logical node order, and mark touched node blocks
+        if (staged) {
+            float4 *dst = staging + (int64_t)d.w * ARENA_NODES;
+            for (int k = lane; k < ARENA_NODES; k += 64) {
+                const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
+                const float4 v = arena[a_slot(kx, ky, kz)];
+                dst[k] = v;
+                const int nx_ = ax + kx, ny_ = ay + ky, nz_ = az + kz;
+                if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
                     flags[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
-                }
             }
         }
         __syncthreads();
@@ -361,15 +465,11 @@ __global__ void __launch_bounds__(256) k_p2g_tile(GridP g, MatP mat, PartView pv
 // Host side.
 static inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-static int tile_size(const mpm_sim *s) { return s->cfg.tile > 0 ? s->cfg.tile : 512; }
-
-static int64_t tile_bound(const mpm_sim *s, int64_t n)
-{
-    return std::min<int64_t>(s->nblocks, n) + n / tile_size(s) + 1;
-}
+static int64_t tile_bound(const mpm_sim *s, int64_t n) { return std::min<int64_t>(s->nblocks, n); }
 
 static int ensure_fast_buffers(mpm_sim *s)
 {
+    if (s->groups.size() > 256) return set_error(MPM_ERR_INVALID, "at most 256 particle groups");
     int64_t nmax = 0;
     for (auto &gi : s->groups) nmax = std::max(nmax, gi.n);
     if (!s->keys || s->sort_hist_len < nmax) {
@@ -392,6 +492,7 @@ static int ensure_fast_buffers(mpm_sim *s)
         MPM_HIP(hipMalloc(&s->blk_start, s->nblocks * 4));
         MPM_HIP(hipMalloc(&s->scan_tmp, s->nblocks * 4));
     }
+    if (!s->cellkey) MPM_HIP(hipMalloc(&s->cellkey, s->cap));
     int64_t need = 0;
     s->group_tile_off.assign(s->groups.size(), 0);
     for (size_t k = 0; k < s->groups.size(); ++k) {
@@ -400,11 +501,18 @@ static int ensure_fast_buffers(mpm_sim *s)
     }
     if (s->tile_cap < need) {
         if (s->tile_desc) (void)hipFree(s->tile_desc);
+        if (s->staging) (void)hipFree(s->staging);
+        if (s->blk_tile) (void)hipFree(s->blk_tile);
+        s->tile_desc = nullptr; s->staging = nullptr; s->blk_tile = nullptr;
         MPM_HIP(hipMalloc(&s->tile_desc, need * sizeof(uint4)));
+        // every non-empty block may need a staging slot; blocks beyond the cap scatter with atomics
+        int64_t scap = std::min<int64_t>(need, (int64_t)1 << 19);
+        MPM_HIP(hipMalloc(&s->staging, scap * ARENA_NODES * sizeof(float4)));
+        MPM_HIP(hipMalloc(&s->blk_tile, (int64_t)s->groups.size() * s->nblocks * sizeof(int32_t)));
+        s->staging_cap = scap;
         s->tile_cap = need;
     }
     if (!s->tile_counts) MPM_HIP(hipMalloc(&s->tile_counts, 256 * 4));
-    if (s->groups.size() > 256) return set_error(MPM_ERR_INVALID, "at most 256 particle groups");
     return MPM_OK;
 }
 
@@ -413,35 +521,35 @@ static int sort_all(mpm_sim *s)
     int rc = ensure_fast_buffers(s);
     if (rc) return rc;
     hipStream_t st = s->stream;
-    int bits = 1;
-    while ((1ll << bits) < s->nblocks) ++bits;
+    int bits = KEY_LOW + 1;
+    while ((1ll << (bits - KEY_LOW)) < s->nblocks) ++bits;
     uint32_t plane_mask = (1u << PL_COUNT) - 1u;
     if (!s->tau_valid) plane_mask &= ~(((1u << 6) - 1u) << PL_TAU);
-    const int T = tile_size(s);
+    const int nb = (int)s->nblocks;
     for (size_t k = 0; k < s->groups.size(); ++k) {
         PartView pv = view_of(s, (int)k);
         int n = pv.n;
-        if ((rc = launch_zero_u32(st, s->blk_count, (int)s->nblocks))) return hip_fail((hipError_t)rc, "zero");
-        hipLaunchKernelGGL(k_block_keys, dim3(nblk(n, 256)), dim3(256), 0, st, s->g, pv, s->keys, s->vals,
-                           s->blk_count);
+        uint32_t *blk_end = s->blk_count;
+        if ((rc = launch_zero_u32(st, s->blk_start, nb)) || (rc = launch_zero_u32(st, blk_end, nb)))
+            return hip_fail((hipError_t)rc, "zero");
+        hipLaunchKernelGGL(k_block_keys, dim3(nblk(n, 256)), dim3(256), 0, st, s->g, pv, s->keys, s->vals);
         bool in_alt = false;
         if ((rc = radix_sort(st, s->keys, s->vals, s->keys_alt, s->vals_alt, n, bits, s->sort_hist, s->sort_scan,
                              in_alt)))
             return hip_fail((hipError_t)rc, "radix_sort");
         const uint32_t *perm = in_alt ? s->vals_alt : s->vals;
+        const uint32_t *skeys = in_alt ? s->keys_alt : s->keys;
         hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, s->P, s->P_alt, s->cap, pv.off, n, perm,
-                           s->pid, s->pid_alt, plane_mask);
-        // tiles: particle starts per block and tile starts per block
-        int nb = (int)s->nblocks;
-        if ((rc = scan_excl(st, s->blk_count, s->blk_start, nb, s->sort_scan, nullptr)))
-            return hip_fail((hipError_t)rc, "scan");
-        hipLaunchKernelGGL(k_tiles_per_block, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_count, s->scan_tmp, nb,
-                           T);
+                           skeys, s->pid, s->pid_alt, s->cellkey, plane_mask);
+        // tiles: one per non-empty block, numbered in block order
+        hipLaunchKernelGGL(k_block_runs, dim3(nblk(n, 256)), dim3(256), 0, st, skeys, n, s->blk_start, blk_end);
+        hipLaunchKernelGGL(k_tile_flags, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_start, blk_end, s->scan_tmp,
+                           nb);
         if ((rc = scan_excl(st, s->scan_tmp, s->scan_tmp, nb, s->sort_scan, s->tile_counts + k)))
             return hip_fail((hipError_t)rc, "scan");
-        hipLaunchKernelGGL(k_emit_tiles, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_count, s->blk_start,
-                           s->scan_tmp, nb, T, (uint4 *)s->tile_desc + s->group_tile_off[k],
-                           (uint32_t)tile_bound(s, n));
+        hipLaunchKernelGGL(k_emit_tiles, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_start, blk_end, s->scan_tmp,
+                           nb, (uint4 *)s->tile_desc + s->group_tile_off[k], (uint32_t)tile_bound(s, n),
+                           s->blk_tile + (int64_t)k * nb, (uint32_t)s->group_tile_off[k]);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "sort kernels");
     }
@@ -476,18 +584,19 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     if (pv.n <= 0) return MPM_OK;
     const MatP &m = s->groups[group].mat;
     int64_t bound = tile_bound(s, pv.n);
-    int wg = (int)std::min<int64_t>(bound, 2048);
+    int wg = (int)std::min<int64_t>(bound, 8192);
     const uint4 *desc = (const uint4 *)s->tile_desc + s->group_tile_off[group];
     const uint32_t *tc = s->tile_counts + group;
     const uint32_t cap = (uint32_t)bound;
-    dim3 gr(wg), bl(256);
+    const uint32_t scap = (uint32_t)s->staging_cap;
+    dim3 gr(wg), bl(64);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_tile<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, dt, s->grid_mp, s->flags, s->counters); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_tile<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, dt, s->grid_mp, s->flags, s->counters); break;
-    default: hipLaunchKernelGGL(k_p2g_tile<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, dt, s->grid_mp, s->flags, s->counters); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->grid_mp, s->flags, s->counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->grid_mp, s->flags, s->counters); break;
+    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->grid_mp, s->flags, s->counters); break;
     }
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? MPM_OK : hip_fail(e, "p2g_tile");
+    return e == hipSuccess ? MPM_OK : hip_fail(e, "p2g_block");
 }
 
 // Stand-alone sort for the probe entry point (own temporaries).

@@ -32,6 +32,10 @@ enum Plane : int {
 // (velocity, mass) after the grid update.
 constexpr int BLK = 4;
 constexpr int BLK_NODES = 64;
+// FAST-form P2G arena of a block: nodes [4B-1, 4B+6] per axis (the block's
+// own 4, one below and three above), staged in logical (z, y, x) order.
+constexpr int ARENA = 8;
+constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
 
 struct GridP {
     int nx, ny, nz;
@@ -105,6 +109,10 @@ struct mpm_sim {
     int64_t tile_cap = 0;
     std::vector<int64_t> group_tile_off;    // per group: first tile slot
     uint32_t *tile_counts = nullptr;        // per group: tiles built by the last sort
+    uint8_t *cellkey = nullptr;             // per particle: cell of its base node inside its block at sort time
+    float4 *staging = nullptr;              // per tile: 8^3-node P2G arena written back without atomics
+    int64_t staging_cap = 0;                // tiles the staging buffer holds
+    int32_t *blk_tile = nullptr;            // [group][block]: staging tile of the block, -1 if none
     bool sorted_valid = false;
     int64_t steps_since_sort = 0;
     int last_phase = -1;

@@ -11,7 +11,8 @@ int launch_p2g_atomic(hipStream_t st, const GridP &g, const MatP &m, const PartV
                       int threads);
 int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters);
 int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
-                       uint32_t *counters, float4 *gmp, float4 *gv, int nblocks);
+                       uint32_t *counters, float4 *gmp, float4 *gv, int nblocks, const float4 *staging,
+                       const int32_t *blk_tile, int ngroups, uint32_t staging_cap);
 int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads);
 int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);

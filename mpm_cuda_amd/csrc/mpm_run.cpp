@@ -1,0 +1,112 @@
+// mpm_run.cpp -- command-line driver: the simulation loop of the reference
+// application without its viewer (src/main.cpp:132-221): build the scene,
+// step until the simulated time passes --time, and write
+// <out>/FrameNNNNNN.bgeo whenever int(t / (1/fps)) reaches the next frame
+// number (main.cpp:28-29, 214-221).
+//
+//   mpm_run --scene snow|jello|water|obj [--obj path] [--time 1.0] [--fps 36]
+//           [--out Animations] [--form fast|plain] [--grid N]
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/mpm/MPMSimulator.hpp"
+
+using namespace mpm;
+
+static const char *arg(int argc, char **argv, const char *name, const char *def)
+{
+    for (int i = 1; i + 1 < argc; ++i)
+        if (std::string(argv[i]) == name) return argv[i + 1];
+    return def;
+}
+
+int main(int argc, char **argv)
+{
+    std::string scene = arg(argc, argv, "--scene", "snow");
+    double t_end = atof(arg(argc, argv, "--time", "0.05"));
+    int fps = atoi(arg(argc, argv, "--fps", "36"));
+    std::string out = arg(argc, argv, "--out", "Animations");
+    std::string form = arg(argc, argv, "--form", "fast");
+    int n = atoi(arg(argc, argv, "--grid", "128"));
+    if (fps <= 0 || n < 16) { fprintf(stderr, "bad --fps or --grid\n"); return 2; }
+    mkdir(out.c_str(), 0755);
+
+    mpm_config cfg;
+    mpm_config_default(&cfg);
+    cfg.form = form == "plain" ? MPM_FORM_PLAIN : MPM_FORM_FAST;
+    double h = 1.0 / n, dt = 1e-4;
+    unsigned ppc = 8;
+    std::vector<ObjInitInfo> infos;
+    try {
+        if (scene == "snow") {
+            for (int s = 0; s < 2; ++s) {
+                ObjInitInfo o;
+                o.shape = MPM_SHAPE_SPHERE;
+                o.shapeParam = {s ? 0.7 : 0.3, 0.5, 0.5, 0.15, 0, 0};
+                o.initVel = {s ? -3.0 : 3.0, 0.0, 0.0};
+                o.mMaterial = Material(1.4e5, 0.2, 400.0, SNOW);
+                o.mMaterial.c.theta_c = 2.5e-2; o.mMaterial.c.theta_s = 7.5e-3;
+                infos.push_back(o);
+            }
+            ppc = 5;
+        } else if (scene == "jello") {
+            ObjInitInfo o;
+            o.shape = MPM_SHAPE_BOX;
+            o.shapeParam = {0.375, 0.1, 0.375, 0.625, 0.35, 0.625};
+            o.initVel = {0.0, -2.0, 0.0};
+            o.mMaterial = Material(5e4, 0.3, 1e3, JELLO);
+            infos.push_back(o);
+            cfg.bc_mode = MPM_BC_STICKY;
+        } else if (scene == "water") {
+            ObjInitInfo o;
+            o.shape = MPM_SHAPE_BOX;
+            double p = 11.0 / n;
+            o.shapeParam = {p, p, p, p + 0.5, p + 0.4, p + 0.5};
+            o.mMaterial = Material(1e4, 0.2, 1e3, WATER);
+            o.mMaterial.c.eos_k = 1e4;
+            o.mMaterial.c.water_c_mode = MPM_WATER_C_GRADV;
+            infos.push_back(o);
+            dt = 5e-5;
+            ppc = 2;
+        } else if (scene == "obj") {
+            ObjInitInfo o;
+            o.objPath = arg(argc, argv, "--obj", "");
+            o.initScale = vec3(0.2f);
+            o.initTranslation = vec3(0.5f, 0.5f, 0.5f);
+            o.mMaterial = Material(1e4, 0.4, 800.0, JELLO);
+            infos.push_back(o);
+        } else {
+            fprintf(stderr, "unknown --scene %s\n", scene.c_str());
+            return 2;
+        }
+        MPMSimulator sim(h, dt, n, n, n, ppc, infos, &cfg);
+        printf("particles: %d\n", sim.totalParticlesNum());
+        const float time_per_frame = 1.f / (float)fps;
+        int frame = 0;
+        std::vector<float> pos;
+        auto t0 = std::chrono::steady_clock::now();
+        while (sim.current_time <= t_end) {
+            sim.step();
+            if ((int)(sim.current_time / time_per_frame) >= frame) {
+                sim.getGLParticlesPos(pos);
+                std::stringstream ss;
+                ss << out << "/Frame" << std::setw(6) << std::setfill('0') << frame << ".bgeo";
+                OutputBego(ss.str(), pos);
+                ++frame;
+            }
+        }
+        double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        printf("steps: %d  frames: %d  wall: %.3f s  particle-steps/s: %.3e\n", sim.current_frame, frame, wall,
+               (double)sim.totalParticlesNum() * sim.current_frame / wall);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

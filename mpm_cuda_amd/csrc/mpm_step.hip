@@ -62,43 +62,79 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(GridP g, MatP mat, PartView 
 
 // ---------------------------------------------------------------------------
 // Active-block list: compacts the block flags written by P2G (and clears them).
-__global__ void k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters)
+// One global atomic per workgroup (256 flags): wave counts are combined in LDS.
+__global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list,
+                                                        uint32_t *counters)
 {
+    __shared__ uint32_t wave_cnt[4], wg_base;
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     bool f = b < nblocks && flags[b] != 0u;
     if (f) flags[b] = 0u;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint64_t mask = __ballot(f);
-    if (mask == 0) return;
-    int lane = threadIdx.x & 63;
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counters + CT_ACTIVE, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
+    if (lane == 0) wave_cnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        wg_base = tot ? atomicAdd(counters + CT_ACTIVE, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = wg_base;
+    for (int k = 0; k < w; ++k) off += wave_cnt[k];
     uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
-    if (f) list[base + __popcll(below)] = (uint32_t)b;
+    if (f) list[off + __popcll(below)] = (uint32_t)b;
 }
 
 // ---------------------------------------------------------------------------
 // Grid velocity update over active blocks (MPMCUDA.cu:435-472): v = p / m,
-// gravity, walls.  Reads grid_mp, writes grid_v and clears grid_mp for the
-// next P2G.  One wave per block, persistent over the active list.
-__global__ void __launch_bounds__(256) k_grid_update(GridP g, float dt, const uint32_t *list,
-                                                     uint32_t *counters, float4 *gmp,
-                                                     float4 *gv)
+// gravity, walls.  One wave per active block, persistent over the active
+// list.  Node (mass, momentum) is the sum of grid_mp (PLAIN-form scatter and
+// FAST-form spills) and, in the FAST form, of the staged P2G arenas of the
+// up to 2x2x2 particle blocks whose arenas cover the node.  grid_mp is
+// cleared for the next step; the result goes to grid_v.
+__global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uint32_t *list, uint32_t *counters,
+                                                    float4 *gmp, float4 *gv, const float4 *__restrict__ staging,
+                                                    const int32_t *__restrict__ blk_tile, int ngroups,
+                                                    uint32_t staging_cap, int64_t nblocks)
 {
     const uint32_t nact = counters[CT_ACTIVE];
-    const int lane = threadIdx.x & 63;
-    for (uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6); k < nact; k += gridDim.x * 4) {
-        uint32_t b = list[k];
-        int64_t idx = (int64_t)b * BLK_NODES + lane;
+    const int lane = threadIdx.x;
+    const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
+    uint32_t nodes = 0;     // nodes with mass seen by this wave
+    for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
+        const uint32_t b = __builtin_amdgcn_readfirstlane(list[k]);
+        const int bx = (int)(b % (uint32_t)g.nbx);
+        const int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
+        const int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
+        const int64_t idx = (int64_t)b * BLK_NODES + lane;
         float4 a = gmp[idx];
         gmp[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (blk_tile) {
+            for (int gi = 0; gi < ngroups; ++gi) {
+                const int32_t *bt = blk_tile + (int64_t)gi * nblocks;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int pz = bz + dz, az = kz + 1 - 4 * dz;
+                    if (pz < 0 || pz >= g.nbz) continue;
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int py = by + dy, ay = ky + 1 - 4 * dy;
+                        if (py < 0 || py >= g.nby) continue;
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int px = bx + dx, ax = kx + 1 - 4 * dx;
+                            if (px < 0 || px >= g.nbx) continue;
+                            const int32_t t = bt[((int64_t)pz * g.nby + py) * g.nbx + px];
+                            if (t < 0 || (uint32_t)t >= staging_cap) continue;
+                            if (ax >= 0 && ax < ARENA && ay >= 0 && ay < ARENA && az >= 0 && az < ARENA) {
+                                const float4 v = staging[(int64_t)t * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
+                                a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
+                            }
+                        }
+                    }
+                }
+            }
+        }
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!(fabsf(a.x) < 2.220446049250313e-16f)) {          // MPMCUDA.cu:444
-            int bx = (int)(b % (uint32_t)g.nbx);
-            int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
-            int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
-            int ix = bx * BLK + (lane & 3), iy = by * BLK + ((lane >> 2) & 3), iz = bz * BLK + (lane >> 4);
+            const int ix = bx * BLK + kx, iy = by * BLK + ky, iz = bz * BLK + kz;
             float vx = a.y / a.x + g.gx * dt;                  // MPMCUDA.cu:452-457
             float vy = a.z / a.x + g.gy * dt;
             float vz = a.w / a.x + g.gz * dt;
@@ -115,9 +151,9 @@ __global__ void __launch_bounds__(256) k_grid_update(GridP g, float dt, const ui
             out = make_float4(vx, vy, vz, a.x);
         }
         gv[idx] = out;
-        uint64_t has_mass = __ballot(out.w != 0.f);
-        if (lane == 0 && has_mass) atomicAdd(counters + CT_NODES, (uint32_t)__popcll(has_mass));
+        nodes += (uint32_t)__popcll(__ballot(out.w != 0.f));
     }
+    if (lane == 0 && nodes) atomicAdd(counters + CT_NODES, nodes);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,13 +235,17 @@ __device__ __forceinline__ uint32_t g2p_particle(const GridP &g, const MatP &mat
             sc[k] = fminf(fmaxf(sg[k], 1.0f - mat.theta_c), 1.0f + mat.theta_s);   // :658-660
             ratio *= sg[k] / sc[k];
         }
+        // Fe = U S* V^T, written as Ftr + U (S* - S) V^T: identical in exact
+        // arithmetic, and exactly Ftr when no singular value is clamped, so
+        // the FP32 SVD's rounding does not leak into F every step.
+        float ds[3] = {sc[0] - sg[0], sc[1] - sg[1], sc[2] - sg[2]};
         float Fe[9];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                Fe[3 * r + c] = U[3 * r] * sc[0] * V[3 * c] + U[3 * r + 1] * sc[1] * V[3 * c + 1] +
-                                U[3 * r + 2] * sc[2] * V[3 * c + 2];
+                Fe[3 * r + c] = Ft[3 * r + c] + (U[3 * r] * ds[0] * V[3 * c] + U[3 * r + 1] * ds[1] * V[3 * c + 1] +
+                                                 U[3 * r + 2] * ds[2] * V[3 * c + 2]);
 #pragma unroll
         for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fe[k]);
         // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
@@ -296,12 +336,13 @@ int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list,
 }
 
 int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
-                       uint32_t *counters, float4 *gmp, float4 *gv, int nblocks)
+                       uint32_t *counters, float4 *gmp, float4 *gv, int nblocks, const float4 *staging,
+                       const int32_t *blk_tile, int ngroups, uint32_t staging_cap)
 {
-    int wg = nblk(nblocks, 4);
-    if (wg > 8192) wg = 8192;
+    int wg = nblocks < 8192 ? nblocks : 8192;
     if (wg < 1) wg = 1;
-    hipLaunchKernelGGL(k_grid_update, dim3(wg), dim3(256), 0, st, g, dt, list, counters, gmp, gv);
+    hipLaunchKernelGGL(k_grid_update, dim3(wg), dim3(64), 0, st, g, dt, list, counters, gmp, gv, staging, blk_tile,
+                       ngroups, staging_cap, (int64_t)nblocks);
     return (int)hipGetLastError();
 }
 

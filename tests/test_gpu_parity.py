@@ -105,6 +105,7 @@ def test_p2g_grid_parity(gpu, case):
     p = Pair(type_=mat, form=form, seed=11)
     try:
         em, ep = _p2g_compare(p)
+        print(MAT_ID[mat], FORM_ID[form], f"grid mass {em:.2e} momentum {ep:.2e}")
         assert em < TOL_GRID and ep < TOL_GRID, (em, ep)
         # mass conservation
         assert abs(p.gpu.grid(capi.G_MASS).astype(np.float64).sum() - p.orc.groups[0].mat.mass * p.n) \
@@ -116,6 +117,7 @@ def test_p2g_grid_parity(gpu, case):
         mo = p.orc.grid_mass()[..., None]
         # mass-weighted (tiny-mass nodes amplify FP32 error in v = p/m)
         ev = rel(vg * mo, vo * mo)
+        print(MAT_ID[mat], FORM_ID[form], f"grid velocity (mass-weighted) {ev:.2e}")
         assert ev < TOL_GRID, ev
     finally:
         p.close()
@@ -129,32 +131,46 @@ def test_one_step_particle_parity(gpu, case):
         p.gpu.step(p.dt)
         assert p.orc.step(p.dt) == 0
         g, o = p.gpu_particles(), p.orc_particles()
-        for k in ("x", "v", "C", "F", "S"):
-            if mat == capi.WATER and k == "F":
-                continue
-            e = rel(g[k], o[k]) if k not in ("F", "S") else rel(g[k] - (np.eye(3) if k == "F" else 1.0),
-                                                              o[k] - (np.eye(3) if k == "F" else 1.0))
+        fields = {capi.JELLO: ("x", "v", "C", "F"), capi.SNOW: ("x", "v", "C", "F", "S"),
+                  capi.WATER: ("x", "v", "C", "S")}[mat]
+        errs = {}
+        for k in fields:
+            if k == "F":
+                errs[k] = rel(g[k] - np.eye(3), o[k] - np.eye(3))
+            elif k == "S":
+                errs[k] = rel(g[k] - 1.0, o[k] - 1.0)
+            else:
+                errs[k] = rel(g[k], o[k])
+        print(MAT_ID[mat], FORM_ID[form], {k: f"{v:.2e}" for k, v in errs.items()})
+        for k, e in errs.items():
             assert e < (TOL_PART if k != "S" else 1e-4), (k, e)
         assert p.gpu.health() & ~capi.H_ARENA_SPILL == 0
     finally:
         p.close()
 
 
-@pytest.mark.parametrize("case", CASES, ids=_ids)
+ROLLOUT = [(m, f, s) for m in MATS for f in FORMS for s in ("rest", "perturbed")]
+
+
+@pytest.mark.parametrize("case", ROLLOUT, ids=lambda c: f"{MAT_ID[c[0]]}-{FORM_ID[c[1]]}-{c[2]}")
 def test_rollout_parity(gpu, case):
-    """50 steps from the rest state with an initial velocity; tolerances grow
-    with FP32 error accumulation."""
-    mat, form = case
-    p = Pair(type_=mat, form=form, seed=5, randomize=False)
+    """50 steps, FP32 device vs FP64 oracle.  'rest': uniform initial
+    velocity, F = I (gravity only); 'perturbed': random v, C, F, J_p / J so
+    elastic / plastic / pressure dynamics run.  Bars: x within 5e-6 (FP32
+    rounding of x += dt v accumulates ~ulp/2 per step), v within 1e-4, F
+    within 1e-5 of ||F||, J_p / J within 1e-5 absolute."""
+    mat, form, state = case
+    p = Pair(type_=mat, form=form, seed=5, randomize=(state == "perturbed"))
     try:
         for _ in range(50):
             p.gpu.step(p.dt)
             assert p.orc.step(p.dt) == 0
         g, o = p.gpu_particles(), p.orc_particles()
-        assert rel(g["x"], o["x"]) < 1e-6
-        assert rel(g["v"], o["v"]) < 2e-4
-        if mat != capi.WATER:
-            assert rel(g["F"] - np.eye(3), o["F"] - np.eye(3)) < 2e-3
+        ex, ev = rel(g["x"], o["x"]), rel(g["v"], o["v"])
+        eF = rel(g["F"], o["F"]) if mat != capi.WATER else 0.0
+        eS = float(np.abs(g["S"] - o["S"]).max()) if mat != capi.JELLO else 0.0
+        print(MAT_ID[mat], FORM_ID[form], state, f"x {ex:.2e} v {ev:.2e} F {eF:.2e} S {eS:.2e}")
+        assert ex < 5e-6 and ev < 1e-4 and eF < 1e-5 and eS < 1e-5
         assert p.gpu.health() & ~capi.H_ARENA_SPILL == 0
     finally:
         p.close()
