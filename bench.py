@@ -27,19 +27,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per particle of each kernel: every persistent FP32
-# particle field counted once per read pass and once per write pass; grid
-# nodes with mass counted at 16 B (P2G flush), 12 B (G2P gather).
-# FAST form: P2G reads {x, v, C, tau(6)}; G2P reads {x, F, J_p} and writes
-# {x, v, C, F, J_p, tau}.  PLAIN form: P2G reads {x, v, C, F, J_p}; G2P reads
-# {x, F, J_p} and writes {x, v, C, F, J_p}.
-PART_BYTES = {
-    # (material, form): (p2g_read, g2p_read, g2p_write)
-    ("snow", "fast"): (84, 52, 124), ("snow", "plain"): (100, 52, 100),
-    ("jello", "fast"): (84, 48, 120), ("jello", "plain"): (96, 48, 96),
-    ("water", "fast"): (64, 16, 64), ("water", "plain"): (64, 16, 64),
+# Algorithmic bytes of each kernel: every persistent FP32 particle field
+# counted once per read pass and once per write pass, and per grid node with
+# mass 16 B for the P2G write-back and 12 B for the G2P velocity gather.
+#   PLAIN: P2G reads {x, v, C, F, J_p}; G2P reads {x, F, J_p}, writes {x, v, C, F, J_p}.
+#   FAST:  P2G reads {x, v, C, tau(6)}; G2P reads {x, F, J_p}, writes {x, v, C, F, J_p, tau}.
+#   FUSED: one kernel reads {x, F, J_p}, writes {x, v, C, F, J_p}, gathers and scatters the grid.
+# (jello has no J_p; water has J instead of F.)
+KERNEL_BYTES = {
+    # (material, form): {timed phase: (bytes per particle, bytes per node with mass)}
+    ("snow", "plain"): {"p2g": (100, 16), "g2p": (152, 12)},
+    ("snow", "fast"): {"p2g": (84, 16), "g2p": (176, 12)},
+    ("snow", "fused"): {"g2p": (152, 28)},
+    ("jello", "plain"): {"p2g": (96, 16), "g2p": (144, 12)},
+    ("jello", "fast"): {"p2g": (84, 16), "g2p": (168, 12)},
+    ("jello", "fused"): {"g2p": (144, 28)},
+    ("water", "plain"): {"p2g": (64, 16), "g2p": (80, 12)},
+    ("water", "fast"): {"p2g": (64, 16), "g2p": (80, 12)},
+    ("water", "fused"): {"g2p": (80, 28)},
 }
-NODE_BYTES = {"p2g": 16, "g2p": 12, "grid": 48}
+KERNEL_NAME = {("plain", "p2g"): "k_p2g_atomic", ("plain", "g2p"): "k_g2p", ("fast", "p2g"): "k_p2g_block",
+               ("fast", "g2p"): "k_g2p", ("fused", "g2p"): "k_g2p2g_block"}
 
 
 def parse():
@@ -48,7 +56,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="snow_spheres", choices=["snow_spheres", "jello_cube", "water_dam"])
-    ap.add_argument("--form", default="fast", choices=["fast", "plain"])
+    ap.add_argument("--form", default="fused", choices=["fused", "fast", "plain"])
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -64,7 +72,7 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, timing=True):
         kw["sort_every"] = sort_every
     if tile:
         kw["tile"] = tile
-    f = capi.FORM_FAST if form == "fast" else capi.FORM_PLAIN
+    f = {"fused": capi.FORM_FUSED, "fast": capi.FORM_FAST, "plain": capi.FORM_PLAIN}[form]
     sim, info = workloads.WORKLOADS[workload](form=f, **kw)
     n = sim.total
     dt = info["cfg"].dt
@@ -88,19 +96,17 @@ def roofline(res, material, form, steps):
     n = res["n"]
     ph = res["phases"]
     nodes = max(ph["nodes"], 1)
-    p2g_r, g2p_r, g2p_w = PART_BYTES[(material, form)]
-    bytes_per = {
-        "p2g": p2g_r * n + NODE_BYTES["p2g"] * nodes,
-        "g2p": (g2p_r + g2p_w) * n + NODE_BYTES["g2p"] * nodes,
-    }
-    avg = {k: ph[k][0] / max(ph[k][1], 1) for k in ("sort", "p2g", "grid", "g2p")}   # ms per launch
-    dom = max(("p2g", "g2p"), key=lambda k: avg[k])
+    model = KERNEL_BYTES[(material, form)]
+    avg = {k: ph[k][0] / max(ph[k][1], 1) for k in ("sort", "p2g", "grid", "g2p")}   # ms per step-phase
+    bytes_per = {k: pb * n + nb * nodes for k, (pb, nb) in model.items()}
+    dom = max(model, key=lambda k: avg[k])
     achieved = bytes_per[dom] / (avg[dom] * 1e-3) / 1e9
-    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_launch_ms": {k: round(v, 4) for k, v in avg.items()},
+    return {"kernel": KERNEL_NAME[(form, dom)], "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_phase_ms": {k: round(v, 4) for k, v in avg.items()},
             "bytes_per_launch": bytes_per[dom], "nodes_with_mass": ph["nodes"],
-            "other": {k: round(bytes_per[k] / (avg[k] * 1e-3) / 1e9, 1) for k in bytes_per if avg[k] > 0}}
+            "per_kernel_GBs": {KERNEL_NAME[(form, k)]: round(bytes_per[k] / (avg[k] * 1e-3) / 1e9, 1)
+                               for k in bytes_per if avg[k] > 0}}
 
 
 def cpu_baseline(workload, steps):
@@ -154,13 +160,18 @@ def main():
         "health_bits": res["health"],
         "fp64_svd_fallbacks": res["stats"]["fp64_fallbacks"],
     }
-    if not a.no_variants and a.form == "fast":
+    if not a.no_variants:
         ksteps = max(10, a.steps // 4)
-        pl = run_gpu(a.workload, "plain", ksteps, min(a.warmup, 5))
-        out["variants"] = {"plain": {"value": round(pl["n"] * ksteps / pl["seconds"], 1),
-                                     "ms_per_step": round(pl["seconds"] / ksteps * 1e3, 4),
-                                     "roofline": roofline(pl, material, "plain", ksteps)}}
-        out["variants"]["fast_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
+        out["variants"] = {}
+        for form in ("fast", "plain"):
+            if form == a.form:
+                continue
+            r = run_gpu(a.workload, form, ksteps, min(a.warmup, 5))
+            v = r["n"] * ksteps / r["seconds"]
+            out["variants"][form] = {"value": round(v, 1), "ms_per_step": round(r["seconds"] / ksteps * 1e3, 4),
+                                     "roofline": roofline(r, material, form, ksteps)}
+        if "plain" in out["variants"]:
+            out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_steps)
     print(json.dumps(out), flush=True)

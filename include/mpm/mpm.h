@@ -47,9 +47,14 @@ enum { MPM_WATER_C_APIC = 0, MPM_WATER_C_GRADV = 1 };
  *   stress SVD runs in P2G and (snow) again in G2P -- the structure of
  *   MPMCUDA.cu:310-741 in FP32 SoA.
  * MPM_FORM_FAST: particles cell-block sorted every `sort_every` steps, P2G
- *   accumulates per block in LDS and flushes once per node, G2P is fused with
- *   the stress evaluation so each step needs one SVD. */
-enum { MPM_FORM_PLAIN = 0, MPM_FORM_FAST = 1 };
+ *   accumulates per block in LDS without atomics and is staged for the grid
+ *   update, G2P is fused with the stress evaluation so each step needs one
+ *   SVD.
+ * MPM_FORM_FUSED: as FAST, but the G2P of step n and the P2G of step n+1 run
+ *   in one kernel per block (the grid velocities of the block's neighbourhood
+ *   in LDS); the stress never goes to memory.  mpm_step_phase runs the FAST
+ *   phases in this form. */
+enum { MPM_FORM_PLAIN = 0, MPM_FORM_FAST = 1, MPM_FORM_FUSED = 2 };
 
 /* Warm start for the polar decomposition in G2P. */
 enum { MPM_WARM_NONE = 0, MPM_WARM_MLP = 1 };

@@ -18,7 +18,7 @@ MPM_OK = 0
 JELLO, SNOW, WATER = 0, 1, 2
 BC_AXIS, BC_STICKY = 0, 1
 WATER_C_APIC, WATER_C_GRADV = 0, 1
-FORM_PLAIN, FORM_FAST = 0, 1
+FORM_PLAIN, FORM_FAST, FORM_FUSED = 0, 1, 2
 WARM_NONE, WARM_MLP = 0, 1
 VOLUME_REF, VOLUME_PPC = 0, 1
 SHAPE_MESH, SHAPE_BOX, SHAPE_SPHERE = 0, 1, 2

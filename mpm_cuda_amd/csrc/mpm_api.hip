@@ -124,6 +124,7 @@ static int validate_material(const mpm_material *m)
 }
 
 static int run_phase(mpm_sim *s, int phase, float dt);
+static int fused_step(mpm_sim *s, float dt);
 
 }  // namespace mpm
 
@@ -176,7 +177,8 @@ int mpm_create(const mpm_config *cfg, mpm_sim **out)
     if (cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4) return set_error(MPM_ERR_INVALID, "grid needs >= 4 nodes per axis");
     if ((int64_t)cfg->nx * cfg->ny * cfg->nz > (1ll << 31))
         return set_error(MPM_ERR_INVALID, "grid too large");
-    if (cfg->form != MPM_FORM_PLAIN && cfg->form != MPM_FORM_FAST) return set_error(MPM_ERR_INVALID, "unknown form");
+    if (cfg->form != MPM_FORM_PLAIN && cfg->form != MPM_FORM_FAST && cfg->form != MPM_FORM_FUSED)
+        return set_error(MPM_ERR_INVALID, "unknown form");
     if (cfg->bc_mode != MPM_BC_AXIS && cfg->bc_mode != MPM_BC_STICKY) return set_error(MPM_ERR_INVALID, "unknown bc_mode");
     if (cfg->svd_sweeps < 1 || cfg->svd_sweeps > 16) return set_error(MPM_ERR_INVALID, "svd_sweeps out of range [1,16]");
     int threads = cfg->threads ? cfg->threads : 256;
@@ -268,6 +270,7 @@ int mpm_add_particles(mpm_sim *s, const mpm_material *mat, int64_t n, const floa
     s->total += n;
     s->tau_valid = false;
     s->sorted_valid = false;
+    s->primed = false;
     if (group) *group = (int32_t)(s->groups.size() - 1);
     return MPM_OK;
 }
@@ -290,6 +293,7 @@ int mpm_step_phase(mpm_sim *s, int32_t phase, double dt)
     if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
     if (phase < MPM_PHASE_P2G || phase > MPM_PHASE_G2P) return set_error(MPM_ERR_INVALID, "unknown phase");
     float fdt = (float)(dt > 0 ? dt : s->cfg.dt);
+    s->primed = false;
     return run_phase(s, phase, fdt);
 }
 
@@ -298,9 +302,14 @@ int mpm_step(mpm_sim *s, double dt)
     if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
     double d = dt > 0 ? dt : s->cfg.dt;
     float fdt = (float)d;
-    for (int ph = MPM_PHASE_P2G; ph <= MPM_PHASE_G2P; ++ph) {
-        int rc = run_phase(s, ph, fdt);
+    if (s->cfg.form == MPM_FORM_FUSED) {
+        int rc = fused_step(s, fdt);
         if (rc) return rc;
+    } else {
+        for (int ph = MPM_PHASE_P2G; ph <= MPM_PHASE_G2P; ++ph) {
+            int rc = run_phase(s, ph, fdt);
+            if (rc) return rc;
+        }
     }
     s->steps += 1;          // ++current_frame (MPMCUDA.cu:923)
     s->time += d;           // current_time += adp_dt (MPMCUDA.cu:924)
@@ -398,6 +407,7 @@ int mpm_write_particles(mpm_sim *s, int32_t group, int32_t field, const float *i
     MPM_HIP(hipStreamSynchronize(s->stream));
     if (field == MPM_P_F || field == MPM_P_JP) s->tau_valid = false;
     if (field == MPM_P_POS) s->sorted_valid = false;
+    s->primed = false;
     return MPM_OK;
 }
 
@@ -434,7 +444,7 @@ int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
     for (uint32_t k = 0; k < nact; ++k) if (lst[k] < (uint32_t)s->nblocks) active[lst[k]] = 1;
     // grid_mp holds (m, p) between P2G and the grid update; grid_v after it
     const bool after_p2g = s->last_phase == MPM_PHASE_P2G;
-    if (after_p2g && s->cfg.form == MPM_FORM_FAST && s->blk_tile) {
+    if (after_p2g && s->cfg.form != MPM_FORM_PLAIN && s->blk_tile) {
         // FAST form: add the staged block arenas of the last P2G
         std::vector<uint32_t> tcount(s->groups.size());
         MPM_HIP(hipMemcpy(tcount.data(), s->tile_counts, tcount.size() * 4, hipMemcpyDeviceToHost));
@@ -606,12 +616,12 @@ struct PhaseTimer {
 // One phase of MPMSimulator::step (MPMCUDA.cu:793-925).
 static int run_phase(mpm_sim *s, int phase, float dt)
 {
-    const bool fast = s->cfg.form == MPM_FORM_FAST;
+    const bool fast = s->cfg.form != MPM_FORM_PLAIN;
     const int thr = s->cfg.threads;
     int rc = MPM_OK;
     s->last_phase = phase;
     if (phase == MPM_PHASE_P2G) {
-        rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 2), "zero");
+        rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 3), "zero");
         if (rc) return rc;
         if (fast) {
             {
@@ -655,6 +665,43 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         if (fast) s->steps_since_sort += 1;
     }
     return rc;
+}
+
+// FUSED form step: grid update of the primed scatter, periodic re-sort, then
+// G2P of this step and P2G of the next in one pass (MPMCUDA.cu:793-925 with
+// the P2G of step n+1 moved into the G2P kernel of step n).  The first step,
+// and any step after the state was rewritten or dt changed, primes the
+// scatter with the FAST P2G.
+int sort_if_due(mpm_sim *s);
+
+static int fused_step(mpm_sim *s, float dt)
+{
+    int rc = MPM_OK;
+    if (!s->primed || s->primed_dt != dt) {
+        rc = run_phase(s, MPM_PHASE_P2G, dt);
+        if (rc) return rc;
+    } else {
+        rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 2), "zero");
+        if (rc) return rc;
+    }
+    rc = run_phase(s, MPM_PHASE_GRID, dt);
+    if (rc) return rc;
+    {
+        PhaseTimer t(s, 0);
+        rc = sort_if_due(s);
+    }
+    if (rc) return rc;
+    s->last_phase = MPM_PHASE_G2P;
+    {
+        PhaseTimer t(s, 3);
+        for (size_t k = 0; k < s->groups.size() && !rc; ++k) rc = launch_g2p2g(s, (int)k, dt, dt);
+    }
+    if (rc) return rc;
+    s->primed = true;
+    s->primed_dt = dt;
+    s->tau_valid = false;           // the stress stayed in registers
+    s->steps_since_sort += 1;
+    return MPM_OK;
 }
 
 }  // namespace mpm

@@ -14,7 +14,7 @@
 // (On gfx950 an LDS float atomic runs ~20x slower than a plain LDS
 // read-modify-write, and global float atomics at ~77 G/s would dominate a
 // per-node flush.)
-#include "mpm_kernels.h"
+#include "mpm_g2p.h"
 #include "mpm_launch.h"
 
 namespace mpm {
@@ -316,12 +316,7 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
 // take turns.  The arena (8^3 nodes: the block's 4 + 1 below + 3 above per
 // axis) is written to the tile's staging slot with plain stores; the grid
 // update gathers the staging slots overlapping each node block.  Particles
-// outside the arena fall back to global atomics on grid_mp.
-constexpr int A_SY = 12, A_SZ = 100;          // LDS slot strides (float4 slots): spread 8-lane groups
-constexpr int A_SLOTS = 7 + 7 * A_SY + 7 * A_SZ + 1;
-
-__device__ __forceinline__ int a_slot(int x, int y, int z) { return x + A_SY * y + A_SZ * z; }
-
+// whose stencil leaves the arena are listed for the atomic fallback kernel.
 __device__ __forceinline__ uint64_t match_key9(uint32_t key)
 {
     uint64_t m = ~0ull;
@@ -344,118 +339,216 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
     return v;
 }
 
+// Per-tile set-up shared by the block kernels: clear the scatter arena and
+// find the [start, end) of every cell's particles (sorted by cell).
+struct TileCtx {
+    int bx, by, bz, ax, ay, az;
+    uint32_t my0, myn, rounds;
+};
+
+__device__ __forceinline__ TileCtx tile_begin(const GridP &g, const PartView &pv, const uint8_t *__restrict__ cellkey,
+                                              const uint4 &d, float4 *arena, uint32_t *cstart, uint32_t *cend)
+{
+    const int lane = threadIdx.x;
+    for (int k = lane; k < A_SLOTS; k += 64) arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    cstart[lane] = 0u;
+    cend[lane] = 0u;
+    __syncthreads();
+    for (uint32_t p = lane; p < d.y; p += 64) {
+        uint32_t c = cellkey[pv.off + d.x + p];
+        uint32_t cp = p > 0 ? cellkey[pv.off + d.x + p - 1] : 0xffu;
+        uint32_t cn = p + 1 < d.y ? cellkey[pv.off + d.x + p + 1] : 0xffu;
+        if (c != cp) cstart[c & 63] = p;
+        if (c != cn) cend[c & 63] = p + 1;
+    }
+    __syncthreads();
+    TileCtx t;
+    t.my0 = min(cstart[lane], d.y);
+    const uint32_t my1 = min(cend[lane], d.y);
+    t.myn = my1 > t.my0 ? my1 - t.my0 : 0u;
+    t.rounds = wave_max_u32(t.myn);
+    t.bx = (int)(d.z % (uint32_t)g.nbx);
+    t.by = (int)((d.z / (uint32_t)g.nbx) % (uint32_t)g.nby);
+    t.bz = (int)(d.z / ((uint32_t)g.nbx * (uint32_t)g.nby));
+    t.ax = t.bx * BLK - 1; t.ay = t.by * BLK - 1; t.az = t.bz * BLK - 1;
+    return t;
+}
+
+// One round of the conflict-free scatter into the LDS arena.  Lanes whose
+// stencil leaves the arena (or whose tile has no staging slot) are appended
+// to the spill list instead.
+__device__ __forceinline__ void scatter_round(const GridP &g, float4 *arena, const TileCtx &t, bool active, int i,
+                                              const Stencil &s, const Scatter &sc, bool staged, uint32_t *spill,
+                                              uint32_t *counters)
+{
+    const int lane = threadIdx.x;
+    const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
+    const bool inside = active && staged && lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
+                        lz <= ARENA - 3;
+    const bool spill_me = active && !inside;
+    const uint64_t sm = __ballot(spill_me);
+    if (sm) {
+        const int leader = __ffsll((unsigned long long)sm) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(counters + CT_SPILL, (uint32_t)__popcll(sm));
+        base = __shfl(base, leader);
+        const uint64_t below = lane ? (sm & ((~0ull) >> (64 - lane))) : 0ull;
+        if (spill_me) spill[base + __popcll(below)] = (uint32_t)i;
+    }
+    const uint32_t key = inside ? (uint32_t)(lx + 6 * (ly + 6 * lz)) : 511u;
+    uint64_t pending = __ballot(inside);
+    const uint64_t same = match_key9(key);
+    while (pending) {
+        const uint64_t grp = same & pending;
+        const bool winner = inside && ((pending >> lane) & 1ull) && (__ffsll((unsigned long long)grp) - 1 == lane);
+        if (winner) {
+            // Rolled loop over the 27 nodes: each contribution is computed
+            // right before its read-modify-write, so the ordered LDS chain
+            // overlaps with arithmetic and the live registers stay few.
+            int ox = 0, oy = 0, oz = 0;
+#pragma unroll 1
+            for (int o = 0; o < 27; ++o) {
+                const float wx = ox == 0 ? s.w[0][0] : (ox == 1 ? s.w[0][1] : s.w[0][2]);
+                const float wy = oy == 0 ? s.w[1][0] : (oy == 1 ? s.w[1][1] : s.w[1][2]);
+                const float wz = oz == 0 ? s.w[2][0] : (oz == 1 ? s.w[2][1] : s.w[2][2]);
+                const float gx = ox == 0 ? s.dw[0][0] : (ox == 1 ? s.dw[0][1] : s.dw[0][2]);
+                const float gy = oy == 0 ? s.dw[1][0] : (oy == 1 ? s.dw[1][1] : s.dw[1][2]);
+                const float gz = oz == 0 ? s.dw[2][0] : (oz == 1 ? s.dw[2][1] : s.dw[2][2]);
+                const float w = wx * wy * wz;
+                const float gr[3] = {g.inv_h * gx * wy * wz, g.inv_h * wx * gy * wz, g.inv_h * wx * wy * gz};
+                const float d[3] = {((float)ox - s.fx[0]) * g.h, ((float)oy - s.fx[1]) * g.h,
+                                    ((float)oz - s.fx[2]) * g.h};
+                float p[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float bd = sc.B[3 * c] * d[0] + sc.B[3 * c + 1] * d[1] + sc.B[3 * c + 2] * d[2];
+                    const float gg = sc.G[3 * c] * gr[0] + sc.G[3 * c + 1] * gr[1] + sc.G[3 * c + 2] * gr[2];
+                    p[c] = w * (sc.a[c] + bd) + gg;
+                }
+                const int a = a_slot(lx + ox, ly + oy, lz + oz);
+                float4 v = arena[a];
+                v.x += w * sc.m; v.y += p[0]; v.z += p[1]; v.w += p[2];
+                arena[a] = v;
+                __asm__ volatile("" ::: "memory");
+                if (++ox == 3) { ox = 0; if (++oy == 3) { oy = 0; ++oz; } }
+            }
+        }
+        pending &= ~__ballot(winner);
+    }
+}
+
+// Write the arena to staging slot `slot` (logical node order) and mark the
+// node blocks that received mass.
+__device__ __forceinline__ void tile_end(const GridP &g, const TileCtx &t, const float4 *arena, float4 *staging,
+                                         uint32_t slot, bool staged, uint32_t *flags)
+{
+    __syncthreads();
+    if (staged) {
+        float4 *dst = staging + (int64_t)slot * ARENA_NODES;
+        for (int k = threadIdx.x; k < ARENA_NODES; k += 64) {
+            const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
+            const float4 v = arena[a_slot(kx, ky, kz)];
+            dst[k] = v;
+            const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
+            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+                flags[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
+        }
+    }
+    __syncthreads();
+}
+
 template <int MAT>
-__global__ void __launch_bounds__(64) k_p2g_block(GridP g, MatP mat, PartView pv, const uint8_t *__restrict__ cellkey,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_p2g_block(GridP g, MatP mat, PartView pv, const uint8_t *__restrict__ cellkey,
                                                   const uint4 *__restrict__ desc, const uint32_t *__restrict__ tile_count,
                                                   uint32_t desc_cap, uint32_t staging_cap, float dt,
-                                                  float4 *__restrict__ staging, float4 *grid_spill, uint32_t *flags,
+                                                  float4 *__restrict__ staging, uint32_t *spill, uint32_t *flags,
                                                   uint32_t *counters)
 {
     __shared__ float4 arena[A_SLOTS];
     __shared__ uint32_t cstart[64], cend[64];
-    const int lane = threadIdx.x;
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        const uint4 d = desc[t];
-        for (int k = lane; k < A_SLOTS; k += 64) arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        cstart[lane] = 0u;
-        cend[lane] = 0u;
-        __syncthreads();
-        // cell runs of this block (particles are sorted by cell)
-        for (uint32_t p = lane; p < d.y; p += 64) {
-            uint32_t c = cellkey[pv.off + d.x + p];
-            uint32_t cp = p > 0 ? cellkey[pv.off + d.x + p - 1] : 0xffu;
-            uint32_t cn = p + 1 < d.y ? cellkey[pv.off + d.x + p + 1] : 0xffu;
-            if (c != cp) cstart[c & 63] = p;
-            if (c != cn) cend[c & 63] = p + 1;
-        }
-        __syncthreads();
-        const uint32_t my0 = min(cstart[lane], d.y);
-        const uint32_t my1 = min(cend[lane], d.y);
-        const uint32_t myn = my1 > my0 ? my1 - my0 : 0u;
-        const uint32_t rounds = wave_max_u32(myn);
-        const int bx = (int)(d.z % (uint32_t)g.nbx);
-        const int by = (int)((d.z / (uint32_t)g.nbx) % (uint32_t)g.nby);
-        const int bz = (int)(d.z / ((uint32_t)g.nbx * (uint32_t)g.nby));
-        const int ax = bx * BLK - 1, ay = by * BLK - 1, az = bz * BLK - 1;
+    for (uint32_t tt = blockIdx.x; tt < nt; tt += gridDim.x) {
+        const uint4 d = desc[tt];
+        const TileCtx t = tile_begin(g, pv, cellkey, d, arena, cstart, cend);
         const bool staged = d.w < staging_cap;
-        for (uint32_t r = 0; r < rounds; ++r) {
-            const bool active = r < myn;
+        for (uint32_t r = 0; r < t.rounds; ++r) {
+            const bool active = r < t.myn;
+            const int i = (int)(d.x + t.my0 + r);
             Stencil s;
+            s.b[0] = s.b[1] = s.b[2] = 0;
             Scatter sc;
-            bool inside = false;
-            uint32_t key = 511u;
             if (active) {
-                const int i = (int)(d.x + my0 + r);
                 float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
                 bits |= make_stencil(g, x, s);
                 load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
-                const int lx = s.b[0] - ax, ly = s.b[1] - ay, lz = s.b[2] - az;
-                inside = staged && lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
-                         lz <= ARENA - 3;
-                if (inside) key = (uint32_t)(lx + 6 * (ly + 6 * lz));
-                else {
-                    bits |= MPM_H_ARENA_SPILL;
-#pragma unroll
-                    for (int oz = 0; oz < 3; ++oz)
-#pragma unroll
-                        for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-                            for (int ox = 0; ox < 3; ++ox) {
-                                float4 c = node_contrib(g, s, sc, ox, oy, oz);
-                                float *dst = (float *)(grid_spill + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
-                                unsafeAtomicAdd(dst + 0, c.x);
-                                unsafeAtomicAdd(dst + 1, c.y);
-                                unsafeAtomicAdd(dst + 2, c.z);
-                                unsafeAtomicAdd(dst + 3, c.w);
-                            }
-                    int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2, by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
-                    int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
-                    for (int zz = bz0; zz <= bz1; ++zz)
-                        for (int yy = by0; yy <= by1; ++yy)
-                            for (int xx = bx0; xx <= bx1; ++xx)
-                                flags[((int64_t)zz * g.nby + yy) * g.nbx + xx] = 1u;
-                }
             }
-            uint64_t pending = __ballot(inside);
-            const uint64_t same = match_key9(key);
-            while (pending) {
-                const uint64_t grp = same & pending;
-                const bool winner = inside && ((pending >> lane) & 1ull) && (__ffsll((unsigned long long)grp) - 1 == lane);
-                if (winner) {
-                    const int lx = s.b[0] - ax, ly = s.b[1] - ay, lz = s.b[2] - az;
-#pragma unroll
-                    for (int oz = 0; oz < 3; ++oz)
-#pragma unroll
-                        for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-                            for (int ox = 0; ox < 3; ++ox) {
-                                float4 c = node_contrib(g, s, sc, ox, oy, oz);
-                                const int a = a_slot(lx + ox, ly + oy, lz + oz);
-                                float4 v = arena[a];
-                                v.x += c.x; v.y += c.y; v.z += c.z; v.w += c.w;
-                                arena[a] = v;
-                                __asm__ volatile("" ::: "memory");
-                            }
-                }
-                pending &= ~__ballot(winner);
-            }
+            scatter_round(g, arena, t, active, i, s, sc, staged, spill, counters);
+        }
+        tile_end(g, t, arena, staging, d.w, staged, flags);
+    }
+    if (bits) atomicOr(counters + CT_HEALTH, bits);
+    if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
+}
+
+// ---------------------------------------------------------------------------
+// FUSED form: G2P of step n and P2G of step n+1 in one pass over a block.
+// The block's 8^3 grid-velocity neighbourhood is copied to LDS; each particle
+// is gathered, updated (one SVD) and advected, its state written back, and
+// scattered at its new position with the new v, C and stress into the
+// scatter arena, which is staged for the grid update of step n+1.
+template <int MAT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_g2p2g_block(GridP g, MatP mat, PartView pv,
+                                                    const uint8_t *__restrict__ cellkey,
+                                                    const uint4 *__restrict__ desc,
+                                                    const uint32_t *__restrict__ tile_count, uint32_t desc_cap,
+                                                    uint32_t staging_cap, float dt, float dt_next,
+                                                    const float4 *__restrict__ gv, float4 *__restrict__ staging,
+                                                    uint32_t *spill, uint32_t *flags, uint32_t *counters, int sweeps)
+{
+    __shared__ float4 arena[A_SLOTS];
+    __shared__ float4 garena[A_SLOTS];
+    __shared__ uint32_t cstart[64], cend[64];
+    const int lane = threadIdx.x;
+    const uint32_t nt = min(*tile_count, desc_cap);
+    uint32_t bits = 0;
+    for (uint32_t tt = blockIdx.x; tt < nt; tt += gridDim.x) {
+        const uint4 d = desc[tt];
+        const TileCtx t = tile_begin(g, pv, cellkey, d, arena, cstart, cend);
+        // grid velocities of the arena nodes (zero outside the grid)
+        for (int k = lane; k < ARENA_NODES; k += 64) {
+            const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
+            const int x = t.ax + kx, y = t.ay + ky, z = t.az + kz;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz) v = gv[node_slot(g, x, y, z)];
+            garena[a_slot(kx, ky, kz)] = v;
         }
         __syncthreads();
-        // write the arena back: staging slot d.w, logical node order, and mark touched node blocks
-        if (staged) {
-            float4 *dst = staging + (int64_t)d.w * ARENA_NODES;
-            for (int k = lane; k < ARENA_NODES; k += 64) {
-                const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
-                const float4 v = arena[a_slot(kx, ky, kz)];
-                dst[k] = v;
-                const int nx_ = ax + kx, ny_ = ay + ky, nz_ = az + kz;
-                if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
-                    flags[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
+        const bool staged = d.w < staging_cap;
+        const ArenaFetch fetch{garena, gv, t.ax, t.ay, t.az};
+        for (uint32_t r = 0; r < t.rounds; ++r) {
+            const bool active = r < t.myn;
+            const int i = (int)(d.x + t.my0 + r);
+            Stencil s;
+            s.b[0] = s.b[1] = s.b[2] = 0;
+            Scatter sc;
+            if (active) {
+                G2POut o;
+                bits |= g2p_core<MAT>(g, mat, pv, i, dt, fetch, MAT != MPM_WATER, sweeps, counters, o);
+                bits |= make_stencil(g, o.x, s);
+                scatter_from_out<MAT>(mat, o, dt_next, sc);
+                const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
+                const bool inside = staged && lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
+                                    lz <= ARENA - 3;
+                if (!inside && MAT != MPM_WATER) {
+                    // the atomic fallback reads the stress from memory
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
+                }
             }
+            scatter_round(g, arena, t, active, i, s, sc, staged, spill, counters);
         }
-        __syncthreads();
+        tile_end(g, t, arena, staging, d.w, staged, flags);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
@@ -493,6 +586,7 @@ static int ensure_fast_buffers(mpm_sim *s)
         MPM_HIP(hipMalloc(&s->scan_tmp, s->nblocks * 4));
     }
     if (!s->cellkey) MPM_HIP(hipMalloc(&s->cellkey, s->cap));
+    if (!s->spill) MPM_HIP(hipMalloc(&s->spill, s->cap * sizeof(uint32_t)));
     int64_t need = 0;
     s->group_tile_off.assign(s->groups.size(), 0);
     for (size_t k = 0; k < s->groups.size(); ++k) {
@@ -561,6 +655,12 @@ static int sort_all(mpm_sim *s)
     return MPM_OK;
 }
 
+int sort_if_due(mpm_sim *s)
+{
+    if (!s->sorted_valid || s->steps_since_sort >= s->cfg.sort_every) return sort_all(s);
+    return MPM_OK;
+}
+
 int fast_prepare(mpm_sim *s)
 {
     int rc = MPM_OK;
@@ -589,14 +689,44 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     const uint32_t *tc = s->tile_counts + group;
     const uint32_t cap = (uint32_t)bound;
     const uint32_t scap = (uint32_t)s->staging_cap;
+    int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
+    if (rc) return hip_fail((hipError_t)rc, "zero");
     dim3 gr(wg), bl(64);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->grid_mp, s->flags, s->counters); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->grid_mp, s->flags, s->counters); break;
-    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->grid_mp, s->flags, s->counters); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
     }
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? MPM_OK : hip_fail(e, "p2g_block");
+    if (e != hipSuccess) return hip_fail(e, "p2g_block");
+    rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags);
+    return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;
+}
+
+int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
+{
+    PartView pv = view_of(s, group);
+    if (pv.n <= 0) return MPM_OK;
+    const MatP &m = s->groups[group].mat;
+    int64_t bound = tile_bound(s, pv.n);
+    int wg = (int)std::min<int64_t>(bound, 8192);
+    const uint4 *desc = (const uint4 *)s->tile_desc + s->group_tile_off[group];
+    const uint32_t *tc = s->tile_counts + group;
+    const uint32_t cap = (uint32_t)bound;
+    const uint32_t scap = (uint32_t)s->staging_cap;
+    const int sw = s->cfg.svd_sweeps;
+    int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
+    if (rc) return hip_fail((hipError_t)rc, "zero");
+    dim3 gr(wg), bl(64);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_g2p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_g2p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    default: hipLaunchKernelGGL(k_g2p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "g2p2g_block");
+    rc = launch_p2g_spill(s->stream, s->g, m, pv, dt_next, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags);
+    return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;
 }
 
 // Stand-alone sort for the probe entry point (own temporaries).

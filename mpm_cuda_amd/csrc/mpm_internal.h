@@ -64,12 +64,14 @@ struct PartView {
 
 // Device counters (one uint32 array).
 enum Counter : int {
-    CT_ACTIVE = 0,      // active grid blocks of the current step   } both reset at
-    CT_NODES = 1,       // grid nodes with mass in the grid update   } the start of P2G
-    CT_HEALTH = 2,      // MPM_H_* bits
-    CT_FALLBACK = 3,    // FP64 SVD fallbacks
-    CT_REJECT = 4,      // warm-start rejections
-    CT_TILES = 5,       // P2G tiles (FAST form)
+    CT_ACTIVE = 0,      // active grid blocks of the current step
+    CT_NODES = 1,       // grid nodes with mass in the grid update
+    CT_SPILL = 2,       // particles listed for the atomic P2G fallback
+    CT_HEALTH = 3,      // MPM_H_* bits
+    CT_FALLBACK = 4,    // FP64 SVD fallbacks
+    CT_REJECT = 5,      // warm-start rejections
+    CT_TILES = 6,       // P2G tiles (FAST / FUSED forms)
+    CT_SPILLED = 7,     // particles that took the atomic fallback (accumulated)
     CT_COUNT = 16
 };
 
@@ -113,6 +115,9 @@ struct mpm_sim {
     float4 *staging = nullptr;              // per tile: 8^3-node P2G arena written back without atomics
     int64_t staging_cap = 0;                // tiles the staging buffer holds
     int32_t *blk_tile = nullptr;            // [group][block]: staging tile of the block, -1 if none
+    uint32_t *spill = nullptr;              // indices of particles for the atomic P2G fallback
+    bool primed = false;                    // FUSED: staging holds the scatter of the current state
+    float primed_dt = 0.f;                  // FUSED: dt the primed scatter used for the force term
     bool sorted_valid = false;
     int64_t steps_since_sort = 0;
     int last_phase = -1;

@@ -16,11 +16,14 @@ int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t 
 int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads);
 int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);
+int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+                     const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags);
 
 // FAST form (mpm_fast.hip)
 int fast_prepare(mpm_sim *s);                 // (re)sort particles and build P2G tiles when due
 int launch_p2g_tiles(mpm_sim *s, int group, float dt);
 int launch_g2p_fast(mpm_sim *s, int group, float dt);
+int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next);   // FUSED form
 
 // Scene sampling and files (scene.cpp)
 int sample_object(const mpm_config &cfg, const mpm_object &obj, std::vector<float> &xyz);

@@ -4,7 +4,7 @@
 // VelUpdate :435-472, InterpolateAndMove :474-741, step :793-925.  Particles
 // are FP32 structure-of-arrays; the grid is block-major (4^3 nodes per block)
 // and only blocks touched by the current P2G are updated and read.
-#include "mpm_kernels.h"
+#include "mpm_g2p.h"
 #include "mpm_launch.h"
 
 namespace mpm {
@@ -157,125 +157,9 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
 }
 
 // ---------------------------------------------------------------------------
-// G2P, deformation update and advection for one particle
+// G2P, deformation update and advection, one thread per particle
 // (MPMCUDA.cu:474-741).  With write_tau the Kirchhoff stress of the updated
-// state is stored for the next P2G, so a step needs one SVD.
-template <int MAT>
-__device__ __forceinline__ uint32_t g2p_particle(const GridP &g, const MatP &mat, const PartView &pv,
-                                                 int i, float dt, const float4 *__restrict__ gv,
-                                                 bool write_tau, int sweeps, uint32_t *counters)
-{
-    float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
-    Stencil s;
-    uint32_t bits = make_stencil(g, x, s);
-    float v[3] = {0.f, 0.f, 0.f}, C[9], L[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) { C[k] = 0.f; L[k] = 0.f; }
-    const float c4 = 4.0f * g.inv_h;       // 4 w / h^2 (x_i - x_p), with x_i - x_p = (o - fx) h
-#pragma unroll
-    for (int ox = 0; ox < 3; ++ox)
-#pragma unroll
-        for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-            for (int oz = 0; oz < 3; ++oz) {
-                float4 nv = gv[node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz)];
-                float wx = s.w[0][ox], wy = s.w[1][oy], wz = s.w[2][oz];
-                float w = wx * wy * wz;
-                float gr[3] = {g.inv_h * s.dw[0][ox] * wy * wz, g.inv_h * wx * s.dw[1][oy] * wz,
-                               g.inv_h * wx * wy * s.dw[2][oz]};
-                float dd[3] = {(float)ox - s.fx[0], (float)oy - s.fx[1], (float)oz - s.fx[2]};
-                float vi[3] = {nv.x, nv.y, nv.z};
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    v[r] += w * vi[r];                                   // :516-518
-                    float cw = c4 * w * vi[r];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        C[3 * r + c] += cw * dd[c];                      // :520-532
-                        L[3 * r + c] += vi[r] * gr[c];                   // :561-585
-                    }
-                }
-            }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) st(pv, PL_V + r, i, v[r]);
-    if (MAT == MPM_WATER && mat.water_c_mode == MPM_WATER_C_GRADV) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) st(pv, PL_C + k, i, L[k]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) st(pv, PL_C + k, i, C[k]);
-    }
-    float A[9];                                                          // I + dt grad v
-#pragma unroll
-    for (int k = 0; k < 9; ++k) A[k] = dt * L[k] + ((k % 4 == 0) ? 1.0f : 0.0f);
-    if (MAT == MPM_JELLO) {                                              // :596-619
-        float F[9], Fn[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
-        matmul3<float>(A, F, Fn);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fn[k]);
-        if (!(det3<float>(Fn) > 0.f)) bits |= MPM_H_DET_NONPOS;
-        if (write_tau) {
-            float t[6];
-            stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, t, counters);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, t[k]);
-        }
-    } else if (MAT == MPM_SNOW) {                                        // :620-709
-        float F[9], Ft[9], U[9], sg[3], V[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
-        matmul3<float>(A, F, Ft);
-        svd_checked(Ft, U, sg, V, sweeps, counters);
-        float ratio = 1.0f;
-        float sc[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            sc[k] = fminf(fmaxf(sg[k], 1.0f - mat.theta_c), 1.0f + mat.theta_s);   // :658-660
-            ratio *= sg[k] / sc[k];
-        }
-        // Fe = U S* V^T, written as Ftr + U (S* - S) V^T: identical in exact
-        // arithmetic, and exactly Ftr when no singular value is clamped, so
-        // the FP32 SVD's rounding does not leak into F every step.
-        float ds[3] = {sc[0] - sg[0], sc[1] - sg[1], sc[2] - sg[2]};
-        float Fe[9];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                Fe[3 * r + c] = Ft[3 * r + c] + (U[3 * r] * ds[0] * V[3 * c] + U[3 * r + 1] * ds[1] * V[3 * c + 1] +
-                                                 U[3 * r + 2] * ds[2] * V[3 * c + 2]);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fe[k]);
-        // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
-        float Jp = ld(pv, PL_S, i) * ratio;
-        st(pv, PL_S, i, Jp);
-        if (!(Jp > 0.f)) bits |= MPM_H_DET_NONPOS;
-        if (write_tau) {
-            float hc = expf(mat.xi * (1.0f - Jp));
-            float t[6];
-            fcr_tau_from_svd<float>(U, sc, mat.mu * hc, mat.lam * hc, t);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, t[k]);
-        }
-    } else {                                                             // :710-713
-        float J = ld(pv, PL_S, i);
-        st(pv, PL_S, i, (1.0f + dt * (L[0] + L[4] + L[8])) * J);
-    }
-    const float lo[3] = {g.ox, g.oy, g.oz};
-    const int nn[3] = {g.nx, g.ny, g.nz};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float xn = x[a] + dt * v[a];                                     // :717-719
-        st(pv, PL_X + a, i, xn);
-        float hi = lo[a] + g.h * (float)(nn[a] - 1);
-        if (!(xn >= lo[a] && xn <= hi)) bits |= MPM_H_LEFT_DOMAIN;       // :722-739
-        if (!isfinite(xn) || !isfinite(v[a])) bits |= MPM_H_NONFINITE;
-    }
-    return bits;
-}
-
+// state is stored for the next P2G, so a FAST-form step needs one SVD.
 template <int MAT>
 __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, float dt,
                                              const float4 *__restrict__ gv, int write_tau,
@@ -283,8 +167,48 @@ __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, flo
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pv.n) return;
-    uint32_t bits = g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters);
+    G2POut o;
+    GlobalFetch f{gv};
+    uint32_t bits = g2p_core<MAT>(g, mat, pv, i, dt, f, write_tau != 0 && MAT != MPM_WATER, sweeps, counters, o);
+    if (write_tau && MAT != MPM_WATER) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
+    }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
+}
+
+// Particles whose stencil left their block arena (listed by the block
+// kernels): scatter with global atomics into grid_mp, which the grid update
+// adds in.  Persistent over the list.
+template <int MAT>
+__global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView pv, float dt,
+                                                   const uint32_t *__restrict__ list, uint32_t *counters,
+                                                   float4 *grid, uint32_t *flags, uint32_t list_cap)
+{
+    const uint32_t n = min(counters[CT_SPILL], list_cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int i = (int)list[k];
+        if (i < 0 || i >= pv.n) continue;
+        float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+        Stencil s;
+        make_stencil(g, x, s);
+        Scatter sc;
+        load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+                for (int oz = 0; oz < 3; ++oz) {
+                    float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                    float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
+                    unsafeAtomicAdd(dst + 0, c.x);
+                    unsafeAtomicAdd(dst + 1, c.y);
+                    unsafeAtomicAdd(dst + 2, c.z);
+                    unsafeAtomicAdd(dst + 3, c.w);
+                }
+        mark_blocks(g, s, flags);
+    }
 }
 
 // Kirchhoff stress of the current state, for the first P2G of the FAST form
@@ -355,6 +279,19 @@ int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv
     case MPM_JELLO: hipLaunchKernelGGL(k_g2p<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
     case MPM_SNOW: hipLaunchKernelGGL(k_g2p<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
     default: hipLaunchKernelGGL(k_g2p<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+                     const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags)
+{
+    if (pv.n <= 0) return 0;
+    dim3 gr(64), bl(256);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_spill<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_spill<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap); break;
+    default: hipLaunchKernelGGL(k_p2g_spill<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap); break;
     }
     return (int)hipGetLastError();
 }

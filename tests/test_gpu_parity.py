@@ -17,9 +17,9 @@ TOL_GRID = 1e-5
 TOL_PART = 1e-5
 
 MATS = [capi.JELLO, capi.SNOW, capi.WATER]
-FORMS = [capi.FORM_PLAIN, capi.FORM_FAST]
+FORMS = [capi.FORM_PLAIN, capi.FORM_FAST, capi.FORM_FUSED]
 MAT_ID = {capi.JELLO: "jello", capi.SNOW: "snow", capi.WATER: "water"}
-FORM_ID = {capi.FORM_PLAIN: "plain", capi.FORM_FAST: "fast"}
+FORM_ID = {capi.FORM_PLAIN: "plain", capi.FORM_FAST: "fast", capi.FORM_FUSED: "fused"}
 
 
 def _ids(p):
@@ -176,19 +176,45 @@ def test_rollout_parity(gpu, case):
         p.close()
 
 
-def test_fast_matches_plain(gpu):
-    """The two execution forms agree with each other after 20 snow steps."""
+@pytest.mark.parametrize("form", [capi.FORM_FAST, capi.FORM_FUSED], ids=["fast", "fused"])
+def test_form_matches_plain(gpu, form):
+    """The sorted forms agree with the plain form after 20 snow steps with
+    re-sorting every 3 steps (particles drift between sorts)."""
     a = Pair(type_=capi.SNOW, form=capi.FORM_PLAIN, seed=9)
-    b = Pair(type_=capi.SNOW, form=capi.FORM_FAST, seed=9, sort_every=3)
+    b = Pair(type_=capi.SNOW, form=form, seed=9, sort_every=3)
     try:
         for _ in range(20):
             a.gpu.step(a.dt)
             b.gpu.step(b.dt)
         ga, gb = a.gpu_particles(), b.gpu_particles()
-        for k in ("x", "v", "C"):
-            assert rel(gb[k], ga[k]) < 1e-4, k
+        errs = {k: rel(gb[k], ga[k]) for k in ("x", "v", "C", "F")}
+        print(FORM_ID[form], "vs plain", {k: f"{v:.2e}" for k, v in errs.items()})
+        for k, e in errs.items():
+            assert e < 1e-5, (k, e)
         st = b.gpu.stats()
         assert st["sorts"] >= 6
+    finally:
+        a.close()
+        b.close()
+
+
+def test_fused_dt_change_and_rewrite(gpu):
+    """FUSED form re-primes its scatter when dt changes or the state is
+    rewritten; results equal the FAST form doing the same."""
+    a = Pair(type_=capi.WATER, form=capi.FORM_FAST, seed=4)
+    b = Pair(type_=capi.WATER, form=capi.FORM_FUSED, seed=4)
+    try:
+        for k in range(6):
+            dt = 1e-4 if k < 3 else 5e-5
+            a.gpu.step(dt)
+            b.gpu.step(dt)
+            if k == 4:
+                v = a.gpu.read(0, capi.P_VEL) * 0.5
+                a.gpu.write(0, capi.P_VEL, v)
+                b.gpu.write(0, capi.P_VEL, v)
+        ga, gb = a.gpu_particles(), b.gpu_particles()
+        for k in ("x", "v", "C", "S"):
+            assert rel(gb[k] - (1.0 if k == "S" else 0.0), ga[k] - (1.0 if k == "S" else 0.0)) < 1e-5, k
     finally:
         a.close()
         b.close()
