@@ -62,7 +62,23 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the plain-form comparison run")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy of known size: under `rocprofv3 --pmc FETCH_SIZE` (or
+    WRITE_SIZE) its per-dispatch counter reads mib * 1024 KiB when the
+    counter is exact, which gives the scale applied to the MPM kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copy_kib": n * 4 // 1024, "launches": 3}), flush=True)
 
 
 def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, timing=True):
@@ -133,6 +149,9 @@ def cpu_baseline(workload, steps):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))

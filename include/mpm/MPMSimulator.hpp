@@ -66,11 +66,31 @@ struct ObjInitInfo {
     long long maxParticles = 0;
 };
 
+// Grid (MPMSimulator.cuh:54-64): the geometry the reference's viewer reads
+// (main.cpp:171-176).  Node data stays on the device; read it with
+// mpm_read_grid on handle().
+struct Grid {
+    double h = 0.0;
+    unsigned int nodeNumDimX = 0, nodeNumDimY = 0, nodeNumDimZ = 0;
+    std::array<double, 3> originCorner{0.0, 0.0, 0.0};
+};
+
+// ParticleGroup (MPMSimulator.cuh:30-52): counts and per-particle constants;
+// the particle planes stay on the device (mpm_read_particles).
+struct ParticleGroup {
+    unsigned int particleNum = 0;
+    double mParticleMass = 0.0;
+    double mParticleVolume = 0.0;
+    Material mMaterial;
+};
+
 class MPMSimulator {
 public:
     double adp_dt = 0.0;        // MPMSimulator.cuh:85 (the reference never adapts it)
     double current_time = 0.0;  // :86
     int current_frame = 0;      // :87
+    std::vector<ParticleGroup> mParticlesGroupsVec;  // :89
+    Grid mGrid{};                                     // :90
 
     // MPMSimulator.cuh:92-96.  `cfg` (optional) carries the settings the
     // reference hard-codes: gravity, wall band, execution form, ...
@@ -84,6 +104,9 @@ public:
         c.nx = (int)nodeNumDimX; c.ny = (int)nodeNumDimY; c.nz = (int)nodeNumDimZ;
         c.ppc = (int)particleNumPerCell;
         adp_dt = dt;
+        mGrid.h = gap;
+        mGrid.nodeNumDimX = nodeNumDimX; mGrid.nodeNumDimY = nodeNumDimY; mGrid.nodeNumDimZ = nodeNumDimZ;
+        for (int k = 0; k < 3; ++k) mGrid.originCorner[k] = c.origin[k];
         throw_if(mpm_create(&c, &sim_));
         for (auto &info : objInitInfoVec) {
             mpm_object o;
@@ -100,13 +123,21 @@ public:
             o.rotation_deg = info.initRotationDegree;
             o.material = info.mMaterial.c;
             o.max_particles = info.maxParticles;
-            int rc = mpm_add_object(sim_, &o, nullptr);
+            int32_t gidx = -1;
+            int rc = mpm_add_object(sim_, &o, &gidx);
             if (rc != MPM_OK) {
                 std::string msg = mpm_last_error();
                 mpm_destroy(sim_);
                 sim_ = nullptr;
                 throw std::runtime_error("mpm: " + msg);
             }
+            ParticleGroup pg;
+            pg.particleNum = (unsigned int)mpm_group_size(sim_, gidx);
+            pg.mMaterial = info.mMaterial;
+            const double vol = c.volume_mode == MPM_VOLUME_PPC ? gap * gap * gap / c.ppc : gap * gap * gap;
+            pg.mParticleVolume = vol;                        // MPMSimulator.cpp:263
+            pg.mParticleMass = vol * info.mMaterial.mDensity; // MPMSimulator.cpp:264
+            mParticlesGroupsVec.push_back(pg);
         }
     }
     MPMSimulator(const MPMSimulator &) = delete;

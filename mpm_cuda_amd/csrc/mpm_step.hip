@@ -111,23 +111,25 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
         gmp[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (blk_tile) {
             for (int gi = 0; gi < ngroups; ++gi) {
+                // lanes 0..26 look up the staging slots of the 27 neighbour
+                // blocks at once; the gathers below are then independent
                 const int32_t *bt = blk_tile + (int64_t)gi * nblocks;
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int pz = bz + dz, az = kz + 1 - 4 * dz;
-                    if (pz < 0 || pz >= g.nbz) continue;
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int py = by + dy, ay = ky + 1 - 4 * dy;
-                        if (py < 0 || py >= g.nby) continue;
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int px = bx + dx, ax = kx + 1 - 4 * dx;
-                            if (px < 0 || px >= g.nbx) continue;
-                            const int32_t t = bt[((int64_t)pz * g.nby + py) * g.nbx + px];
-                            if (t < 0 || (uint32_t)t >= staging_cap) continue;
-                            if (ax >= 0 && ax < ARENA && ay >= 0 && ay < ARENA && az >= 0 && az < ARENA) {
-                                const float4 v = staging[(int64_t)t * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
-                                a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
-                            }
-                        }
+                int32_t my_t = -1;
+                if (lane < 27) {
+                    const int px = bx + lane % 3 - 1, py = by + (lane / 3) % 3 - 1, pz = bz + lane / 9 - 1;
+                    if (px >= 0 && py >= 0 && pz >= 0 && px < g.nbx && py < g.nby && pz < g.nbz) {
+                        const int32_t t = bt[((int64_t)pz * g.nby + py) * g.nbx + px];
+                        if (t >= 0 && (uint32_t)t < staging_cap) my_t = t;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 27; ++j) {
+                    const int32_t t = __builtin_amdgcn_readfirstlane(__shfl(my_t, j));
+                    const int dx = j % 3 - 1, dy = (j / 3) % 3 - 1, dz = j / 9 - 1;
+                    const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
+                    if (t >= 0 && ax >= 0 && ax < ARENA && ay >= 0 && ay < ARENA && az >= 0 && az < ARENA) {
+                        const float4 v = staging[(int64_t)t * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
+                        a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
                     }
                 }
             }
