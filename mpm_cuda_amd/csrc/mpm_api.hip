@@ -571,7 +571,7 @@ void mpm_destroy(mpm_sim *s)
     void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
                     s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
                     s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
-                    s->cellkey, s->staging, s->blk_tile};
+                    s->staging, s->blk_tile, s->spill};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
@@ -641,15 +641,20 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         return rc;
     }
     if (phase == MPM_PHASE_GRID) {
-        PhaseTimer t(s, 2);
-        rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
-        if (rc) return rc;
-        const bool staged = fast && s->blk_tile;
-        return check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,
-                                               (int)s->nblocks, staged ? s->staging : nullptr,
-                                               staged ? s->blk_tile : nullptr, (int)s->groups.size(),
-                                               (uint32_t)s->staging_cap),
-                            "grid_update");
+        {
+            PhaseTimer t(s, 2);
+            rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
+            if (rc) return rc;
+            const bool staged = fast && s->blk_tile;
+            rc = check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,
+                                                 (int)s->nblocks, staged ? s->staging : nullptr,
+                                                 staged ? s->blk_tile : nullptr, (int)s->groups.size(),
+                                                 (uint32_t)s->staging_cap),
+                              "grid_update");
+            if (rc) return rc;
+        }
+        if (s->timing) rc = check_launch(launch_count_nodes(s->stream, s->list, s->counters, s->grid_v), "count");
+        return rc;
     }
     // G2P
     PhaseTimer t(s, 3);

@@ -222,11 +222,10 @@ static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *
 // ---------------------------------------------------------------------------
 // Keys, permutation and tiles.
 //
-// Sort key: (linear index of the 4^3-node block holding the particle's base
-// node) << 6 | (cell of the base inside that block).  A stable sort leaves
-// every block's particles grouped by cell, which P2G uses to give each lane
-// of a wave its own cell.
-constexpr int KEY_LOW = 6;
+// Sort key: linear index of the 4^3-node block holding the particle's base
+// node.  The stable sort keeps the previous order inside a block; P2G re-bins
+// every block's particles by their current base node in LDS.
+constexpr int KEY_LOW = 0;
 
 __device__ __forceinline__ void base_of(const GridP &g, const PartView &pv, int i, int b[3])
 {
@@ -247,16 +246,14 @@ __global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *val
     base_of(g, pv, i, b);
     uint32_t blk = ((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
                    (uint32_t)(b[0] >> 2);
-    uint32_t cell = (uint32_t)(((b[2] & 3) * BLK + (b[1] & 3)) * BLK + (b[0] & 3));
-    keys[i] = (blk << KEY_LOW) | cell;
+    keys[i] = blk << KEY_LOW;
     vals[i] = (uint32_t)i;
 }
 
-// Gathers every permuted SoA plane, the particle ids and the sort-time cell.
+// Gathers every permuted SoA plane and the particle ids.
 __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, int64_t cap, int64_t off, int n,
-                          const uint32_t *__restrict__ perm, const uint32_t *__restrict__ skeys,
-                          const uint32_t *__restrict__ pid, uint32_t *__restrict__ pida, uint8_t *__restrict__ cellkey,
-                          uint32_t plane_mask)
+                          const uint32_t *__restrict__ perm, const uint32_t *__restrict__ pid,
+                          uint32_t *__restrict__ pida, uint32_t plane_mask)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -265,7 +262,6 @@ __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, i
     for (int p = 0; p < PL_COUNT; ++p)
         if ((plane_mask >> p) & 1u) Pa[(int64_t)p * cap + off + j] = P[(int64_t)p * cap + off + src];
     pida[off + j] = pid[off + src];
-    cellkey[off + j] = (uint8_t)(skeys[j] & ((1u << KEY_LOW) - 1u));
 }
 
 // First and one-past-last sorted position of every block (untouched blocks
@@ -305,29 +301,41 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
 // ---------------------------------------------------------------------------
 // P2G over blocks without atomics.
 //
-// One wave owns one block's particles.  Lane l takes cell l of the block;
-// in round r it scatters the r-th particle of that cell, so the lanes of a
-// round normally have 64 different base nodes and the 27 stencil updates of
-// one instruction touch 64 different LDS slots: a plain read-modify-write is
-// race-free.  Updates of different stencil offsets can alias across lanes,
-// so they are issued in program order (compiler barrier between offsets; LDS
-// executes a wave's operations in order).  Particles that drifted since the
-// sort may share a base: lanes are matched on their base and same-base lanes
-// take turns.  The arena (8^3 nodes: the block's 4 + 1 below + 3 above per
-// axis) is written to the tile's staging slot with plain stores; the grid
-// update gathers the staging slots overlapping each node block.  Particles
-// whose stencil leaves the arena are listed for the atomic fallback kernel.
-__device__ __forceinline__ uint64_t match_key9(uint32_t key)
-{
-    uint64_t m = ~0ull;
-#pragma unroll
-    for (int b = 0; b < 9; ++b) {
-        bool bit = (key >> b) & 1u;
-        uint64_t bb = __ballot(bit);
-        m &= bit ? bb : ~bb;
-    }
-    return m;
-}
+// A tile is one node block's particles (grouped by the last sort).  It is
+// processed in chunks of at most CHUNK particles:
+//   A. one lane per particle (dense and coalesced): load the particle (FAST)
+//      or run its G2P update (FUSED), find its base node and write a record
+//      (below) to LDS.  The base, relative to the block's 8^3-node arena
+//      (the block's 4 nodes + 1 below + 3 above per axis), is one of 6^3
+//      bins.
+//   B. wave 0 sorts the chunk by bin (stable counting sort in LDS) and cuts
+//      every bin into items of at most ITEM_CAP particles.
+//   C. one lane per item: the item's particles share their base node, so the
+//      lane sums their 27 node contributions in registers and adds them to
+//      its wave's LDS arena once, in a fixed node order.  In each of the 27
+//      read-modify-writes the lanes of different bins touch different slots;
+//      lanes of one bin (a bin cut into several items) take turns.
+// The waves' arenas are summed into the tile's staging slot, which the grid
+// update gathers.  Particles whose stencil leaves the arena (or whose tile
+// has no staging slot) are listed for the atomic fallback kernel.
+// (On gfx950 an LDS float atomic runs ~20x slower than a plain LDS
+// read-modify-write; summing in registers first also divides the LDS traffic
+// by the particles per item.)
+constexpr int TILE_WAVES = 2;
+constexpr int TILE_THREADS = 64 * TILE_WAVES;
+static_assert(ARENA_NODES % TILE_THREADS == 0, "arena nodes per thread must be whole");
+constexpr int CHUNK = 256;          // particles per pass over a tile
+constexpr int ITEM_CAP = 4;         // particles per lane item
+constexpr int NBIN = 216;           // 6^3 base positions inside an arena
+constexpr uint32_t NO_BIN = 255u;
+// Record of one particle for the register scatter (FP32):
+//   [0..2]  fx: position relative to the base node, in cells
+//   [3..5]  q0 = m v - h m C fx: the APIC momentum at the base node
+//   [6..14] hB = h m C (row-major)
+//   [15..20] G = -dt V tau (xx, yy, zz, xy, xz, yz); water: [15] = dt V p J
+// so node o (offsets in {0,1,2}^3) receives
+//   m w_o  and  w_o (q0 + hB o) + G grad w_o     (MPMCUDA.cu:391-414).
+constexpr int REC = 21;
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 {
@@ -339,56 +347,48 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
     return v;
 }
 
-// A tile (one block's particles) is processed by TILE_WAVES waves: wave w
-// takes rounds w, w + TILE_WAVES, ... and scatters into its own LDS arena,
-// so no two waves ever write the same slot; the arenas are summed when the
-// tile is written back.  (Measured on 1M-particle snow: 1, 2 and 4 waves per
-// tile run within 10% of each other; 2 is the fastest for both forms.)
-constexpr int TILE_WAVES = 2;
-constexpr int TILE_THREADS = 64 * TILE_WAVES;
-
-// Per-tile set-up shared by the block kernels: clear the scatter arenas and
-// find the [start, end) of every cell's particles (sorted by cell).
-struct TileCtx {
-    int bx, by, bz, ax, ay, az;
-    uint32_t my0, myn, rounds;
-};
-
-__device__ __forceinline__ TileCtx tile_begin(const GridP &g, const PartView &pv, const uint8_t *__restrict__ cellkey,
-                                              const uint4 &d, float4 (*arena)[A_SLOTS], uint32_t *cstart,
-                                              uint32_t *cend)
+// Mask of the lanes whose 8-bit key equals this lane's.
+__device__ __forceinline__ uint64_t match_key8(uint32_t key)
 {
-    const int tid = threadIdx.x, lane = tid & 63;
-    for (int k = tid; k < TILE_WAVES * A_SLOTS; k += TILE_THREADS)
-        arena[k / A_SLOTS][k % A_SLOTS] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (tid < 64) { cstart[tid] = 0u; cend[tid] = 0u; }
-    __syncthreads();
-    for (uint32_t p = tid; p < d.y; p += TILE_THREADS) {
-        uint32_t c = cellkey[pv.off + d.x + p];
-        uint32_t cp = p > 0 ? cellkey[pv.off + d.x + p - 1] : 0xffu;
-        uint32_t cn = p + 1 < d.y ? cellkey[pv.off + d.x + p + 1] : 0xffu;
-        if (c != cp) cstart[c & 63] = p;
-        if (c != cn) cend[c & 63] = p + 1;
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        bool bit = (key >> b) & 1u;
+        uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
     }
-    __syncthreads();
-    TileCtx t;
-    t.my0 = min(cstart[lane], d.y);
-    const uint32_t my1 = min(cend[lane], d.y);
-    t.myn = my1 > t.my0 ? my1 - t.my0 : 0u;
-    t.rounds = wave_max_u32(t.myn);
-    t.bx = (int)(d.z % (uint32_t)g.nbx);
-    t.by = (int)((d.z / (uint32_t)g.nbx) % (uint32_t)g.nby);
-    t.bz = (int)(d.z / ((uint32_t)g.nbx * (uint32_t)g.nby));
-    t.ax = t.bx * BLK - 1; t.ay = t.by * BLK - 1; t.az = t.bz * BLK - 1;
-    return t;
+    return m;
 }
 
-// One round of the conflict-free scatter into this wave's LDS arena.  Lanes
-// whose stencil leaves the arena (or whose tile has no staging slot) are
-// appended to the spill list instead.
-__device__ __forceinline__ void scatter_round(const GridP &g, float4 *arena, const TileCtx &t, bool active, int i,
-                                              const Stencil &s, const Scatter &sc, bool staged, uint32_t *spill,
-                                              uint32_t *counters)
+struct TileLds {
+    float4 arena[TILE_WAVES][A_SLOTS];
+    float rec[REC][CHUNK];
+    uint32_t items[CHUNK];
+    uint16_t perm[CHUNK];
+    uint16_t rnk[CHUNK];
+    uint32_t cnt[NBIN];
+    uint8_t key[CHUNK];
+    uint32_t nitems;
+};
+
+struct TileGeo {
+    int ax, ay, az;     // arena origin (node index of arena slot 0)
+};
+
+__device__ __forceinline__ TileGeo tile_geo(const GridP &g, uint32_t blk)
+{
+    const int bx = (int)(blk % (uint32_t)g.nbx);
+    const int by = (int)((blk / (uint32_t)g.nbx) % (uint32_t)g.nby);
+    const int bz = (int)(blk / ((uint32_t)g.nbx * (uint32_t)g.nby));
+    return TileGeo{bx * BLK - 1, by * BLK - 1, bz * BLK - 1};
+}
+
+// Phase A tail: bin the particle and write its record, or list it for the
+// atomic fallback.  Returns true when the particle was listed.
+template <int MAT>
+__device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const TileGeo &t, bool active, int slot,
+                                             int i, const Stencil &s, const Scatter &sc, bool staged,
+                                             uint32_t *spill, uint32_t *counters)
 {
     const int lane = threadIdx.x & 63;
     const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
@@ -404,52 +404,233 @@ __device__ __forceinline__ void scatter_round(const GridP &g, float4 *arena, con
         const uint64_t below = lane ? (sm & ((~0ull) >> (64 - lane))) : 0ull;
         if (spill_me) spill[base + __popcll(below)] = (uint32_t)i;
     }
-    const uint32_t key = inside ? (uint32_t)(lx + 6 * (ly + 6 * lz)) : 511u;
-    uint64_t pending = __ballot(inside);
-    const uint64_t same = match_key9(key);
-    while (pending) {
-        const uint64_t grp = same & pending;
-        const bool winner = inside && ((pending >> lane) & 1ull) && (__ffsll((unsigned long long)grp) - 1 == lane);
-        if (winner) {
-            // 27 nodes in a fixed order; each contribution is computed right
-            // before its read-modify-write.  The empty asm keeps the compiler
-            // from reordering the LDS reads and writes across nodes.
-            int ox = 0, oy = 0, oz = 0;
+    if (active) L.key[slot] = inside ? (uint8_t)(lx + 6 * (ly + 6 * lz)) : (uint8_t)NO_BIN;
+    if (inside) {
+        const float h = 1.0f / g.inv_h;
 #pragma unroll
-            for (int o = 0; o < 27; ++o) {
-                const float wx = ox == 0 ? s.w[0][0] : (ox == 1 ? s.w[0][1] : s.w[0][2]);
-                const float wy = oy == 0 ? s.w[1][0] : (oy == 1 ? s.w[1][1] : s.w[1][2]);
-                const float wz = oz == 0 ? s.w[2][0] : (oz == 1 ? s.w[2][1] : s.w[2][2]);
-                const float gx = ox == 0 ? s.dw[0][0] : (ox == 1 ? s.dw[0][1] : s.dw[0][2]);
-                const float gy = oy == 0 ? s.dw[1][0] : (oy == 1 ? s.dw[1][1] : s.dw[1][2]);
-                const float gz = oz == 0 ? s.dw[2][0] : (oz == 1 ? s.dw[2][1] : s.dw[2][2]);
-                const float w = wx * wy * wz;
-                const float gr[3] = {g.inv_h * gx * wy * wz, g.inv_h * wx * gy * wz, g.inv_h * wx * wy * gz};
-                const float d[3] = {((float)ox - s.fx[0]) * g.h, ((float)oy - s.fx[1]) * g.h,
-                                    ((float)oz - s.fx[2]) * g.h};
-                float p[3];
+        for (int a = 0; a < 3; ++a) L.rec[a][slot] = s.fx[a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float bf = sc.B[3 * c] * s.fx[0] + sc.B[3 * c + 1] * s.fx[1] + sc.B[3 * c + 2] * s.fx[2];
+            L.rec[3 + c][slot] = sc.a[c] - h * bf;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) L.rec[6 + k][slot] = h * sc.B[k];
+        if (MAT == MPM_WATER) {
+            L.rec[15][slot] = sc.G[0];
+        } else {
+            L.rec[15][slot] = sc.G[0]; L.rec[16][slot] = sc.G[4]; L.rec[17][slot] = sc.G[8];
+            L.rec[18][slot] = sc.G[1]; L.rec[19][slot] = sc.G[2]; L.rec[20][slot] = sc.G[5];
+        }
+    }
+    return spill_me;
+}
+
+// Phase B (wave 0 only): stable counting sort of the chunk's nc particles by
+// bin and the item list.  Items pack bin | first sorted position << 8 |
+// count << 20.
+__device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (int b = lane; b < NBIN; b += 64) L.cnt[b] = 0u;
+    __asm__ volatile("" ::: "memory");
+    for (int s0 = 0; s0 < nc; s0 += 64) {
+        const int sl = s0 + lane;
+        const uint32_t key = sl < nc ? (uint32_t)L.key[sl] : NO_BIN;
+        const bool v = key != NO_BIN;
+        const uint64_t m = match_key8(key) & __ballot(v);
+        const uint32_t below = (uint32_t)__popcll(m & lt);
+        uint32_t prev = 0;
+        if (v) prev = L.cnt[key];
+        if (v && below == 0) L.cnt[key] = prev + (uint32_t)__popcll(m);
+        if (v) L.rnk[sl] = (uint16_t)(prev + below);
+        __asm__ volatile("" ::: "memory");
+    }
+    // lane l owns bins 4l .. 4l+3
+    uint32_t c[4], nit[4], csum = 0, isum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int b = 4 * lane + j;
+        c[j] = b < NBIN ? L.cnt[b] : 0u;
+        nit[j] = (c[j] + ITEM_CAP - 1) / ITEM_CAP;
+        csum += c[j];
+        isum += nit[j];
+    }
+    uint32_t crun = wave_incl_scan(csum) - csum;
+    uint32_t irun = wave_incl_scan(isum) - isum;
+    if (lane == 63) L.nitems = irun + isum;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int b = 4 * lane + j;
+        if (b < NBIN) L.cnt[b] = crun;          // now the bin's first sorted position
+        for (uint32_t q = 0; q < nit[j]; ++q) {
+            const uint32_t first = crun + q * ITEM_CAP;
+            const uint32_t n = min((uint32_t)ITEM_CAP, c[j] - q * ITEM_CAP);
+            L.items[irun + q] = (uint32_t)b | (first << 8) | (n << 20);
+        }
+        crun += c[j];
+        irun += nit[j];
+    }
+    __asm__ volatile("" ::: "memory");
+    for (int s0 = 0; s0 < nc; s0 += 64) {
+        const int sl = s0 + lane;
+        const uint32_t key = sl < nc ? (uint32_t)L.key[sl] : NO_BIN;
+        if (key != NO_BIN) L.perm[L.cnt[key] + L.rnk[sl]] = (uint16_t)sl;
+    }
+}
+
+// Phase C, one particle: add its 27 node contributions to acc (node
+// ox + 3 oy + 9 oz).  The sum is factored per axis:
+//   p_c(o) = Nz (alpha_c + oz beta_c) + Dz gamma_c
+// with alpha, beta, gamma depending on (ox, oy) only, N the B-spline weights
+// and D = dN/dx.  Offsets enter through kadd/kmul (no multiply by a
+// constant 0, which the compiler must keep); water's G is diagonal.
+__device__ __forceinline__ float kadd(int k, float a, float b) { return k == 0 ? a : (k == 1 ? a + b : a + (b + b)); }
+
+template <int MAT>
+__device__ __forceinline__ void accum_particle(const TileLds &L, int slot, float inv_h, float m, float4 acc[27])
+{
+    constexpr bool W = MAT == MPM_WATER;
+    float fx[3], q0[3], hB[9], G[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fx[a] = L.rec[a][slot];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q0[c] = L.rec[3 + c][slot];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hB[k] = L.rec[6 + k][slot];
+    if (W) {
+        const float gw = L.rec[15][slot];
+        G[0] = G[4] = G[8] = gw;
+        G[1] = G[2] = G[3] = G[5] = G[6] = G[7] = 0.0f;     // never read
+    } else {
+        G[0] = L.rec[15][slot]; G[4] = L.rec[16][slot]; G[8] = L.rec[17][slot];
+        G[1] = G[3] = L.rec[18][slot];
+        G[2] = G[6] = L.rec[19][slot];
+        G[5] = G[7] = L.rec[20][slot];
+    }
+    // G_ck is structurally zero for water off the diagonal
+#define GZ(c, k) (W && (c) != (k))
+    float N[3][3], D[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bspline_axis_fast<float>(fx[a], N[a], D[a]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) D[a][k] *= inv_h;
+    }
+    // ey[oy][c] = N_y oy hB_c1 + G_c1 D_y ; eyz marks a structural zero
+    float ey[3][3];
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float gd = GZ(c, 1) ? 0.0f : G[3 * c + 1] * D[1][oy];
+            const float nb = oy == 0 ? 0.0f : N[1][oy] * (oy == 1 ? hB[3 * c + 1] : hB[3 * c + 1] + hB[3 * c + 1]);
+            ey[oy][c] = oy == 0 ? gd : (GZ(c, 1) ? nb : nb + gd);
+        }
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+        float sx[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float t = N[0][ox] * kadd(ox, q0[c], hB[3 * c]);
+            sx[c] = GZ(c, 0) ? t : fmaf(G[3 * c], D[0][ox], t);
+        }
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy) {
+            const float nxy = N[0][ox] * N[1][oy];
+            const float mxy = nxy * m;
+            float al[3], be[3], ga[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const bool eyz = oy == 0 && GZ(c, 1);
+                al[c] = eyz ? N[1][oy] * sx[c] : fmaf(N[1][oy], sx[c], N[0][ox] * ey[oy][c]);
+                be[c] = nxy * hB[3 * c + 2];
+                ga[c] = GZ(c, 2) ? 0.0f : nxy * G[3 * c + 2];
+            }
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                float4 &A = acc[ox + 3 * oy + 9 * oz];
+                const float nz = N[2][oz], dz = D[2][oz];
+                A.x = fmaf(mxy, nz, A.x);
+                float *Ac[3] = {&A.y, &A.z, &A.w};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float bd = sc.B[3 * c] * d[0] + sc.B[3 * c + 1] * d[1] + sc.B[3 * c + 2] * d[2];
-                    const float gg = sc.G[3 * c] * gr[0] + sc.G[3 * c + 1] * gr[1] + sc.G[3 * c + 2] * gr[2];
-                    p[c] = w * (sc.a[c] + bd) + gg;
+                    float v = fmaf(nz, kadd(oz, al[c], be[c]), *Ac[c]);
+                    if (!GZ(c, 2)) v = fmaf(dz, ga[c], v);
+                    *Ac[c] = v;
                 }
-                const int a = a_slot(lx + ox, ly + oy, lz + oz);
-                float4 v = arena[a];
-                v.x += w * sc.m; v.y += p[0]; v.z += p[1]; v.w += p[2];
-                arena[a] = v;
-                __asm__ volatile("" ::: "memory");
-                if (++ox == 3) { ox = 0; if (++oy == 3) { oy = 0; ++oz; } }
             }
+        }
+    }
+#undef GZ
+}
+
+// Phase C for one wave: items [batch * 64, ...) of the chunk.
+template <int MAT>
+__device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int batch, float inv_h, float m)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t it = (uint32_t)(batch * 64 + lane);
+    const bool has = it < L.nitems;
+    const uint32_t item = has ? L.items[it] : 0u;
+    const uint32_t bin = has ? (item & 255u) : NO_BIN;
+    const uint32_t first = (item >> 8) & 4095u, cnt = has ? (item >> 20) : 0u;
+    float4 acc[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t rounds = wave_max_u32(cnt);
+    for (uint32_t r = 0; r < rounds; ++r)
+        if (r < cnt) accum_particle<MAT>(L, L.perm[first + r], inv_h, m, acc);
+    // ordered read-modify-write of the 27 nodes; same-bin lanes take turns
+    const int bx = (int)(bin % 6u), by = (int)((bin / 6u) % 6u), bz = (int)(bin / 36u);
+    uint64_t pending = __ballot(has);
+    const uint64_t same = match_key8(bin);
+    while (pending) {
+        const uint64_t grp = same & pending;
+        const bool winner = has && ((pending >> lane) & 1ull) && (__ffsll((unsigned long long)grp) - 1 == lane);
+        if (winner) {
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+                for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+                    for (int ox = 0; ox < 3; ++ox) {
+                        const int a = a_slot(bx + ox, by + oy, bz + oz);
+                        const float4 &c = acc[ox + 3 * oy + 9 * oz];
+                        float4 v = arena[a];
+                        v.x += c.x; v.y += c.y; v.z += c.z; v.w += c.w;
+                        arena[a] = v;
+                        __asm__ volatile("" ::: "memory");
+                    }
         }
         pending &= ~__ballot(winner);
     }
 }
 
+// Phases B and C of one chunk (all threads; ends synchronized).
+template <int MAT>
+__device__ __forceinline__ void scatter_chunk(TileLds &L, int nc, float inv_h, float m)
+{
+    __syncthreads();
+    if ((threadIdx.x >> 6) == 0) sort_chunk(L, nc);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const int nb = (int)((L.nitems + 63u) >> 6);
+    for (int b = wave; b < nb; b += TILE_WAVES) scatter_items<MAT>(L, L.arena[wave], b, inv_h, m);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void tile_clear(TileLds &L)
+{
+    for (int k = threadIdx.x; k < TILE_WAVES * A_SLOTS; k += TILE_THREADS)
+        L.arena[k / A_SLOTS][k % A_SLOTS] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // Sum the waves' arenas into staging slot `slot` (logical node order) and
 // mark the node blocks that received mass.
-__device__ __forceinline__ void tile_end(const GridP &g, const TileCtx &t, const float4 (*arena)[A_SLOTS],
-                                         float4 *staging, uint32_t slot, bool staged, uint32_t *flags)
+__device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const TileLds &L, float4 *staging,
+                                         uint32_t slot, bool staged, uint32_t *flags)
 {
     __syncthreads();
     if (staged) {
@@ -457,10 +638,10 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileCtx &t, const
         for (int k = threadIdx.x; k < ARENA_NODES; k += TILE_THREADS) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
             const int a = a_slot(kx, ky, kz);
-            float4 v = arena[0][a];
+            float4 v = L.arena[0][a];
 #pragma unroll
             for (int w = 1; w < TILE_WAVES; ++w) {
-                const float4 u = arena[w][a];
+                const float4 u = L.arena[w][a];
                 v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
             }
             dst[k] = v;
@@ -473,37 +654,44 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileCtx &t, const
 }
 
 template <int MAT>
-__global__ void __launch_bounds__(TILE_THREADS) k_p2g_block(GridP g, MatP mat, PartView pv,
-                                                            const uint8_t *__restrict__ cellkey,
+__global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_block(GridP g, MatP mat, PartView pv,
                                                             const uint4 *__restrict__ desc,
                                                             const uint32_t *__restrict__ tile_count, uint32_t desc_cap,
                                                             uint32_t staging_cap, float dt,
                                                             float4 *__restrict__ staging, uint32_t *spill,
                                                             uint32_t *flags, uint32_t *counters)
 {
-    __shared__ float4 arena[TILE_WAVES][A_SLOTS];
-    __shared__ uint32_t cstart[64], cend[64];
-    const int wave = threadIdx.x >> 6;
+    __shared__ TileLds L;
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
     for (uint32_t tt = blockIdx.x; tt < nt; tt += gridDim.x) {
         const uint4 d = desc[tt];
-        const TileCtx t = tile_begin(g, pv, cellkey, d, arena, cstart, cend);
+        const TileGeo t = tile_geo(g, d.z);
         const bool staged = d.w < staging_cap;
-        for (uint32_t r = wave; r < t.rounds; r += TILE_WAVES) {
-            const bool active = r < t.myn;
-            const int i = (int)(d.x + t.my0 + r);
-            Stencil s;
-            s.b[0] = s.b[1] = s.b[2] = 0;
-            Scatter sc;
-            if (active) {
-                float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
-                bits |= make_stencil(g, x, s);
-                load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+        tile_clear(L);
+        for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
+            const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
+            for (int s0 = (int)(threadIdx.x & ~63u); s0 < nc; s0 += TILE_THREADS) {
+                const int slot = s0 + (int)(threadIdx.x & 63u);
+                const bool active = slot < nc;
+                const int i = (int)(d.x + c0) + slot;
+                Stencil s;
+                s.b[0] = s.b[1] = s.b[2] = 0;
+                Scatter sc;
+                if (active) {
+                    float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+                    load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+                    // issue all loads before the first use: under the kernel's
+                    // register cap the scheduler would otherwise wait for each
+                    // load in turn
+                    __builtin_amdgcn_sched_group_barrier(0x0020, 21, 0);
+                    bits |= make_stencil(g, x, s);
+                }
+                bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
             }
-            scatter_round(g, arena[wave], t, active, i, s, sc, staged, spill, counters);
+            scatter_chunk<MAT>(L, nc, g.inv_h, mat.mass);
         }
-        tile_end(g, t, arena, staging, d.w, staged, flags);
+        tile_end(g, t, L, staging, d.w, staged, flags);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
@@ -511,13 +699,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_p2g_block(GridP g, MatP mat, P
 
 // ---------------------------------------------------------------------------
 // FUSED form: G2P of step n and P2G of step n+1 in one pass over a block.
-// The block's 8^3 grid-velocity neighbourhood is copied to LDS; each particle
-// is gathered, updated (one SVD) and advected, its state written back, and
-// scattered at its new position with the new v, C and stress into the
-// scatter arenas, which are staged for the grid update of step n+1.
+// The block's 8^3 grid-velocity neighbourhood is copied to LDS; in phase A
+// each particle is gathered, updated (one SVD) and advected, its state is
+// written back, and its record for step n+1 is built from the new state;
+// phases B and C scatter the chunk into the arenas, which are staged for the
+// grid update of step n+1.
 template <int MAT>
-__global__ void __launch_bounds__(TILE_THREADS) k_g2p2g_block(GridP g, MatP mat, PartView pv,
-                                                              const uint8_t *__restrict__ cellkey,
+__global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_g2p2g_block(GridP g, MatP mat, PartView pv,
                                                               const uint4 *__restrict__ desc,
                                                               const uint32_t *__restrict__ tile_count,
                                                               uint32_t desc_cap, uint32_t staging_cap, float dt,
@@ -525,49 +713,61 @@ __global__ void __launch_bounds__(TILE_THREADS) k_g2p2g_block(GridP g, MatP mat,
                                                               float4 *__restrict__ staging, uint32_t *spill,
                                                               uint32_t *flags, uint32_t *counters, int sweeps)
 {
-    __shared__ float4 arena[TILE_WAVES][A_SLOTS];
+    __shared__ TileLds L;
     __shared__ float4 garena[A_SLOTS];
-    __shared__ uint32_t cstart[64], cend[64];
-    const int wave = threadIdx.x >> 6;
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
     for (uint32_t tt = blockIdx.x; tt < nt; tt += gridDim.x) {
         const uint4 d = desc[tt];
-        const TileCtx t = tile_begin(g, pv, cellkey, d, arena, cstart, cend);
-        // grid velocities of the arena nodes (zero outside the grid)
-        for (int k = threadIdx.x; k < ARENA_NODES; k += TILE_THREADS) {
-            const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
-            const int x = t.ax + kx, y = t.ay + ky, z = t.az + kz;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz) v = gv[node_slot(g, x, y, z)];
-            garena[a_slot(kx, ky, kz)] = v;
+        const TileGeo t = tile_geo(g, d.z);
+        const bool staged = d.w < staging_cap;
+        tile_clear(L);
+        // grid velocities of the arena nodes (zero outside the grid); all
+        // loads are issued before the first LDS write
+        {
+            constexpr int R = ARENA_NODES / TILE_THREADS;
+            float4 v[R];
+            bool in[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int k = (int)threadIdx.x + r * TILE_THREADS;
+                const int x = t.ax + (k & 7), y = t.ay + ((k >> 3) & 7), z = t.az + (k >> 6);
+                in[r] = x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz;
+                v[r] = gv[node_slot(g, min(max(x, 0), g.nx - 1), min(max(y, 0), g.ny - 1), min(max(z, 0), g.nz - 1))];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int k = (int)threadIdx.x + r * TILE_THREADS;
+                garena[a_slot(k & 7, (k >> 3) & 7, k >> 6)] = in[r] ? v[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
         __syncthreads();
-        const bool staged = d.w < staging_cap;
         const ArenaFetch fetch{garena, gv, t.ax, t.ay, t.az};
-        for (uint32_t r = wave; r < t.rounds; r += TILE_WAVES) {
-            const bool active = r < t.myn;
-            const int i = (int)(d.x + t.my0 + r);
-            Stencil s;
-            s.b[0] = s.b[1] = s.b[2] = 0;
-            Scatter sc;
-            if (active) {
+        for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
+            const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
+            for (int s0 = (int)(threadIdx.x & ~63u); s0 < nc; s0 += TILE_THREADS) {
+                const int slot = s0 + (int)(threadIdx.x & 63u);
+                const bool active = slot < nc;
+                const int i = (int)(d.x + c0) + slot;
+                Stencil s;
+                s.b[0] = s.b[1] = s.b[2] = 0;
+                Scatter sc;
                 G2POut o;
-                bits |= g2p_core<MAT>(g, mat, pv, i, dt, fetch, MAT != MPM_WATER, sweeps, counters, o);
-                bits |= make_stencil(g, o.x, s);
-                scatter_from_out<MAT>(mat, o, dt_next, sc);
-                const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
-                const bool inside = staged && lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
-                                    lz <= ARENA - 3;
-                if (!inside && MAT != MPM_WATER) {
+                if (active) {
+                    bits |= g2p_core<MAT>(g, mat, pv, i, dt, fetch, MAT != MPM_WATER, sweeps, counters, o);
+                    bits |= make_stencil(g, o.x, s);
+                    scatter_from_out<MAT>(mat, o, dt_next, sc);
+                }
+                const bool listed = bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
+                if (listed && MAT != MPM_WATER) {
                     // the atomic fallback reads the stress from memory
 #pragma unroll
                     for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
                 }
             }
-            scatter_round(g, arena[wave], t, active, i, s, sc, staged, spill, counters);
+            scatter_chunk<MAT>(L, nc, g.inv_h, mat.mass);
         }
-        tile_end(g, t, arena, staging, d.w, staged, flags);
+        tile_end(g, t, L, staging, d.w, staged, flags);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
@@ -604,7 +804,6 @@ static int ensure_fast_buffers(mpm_sim *s)
         MPM_HIP(hipMalloc(&s->blk_start, s->nblocks * 4));
         MPM_HIP(hipMalloc(&s->scan_tmp, s->nblocks * 4));
     }
-    if (!s->cellkey) MPM_HIP(hipMalloc(&s->cellkey, s->cap));
     if (!s->spill) MPM_HIP(hipMalloc(&s->spill, s->cap * sizeof(uint32_t)));
     int64_t need = 0;
     s->group_tile_off.assign(s->groups.size(), 0);
@@ -653,7 +852,7 @@ static int sort_all(mpm_sim *s)
         const uint32_t *perm = in_alt ? s->vals_alt : s->vals;
         const uint32_t *skeys = in_alt ? s->keys_alt : s->keys;
         hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, s->P, s->P_alt, s->cap, pv.off, n, perm,
-                           skeys, s->pid, s->pid_alt, s->cellkey, plane_mask);
+                           s->pid, s->pid_alt, plane_mask);
         // tiles: one per non-empty block, numbered in block order
         hipLaunchKernelGGL(k_block_runs, dim3(nblk(n, 256)), dim3(256), 0, st, skeys, n, s->blk_start, blk_end);
         hipLaunchKernelGGL(k_tile_flags, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_start, blk_end, s->scan_tmp,
@@ -712,9 +911,9 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     if (rc) return hip_fail((hipError_t)rc, "zero");
     dim3 gr(wg), bl(TILE_THREADS);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
-    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
@@ -738,9 +937,9 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     if (rc) return hip_fail((hipError_t)rc, "zero");
     dim3 gr(wg), bl(TILE_THREADS);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_g2p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_g2p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
-    default: hipLaunchKernelGGL(k_g2p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, s->cellkey, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_g2p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_g2p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    default: hipLaunchKernelGGL(k_g2p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "g2p2g_block");

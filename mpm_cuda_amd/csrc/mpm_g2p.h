@@ -17,6 +17,7 @@ constexpr int A_SLOTS = ARENA_NODES;
 __device__ __forceinline__ int a_slot(int x, int y, int z) { return (x ^ ((y & 1) << 2)) + ARENA * (y + ARENA * z); }
 
 struct GlobalFetch {
+    static constexpr bool kArena = false;
     const float4 *gv;
     __device__ __forceinline__ float4 operator()(const GridP &g, int x, int y, int z) const
     {
@@ -26,6 +27,7 @@ struct GlobalFetch {
 
 // Velocities of the block arena [a, a+8)^3 from LDS, other nodes from HBM.
 struct ArenaFetch {
+    static constexpr bool kArena = true;
     const float4 *lds;
     const float4 *gv;
     int ax, ay, az;
@@ -35,7 +37,84 @@ struct ArenaFetch {
         if (lx < (unsigned)ARENA && ly < (unsigned)ARENA && lz < (unsigned)ARENA) return lds[a_slot(lx, ly, lz)];
         return gv[node_slot(g, x, y, z)];
     }
+    // all 27 stencil nodes of base b lie in the arena
+    __device__ __forceinline__ bool covers(const int b[3]) const
+    {
+        return (unsigned)(b[0] - ax) <= (unsigned)(ARENA - 3) && (unsigned)(b[1] - ay) <= (unsigned)(ARENA - 3) &&
+               (unsigned)(b[2] - az) <= (unsigned)(ARENA - 3);
+    }
 };
+
+// The arena only, for stencils known to lie inside it: no per-node test,
+// so the 27 LDS reads issue back to back.
+struct ArenaOnlyFetch {
+    const float4 *lds;
+    int ax, ay, az;
+    __device__ __forceinline__ float4 operator()(const GridP &, int x, int y, int z) const
+    {
+        return lds[a_slot(x - ax, y - ay, z - az)];
+    }
+};
+
+// Interpolation (MPMCUDA.cu:516-532, 561-585), summed axis by axis:
+//   v_r  = sum_o w_o v_or
+//   M_rc = sum_o w_o o_c v_or          (C_rc = 4/h (M_rc - fx_c v_r))
+//   L_rc = sum_o v_or dw_o/dx_c
+// with w_o = Nx Ny Nz and D = dN/dx per axis; the inner sums over oz and oy
+// are shared by all three outputs.
+template <class Fetch>
+__device__ __forceinline__ void gather_grid(const GridP &g, const Stencil &s, const Fetch &fetch, float v[3],
+                                            float M[9], float L[9])
+{
+    float N[3][3], D[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { N[a][k] = s.w[a][k]; D[a][k] = g.inv_h * s.dw[a][k]; }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) v[r] = -0.0f;     // -0 + x == x: the first add folds away
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { M[k] = -0.0f; L[k] = -0.0f; }
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+        float Y0[3] = {-0.f, -0.f, -0.f}, Y1[3] = {-0.f, -0.f, -0.f}, Y2[3] = {-0.f, -0.f, -0.f};
+        float Y3[3] = {-0.f, -0.f, -0.f}, Y4[3] = {-0.f, -0.f, -0.f};
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy) {
+            float S0[3] = {-0.f, -0.f, -0.f}, S1[3] = {-0.f, -0.f, -0.f}, S2[3] = {-0.f, -0.f, -0.f};
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                const float4 nv = fetch(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz);
+                const float vi[3] = {nv.x, nv.y, nv.z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    S0[r] += N[2][oz] * vi[r];
+                    S1[r] += D[2][oz] * vi[r];
+                    if (oz) S2[r] += ((float)oz * N[2][oz]) * vi[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                Y0[r] += N[1][oy] * S0[r];
+                Y1[r] += D[1][oy] * S0[r];
+                if (oy) Y2[r] += ((float)oy * N[1][oy]) * S0[r];
+                Y3[r] += N[1][oy] * S2[r];
+                Y4[r] += N[1][oy] * S1[r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            v[r] += N[0][ox] * Y0[r];
+            if (ox) M[3 * r] += ((float)ox * N[0][ox]) * Y0[r];
+            M[3 * r + 1] += N[0][ox] * Y2[r];
+            M[3 * r + 2] += N[0][ox] * Y3[r];
+            L[3 * r] += D[0][ox] * Y0[r];
+            L[3 * r + 1] += N[0][ox] * Y1[r];
+            L[3 * r + 2] += N[0][ox] * Y4[r];
+        }
+    }
+}
 
 // Updated particle state kept in registers for a following scatter.
 struct G2POut {
@@ -48,36 +127,32 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
                                              G2POut &out)
 {
     float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+    // every particle load is issued up front (one memory latency, not one
+    // per load when the kernel runs near its register cap)
+    float F[9], Sp = 1.0f;
+    if (MAT != MPM_WATER) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+    }
+    if (MAT != MPM_JELLO) Sp = ld(pv, PL_S, i);
+    __builtin_amdgcn_sched_group_barrier(0x0020, MAT == MPM_WATER ? 4 : (MAT == MPM_JELLO ? 12 : 13), 0);
     Stencil s;
     uint32_t bits = make_stencil(g, x, s);
-    float v[3] = {0.f, 0.f, 0.f}, C[9], L[9];
+    float v[3], M[9], L[9];
+    if constexpr (Fetch::kArena) {
+        // a particle scattered into this arena last step gathers from it
+        // alone; one outside it falls back to per-node tests
+        if (__all(fetch.covers(s.b))) gather_grid(g, s, ArenaOnlyFetch{fetch.lds, fetch.ax, fetch.ay, fetch.az}, v, M, L);
+        else gather_grid(g, s, fetch, v, M, L);
+    } else {
+        gather_grid(g, s, fetch, v, M, L);
+    }
+    const float c4 = 4.0f * g.inv_h;
+    float C[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) { C[k] = 0.f; L[k] = 0.f; }
-    const float c4 = 4.0f * g.inv_h;       // 4 w / h^2 (x_i - x_p), with x_i - x_p = (o - fx) h
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int ox = 0; ox < 3; ++ox)
-#pragma unroll
-        for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-            for (int oz = 0; oz < 3; ++oz) {
-                float4 nv = fetch(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz);
-                float wx = s.w[0][ox], wy = s.w[1][oy], wz = s.w[2][oz];
-                float w = wx * wy * wz;
-                float gr[3] = {g.inv_h * s.dw[0][ox] * wy * wz, g.inv_h * wx * s.dw[1][oy] * wz,
-                               g.inv_h * wx * wy * s.dw[2][oz]};
-                float dd[3] = {(float)ox - s.fx[0], (float)oy - s.fx[1], (float)oz - s.fx[2]};
-                float vi[3] = {nv.x, nv.y, nv.z};
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    v[r] += w * vi[r];                                   // :516-518
-                    float cw = c4 * w * vi[r];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        C[3 * r + c] += cw * dd[c];                      // :520-532
-                        L[3 * r + c] += vi[r] * gr[c];                   // :561-585
-                    }
-                }
-            }
+        for (int c = 0; c < 3; ++c) C[3 * r + c] = c4 * (M[3 * r + c] - s.fx[c] * v[r]);
 #pragma unroll
     for (int r = 0; r < 3; ++r) { st(pv, PL_V + r, i, v[r]); out.v[r] = v[r]; }
     const bool c_is_gradv = (MAT == MPM_WATER && mat.water_c_mode == MPM_WATER_C_GRADV);
@@ -90,9 +165,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
 #pragma unroll
     for (int k = 0; k < 9; ++k) A[k] = dt * L[k] + ((k % 4 == 0) ? 1.0f : 0.0f);
     if (MAT == MPM_JELLO) {                                              // :596-619
-        float F[9], Fn[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+        float Fn[9];
         matmul3<float>(A, F, Fn);
 #pragma unroll
         for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fn[k]);
@@ -100,9 +173,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters);
         out.S = 1.0f;
     } else if (MAT == MPM_SNOW) {                                        // :620-709
-        float F[9], Ft[9], U[9], sg[3], V[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+        float Ft[9], U[9], sg[3], V[9];
         matmul3<float>(A, F, Ft);
         svd_checked(Ft, U, sg, V, sweeps, counters);
         float ratio = 1.0f;
@@ -124,7 +195,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
                    Ft[3 * r + c] + (U[3 * r] * ds[0] * V[3 * c] + U[3 * r + 1] * ds[1] * V[3 * c + 1] +
                                     U[3 * r + 2] * ds[2] * V[3 * c + 2]));
         // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
-        float Jp = ld(pv, PL_S, i) * ratio;
+        float Jp = Sp * ratio;
         st(pv, PL_S, i, Jp);
         if (!(Jp > 0.f)) bits |= MPM_H_DET_NONPOS;
         if (want_tau) {
@@ -133,7 +204,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         }
         out.S = Jp;
     } else {                                                             // :710-713
-        float J = (1.0f + dt * (L[0] + L[4] + L[8])) * ld(pv, PL_S, i);
+        float J = (1.0f + dt * (L[0] + L[4] + L[8])) * Sp;
         st(pv, PL_S, i, J);
         out.S = J;
     }

@@ -111,7 +111,6 @@ struct mpm_sim {
     int64_t tile_cap = 0;
     std::vector<int64_t> group_tile_off;    // per group: first tile slot
     uint32_t *tile_counts = nullptr;        // per group: tiles built by the last sort
-    uint8_t *cellkey = nullptr;             // per particle: cell of its base node inside its block at sort time
     float4 *staging = nullptr;              // per tile: 8^3-node P2G arena written back without atomics
     int64_t staging_cap = 0;                // tiles the staging buffer holds
     int32_t *blk_tile = nullptr;            // [group][block]: staging tile of the block, -1 if none

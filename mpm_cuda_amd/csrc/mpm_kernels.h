@@ -37,7 +37,7 @@ __device__ __forceinline__ uint32_t make_stencil(const GridP &g, const float x[3
         s.b[a] = b;
         // relative coordinate, one rounding: (x - o) * inv_h - b
         s.fx[a] = fmaf(x[a] - o[a], g.inv_h, -(float)b);
-        bspline_axis<float>(s.fx[a], s.w[a], s.dw[a]);
+        bspline_axis_fast<float>(s.fx[a], s.w[a], s.dw[a]);
     }
     return bits;
 }

@@ -10,6 +10,7 @@ int launch_p2g_atomic(hipStream_t st, const GridP &g, const MatP &m, const PartV
                       int sweeps, int tau_stored, float4 *grid, uint32_t *flags, uint32_t *counters,
                       int threads);
 int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters);
+int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters, const float4 *gv);
 int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
                        uint32_t *counters, float4 *gmp, float4 *gv, int nblocks, const float4 *staging,
                        const int32_t *blk_tile, int ngroups, uint32_t staging_cap);

@@ -43,6 +43,24 @@ MPM_HD void bspline_axis(T fx, T w[3], T dw[3])
     }
 }
 
+// bspline_axis for the usual case fx in [0.5, 1.5] (base node not clamped)
+// in closed form without branches; any other fx takes the general form.
+template <class T>
+MPM_HD void bspline_axis_fast(T fx, T w[3], T dw[3])
+{
+    if (fx >= T(0.5) && fx <= T(1.5)) {
+        const T a0 = T(1.5) - fx, d1 = fx - T(1), a2 = fx - T(0.5);
+        w[0] = T(0.5) * a0 * a0;
+        w[1] = T(0.75) - d1 * d1;
+        w[2] = T(0.5) * a2 * a2;
+        dw[0] = -a0;
+        dw[1] = T(-2) * d1;
+        dw[2] = a2;
+    } else {
+        bspline_axis<T>(fx, w, dw);
+    }
+}
+
 // Base node index on FP32 positions: t = (x - o) * inv_h - 0.5f with every
 // operation rounded to FP32 (no contraction), base = max(0, floor(t)).  The
 // oracle's orc_base_f32 is the same formula; the reference form is

@@ -100,7 +100,6 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
     const uint32_t nact = counters[CT_ACTIVE];
     const int lane = threadIdx.x;
     const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-    uint32_t nodes = 0;     // nodes with mass seen by this wave
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
         const uint32_t b = __builtin_amdgcn_readfirstlane(list[k]);
         const int bx = (int)(b % (uint32_t)g.nbx);
@@ -122,16 +121,23 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
                         if (t >= 0 && (uint32_t)t < staging_cap) my_t = t;
                     }
                 }
+                // A node lies in exactly 8 of the 27 arenas: per axis its own
+                // block's (d = 0) and the one below (d = -1, local k <= 2) or
+                // above (d = +1, k = 3).  All 8 loads are issued before any is
+                // used.
+                const int sx = kx == 3 ? 1 : -1, sy = ky == 3 ? 1 : -1, sz = kz == 3 ? 1 : -1;
+                float4 v[8];
 #pragma unroll
-                for (int j = 0; j < 27; ++j) {
-                    const int32_t t = __builtin_amdgcn_readfirstlane(__shfl(my_t, j));
-                    const int dx = j % 3 - 1, dy = (j / 3) % 3 - 1, dz = j / 9 - 1;
+                for (int e = 0; e < 8; ++e) {
+                    const int dx = (e & 1) ? sx : 0, dy = (e & 2) ? sy : 0, dz = (e & 4) ? sz : 0;
+                    const int32_t t = __shfl(my_t, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
                     const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
-                    if (t >= 0 && ax >= 0 && ax < ARENA && ay >= 0 && ay < ARENA && az >= 0 && az < ARENA) {
-                        const float4 v = staging[(int64_t)t * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
-                        a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
-                    }
+                    const int64_t ts = t >= 0 ? t : 0;
+                    v[e] = staging[ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
+                    if (t < 0) v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) { a.x += v[e].x; a.y += v[e].y; a.z += v[e].z; a.w += v[e].w; }
             }
         }
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -153,9 +159,35 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
             out = make_float4(vx, vy, vz, a.x);
         }
         gv[idx] = out;
-        nodes += (uint32_t)__popcll(__ballot(out.w != 0.f));
     }
-    if (lane == 0 && nodes) atomicAdd(counters + CT_NODES, nodes);
+}
+
+// Nodes with mass after the grid update (statistics only; launched when
+// timing is on).  One atomic per workgroup: same-address atomics from every
+// wave of the grid update would serialize in L2.
+__global__ void __launch_bounds__(256) k_count_nodes(const uint32_t *list, uint32_t *counters,
+                                                     const float4 *__restrict__ gv)
+{
+    __shared__ uint32_t part[4];
+    const uint32_t nact = counters[CT_ACTIVE];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t nodes = 0;
+    for (uint32_t k = blockIdx.x * 4 + w; k < nact; k += gridDim.x * 4) {
+        const float4 v = gv[(int64_t)list[k] * BLK_NODES + lane];
+        nodes += (uint32_t)__popcll(__ballot(v.w != 0.f));
+    }
+    if (lane == 0) part[w] = nodes;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(counters + CT_NODES, t);
+    }
+}
+
+int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters, const float4 *gv)
+{
+    hipLaunchKernelGGL(k_count_nodes, dim3(256), dim3(256), 0, st, list, counters, gv);
+    return (int)hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
