@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="snow_spheres", choices=["snow_spheres", "jello_cube", "water_dam"])
-    ap.add_argument("--form", default="fused", choices=["fused", "fast", "plain"])
+    ap.add_argument("--form", default="fast", choices=["fused", "fast", "plain"])
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -182,7 +182,7 @@ def main():
     if not a.no_variants:
         ksteps = max(10, a.steps // 4)
         out["variants"] = {}
-        for form in ("fast", "plain"):
+        for form in ("fast", "fused", "plain"):
             if form == a.form:
                 continue
             r = run_gpu(a.workload, form, ksteps, min(a.warmup, 5))

@@ -195,7 +195,9 @@ int mpm_read_grid(mpm_sim *sim, int32_t field, float *out);
 /* Health bits since the last call (cleared on read), and counters. */
 int mpm_health(mpm_sim *sim, uint32_t *bits);
 /* stats[0] steps, [1] active grid blocks of the last step, [2] FP64 SVD
- * fallbacks, [3] warm-start rejections, [4] sorts, [5] time (s). */
+ * fallbacks, [3] warm-start rejections, [4] sorts, [5] time (s), [6] P2G
+ * tiles of the last step, [7] particles of the last P2G that left their
+ * tile's arena and took the atomic fallback. */
 int mpm_stats(mpm_sim *sim, double stats[8]);
 
 int mpm_synchronize(mpm_sim *sim);

@@ -243,7 +243,7 @@ class Sim:
         s = (C.c_double * 8)()
         check(lib().mpm_stats(self._h, s))
         return {"steps": s[0], "active_blocks": s[1], "fp64_fallbacks": s[2], "warm_rejects": s[3],
-                "sorts": s[4], "time": s[5], "tiles": s[6]}
+                "sorts": s[4], "time": s[5], "tiles": s[6], "spilled": s[7]}
 
     def set_timing(self, on: bool = True) -> None:
         check(lib().mpm_set_timing(self._h, 1 if on else 0))

@@ -513,7 +513,7 @@ int mpm_stats(mpm_sim *s, double st[8])
     st[4] = (double)s->sorts;
     st[5] = s->time;
     st[6] = c[CT_TILES];
-    st[7] = 0;
+    st[7] = c[CT_SPILL];
     return MPM_OK;
 }
 

@@ -324,7 +324,7 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
 constexpr int TILE_WAVES = 2;
 constexpr int TILE_THREADS = 64 * TILE_WAVES;
 static_assert(ARENA_NODES % TILE_THREADS == 0, "arena nodes per thread must be whole");
-constexpr int CHUNK = 256;          // particles per pass over a tile
+constexpr int CHUNK = 384;          // particles per pass over a tile (a full 4^3-cell block holds ~320 at 5 per cell)
 constexpr int ITEM_CAP = 4;         // particles per lane item
 constexpr int NBIN = 216;           // 6^3 base positions inside an arena
 constexpr uint32_t NO_BIN = 255u;

@@ -20,6 +20,20 @@ template <class T> MPM_HD T m_sqrt(T x) { return sqrt(x); }
 template <> MPM_HD float m_sqrt<float>(float x) { return sqrtf(x); }
 template <class T> MPM_HD T m_abs(T x) { return x < T(0) ? -x : x; }
 
+// Reciprocal, reciprocal square root and square root for the rotations of
+// the SVD, where a 1-ulp result is enough (the rotation stays orthogonal to
+// working precision).  On the device the FP32 forms are the single hardware
+// instructions (v_rcp_f32, v_rsq_f32, v_sqrt_f32) instead of the correctly
+// rounded multi-instruction sequences; FP64 and host builds divide exactly.
+template <class T> MPM_HD T r_rcp(T x) { return T(1) / x; }
+template <class T> MPM_HD T r_rsqrt(T x) { return T(1) / m_sqrt(x); }
+template <class T> MPM_HD T r_sqrt(T x) { return m_sqrt(x); }
+#if defined(__HIP_DEVICE_COMPILE__)
+template <> MPM_HD float r_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> MPM_HD float r_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> MPM_HD float r_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+#endif
+
 // ---------------------------------------------------------------------------
 // Quadratic B-spline N and N' (reference MPMCUDA.cu:267-290, branchy |x| form
 // with the same half-open intervals), evaluated at the three stencil offsets
@@ -116,11 +130,11 @@ MPM_HD void jacobi_rotate(T B[9], T W[9], int p, int q)
     T a = B[p] * B[p] + B[3 + p] * B[3 + p] + B[6 + p] * B[6 + p];
     T b = B[q] * B[q] + B[3 + q] * B[3 + q] + B[6 + q] * B[6 + q];
     T g = B[p] * B[q] + B[3 + p] * B[3 + q] + B[6 + p] * B[6 + q];
-    T zeta = (b - a) / (T(2) * g);
-    T t = (zeta >= T(0) ? T(1) : T(-1)) / (m_abs(zeta) + m_sqrt(T(1) + zeta * zeta));
+    T zeta = (b - a) * (T(0.5) * r_rcp(g));
+    T t = (zeta >= T(0) ? T(1) : T(-1)) * r_rcp(m_abs(zeta) + r_sqrt(T(1) + zeta * zeta));
     // g == 0 (already orthogonal, or 0/0) and overflowing zeta: no rotation
     if (!(g != T(0)) || !(t == t)) t = T(0);
-    T c = T(1) / m_sqrt(T(1) + t * t), s = c * t;
+    T c = r_rsqrt(T(1) + t * t), s = c * t;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         T bp = B[3 * r + p], bq = B[3 * r + q];
@@ -163,9 +177,9 @@ template <class T>
 MPM_HD void givens_rows(T R[9], T Q[9], int i, int j, int k)
 {
     T a = R[3 * i + k], b = R[3 * j + k];
-    T rr = m_sqrt(a * a + b * b);
+    T rr2 = a * a + b * b;
     T c = T(1), s = T(0);
-    if (rr > T(0)) { c = a / rr; s = b / rr; }
+    if (rr2 > T(0)) { T inv = r_rsqrt(rr2); c = a * inv; s = b * inv; }
 #pragma unroll
     for (int col = 0; col < 3; ++col) {
         T ri = R[3 * i + col], rj = R[3 * j + col];
