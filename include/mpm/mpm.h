@@ -46,10 +46,11 @@ enum { MPM_WATER_C_APIC = 0, MPM_WATER_C_GRADV = 1 };
  * MPM_FORM_PLAIN: unsorted particles, P2G scatters with global atomics, the
  *   stress SVD runs in P2G and (snow) again in G2P -- the structure of
  *   MPMCUDA.cu:310-741 in FP32 SoA.
- * MPM_FORM_FAST: particles cell-block sorted every `sort_every` steps, P2G
- *   accumulates per block in LDS without atomics and is staged for the grid
- *   update, G2P is fused with the stress evaluation so each step needs one
- *   SVD.
+ * MPM_FORM_FAST: particles sorted by 4^3-node block (at most every
+ *   `sort_every` steps, earlier when particles drift out of their block's
+ *   arena), P2G accumulates per block in LDS without atomics and is staged
+ *   for the grid update, G2P is fused with the stress evaluation so each
+ *   step needs one SVD.
  * MPM_FORM_FUSED: as FAST, but the G2P of step n and the P2G of step n+1 run
  *   in one kernel per block (the grid velocities of the block's neighbourhood
  *   in LDS); the stress never goes to memory.  mpm_step_phase runs the FAST
@@ -75,7 +76,8 @@ typedef struct mpm_config {
     int32_t volume_mode;      /* MPM_VOLUME_* */
     uint64_t seed;            /* jitter seed for mpm_add_object */
     int32_t form;             /* MPM_FORM_* */
-    int32_t sort_every;       /* FAST: re-sort particles every k steps */
+    int32_t sort_every;       /* FAST/FUSED: most steps between re-sorts (24); a re-sort
+                                 comes earlier once >0.1% of particles leave their arena */
     int32_t svd_sweeps;       /* FP32 Jacobi sweeps (plain solve) */
     int32_t warm_start;       /* MPM_WARM_* */
     int32_t warm_sweeps;      /* Jacobi sweeps after a warm start */

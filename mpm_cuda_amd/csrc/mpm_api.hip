@@ -148,7 +148,7 @@ void mpm_config_default(mpm_config *c)
     c->volume_mode = MPM_VOLUME_REF;
     c->seed = 20261202ull;
     c->form = MPM_FORM_FAST;
-    c->sort_every = 8;
+    c->sort_every = 24;
     c->svd_sweeps = 4;
     c->warm_start = MPM_WARM_NONE;
     c->warm_sweeps = 1;
@@ -573,6 +573,8 @@ void mpm_destroy(mpm_sim *s)
                     s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
                     s->staging, s->blk_tile, s->spill};
     for (void *p : ptrs) if (p) (void)hipFree(p);
+    if (s->spill_host) (void)hipHostFree(s->spill_host);
+    for (hipEvent_t e : s->spill_ev) if (e) (void)hipEventDestroy(e);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);

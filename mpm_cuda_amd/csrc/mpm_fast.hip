@@ -17,6 +17,8 @@
 #include "mpm_g2p.h"
 #include "mpm_launch.h"
 
+#include <cstring>
+
 namespace mpm {
 
 PartView view_of(mpm_sim *s, int g);
@@ -447,30 +449,47 @@ __device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
         if (v) L.rnk[sl] = (uint16_t)(prev + below);
         __asm__ volatile("" ::: "memory");
     }
-    // lane l owns bins 4l .. 4l+3
-    uint32_t c[4], nit[4], csum = 0, isum = 0;
+    // Lane l owns bins 4l .. 4l+3.  A bin of c particles gives c / ITEM_CAP
+    // full items and one item of the remainder.  Full items come first, then
+    // remainders by decreasing size, so a batch of 64 items mostly holds
+    // items of equal length (fewer idle lanes per round) and the items of one
+    // bin mostly fall into different batches (fewer turns in the flush).
+    uint32_t c[4], nfull = 0, csum = 0, nrem[ITEM_CAP];
+#pragma unroll
+    for (int r = 0; r < ITEM_CAP; ++r) nrem[r] = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int b = 4 * lane + j;
         c[j] = b < NBIN ? L.cnt[b] : 0u;
-        nit[j] = (c[j] + ITEM_CAP - 1) / ITEM_CAP;
         csum += c[j];
-        isum += nit[j];
+        nfull += c[j] / ITEM_CAP;
+#pragma unroll
+        for (int r = 1; r < ITEM_CAP; ++r) nrem[r] += (c[j] % ITEM_CAP) == (uint32_t)r ? 1u : 0u;
     }
     uint32_t crun = wave_incl_scan(csum) - csum;
-    uint32_t irun = wave_incl_scan(isum) - isum;
-    if (lane == 63) L.nitems = irun + isum;
+    const uint32_t fincl = wave_incl_scan(nfull);
+    uint32_t frun = fincl - nfull;
+    uint32_t total = __shfl(fincl, 63);
+    uint32_t rrun[ITEM_CAP];
+#pragma unroll
+    for (int r = ITEM_CAP - 1; r >= 1; --r) {
+        const uint32_t inc = wave_incl_scan(nrem[r]);
+        rrun[r] = total + inc - nrem[r];
+        total += __shfl(inc, 63);
+    }
+    if (lane == 0) L.nitems = total;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int b = 4 * lane + j;
         if (b < NBIN) L.cnt[b] = crun;          // now the bin's first sorted position
-        for (uint32_t q = 0; q < nit[j]; ++q) {
-            const uint32_t first = crun + q * ITEM_CAP;
-            const uint32_t n = min((uint32_t)ITEM_CAP, c[j] - q * ITEM_CAP);
-            L.items[irun + q] = (uint32_t)b | (first << 8) | (n << 20);
-        }
+        const uint32_t nf = c[j] / ITEM_CAP, rm = c[j] % ITEM_CAP;
+        for (uint32_t q = 0; q < nf; ++q)
+            L.items[frun + q] = (uint32_t)b | ((crun + q * ITEM_CAP) << 8) | ((uint32_t)ITEM_CAP << 20);
+        frun += nf;
+#pragma unroll
+        for (int r = 1; r < ITEM_CAP; ++r)
+            if (rm == (uint32_t)r) L.items[rrun[r]++] = (uint32_t)b | ((crun + nf * ITEM_CAP) << 8) | (rm << 20);
         crun += c[j];
-        irun += nit[j];
     }
     __asm__ volatile("" ::: "memory");
     for (int s0 = 0; s0 < nc; s0 += 64) {
@@ -664,7 +683,10 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     __shared__ TileLds L;
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
-    for (uint32_t tt = blockIdx.x; tt < nt; tt += gridDim.x) {
+    const uint32_t nt8 = round_xcd(nt);
+    for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
+        const uint32_t tt = xcd_spread(k8, nt8);
+        if (tt >= nt) continue;
         const uint4 d = desc[tt];
         const TileGeo t = tile_geo(g, d.z);
         const bool staged = d.w < staging_cap;
@@ -717,7 +739,10 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     __shared__ float4 garena[A_SLOTS];
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
-    for (uint32_t tt = blockIdx.x; tt < nt; tt += gridDim.x) {
+    const uint32_t nt8 = round_xcd(nt);
+    for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
+        const uint32_t tt = xcd_spread(k8, nt8);
+        if (tt >= nt) continue;
         const uint4 d = desc[tt];
         const TileGeo t = tile_geo(g, d.z);
         const bool staged = d.w < staging_cap;
@@ -869,20 +894,76 @@ static int sort_all(mpm_sim *s)
     std::swap(s->pid, s->pid_alt);
     s->sorted_valid = true;
     s->steps_since_sort = 0;
+    s->sorted_at_launch = s->tile_launches;
     s->sorts += 1;
+    return MPM_OK;
+}
+
+// Re-sort when the interval is up, or earlier once a tile P2G listed more
+// than 0.01% of the particles (at least 64) for the atomic fallback:
+// particles drift out of their block arenas at a rate set by the flow, and
+// every spilled particle costs 108 global atomics.  The count read is that of
+// the tile launch two back; waiting for its copy never idles the device
+// (a later step is still queued) and bounds how far the host runs ahead.
+// The interval adapts: a re-sort forced by spills after k steps sets the
+// next interval to k - 3 (the count arrives two launches late, and in a
+// steady flow whole layers of particles leave their arenas together); an
+// interval that ends without spills lets the next one grow by a step, up to
+// sort_every.
+static bool sort_due(mpm_sim *s)
+{
+    if (!s->sorted_valid) return true;
+    const int64_t cap = s->cfg.sort_every;
+    if (s->sort_interval <= 0 || s->sort_interval > cap) s->sort_interval = cap;
+    if (s->steps_since_sort >= s->sort_interval) {
+        s->sort_interval = std::min<int64_t>(cap, s->sort_interval + 1);
+        return true;
+    }
+    const int64_t want = s->tile_launches - mpm_sim::SPILL_SLOTS;
+    if (want < s->sorted_at_launch || !s->spill_host) return false;    // counts of the current order only
+    const int slot = (int)(want % mpm_sim::SPILL_SLOTS);
+    if (s->spill_launch[slot] != want) return false;
+    if (hipEventSynchronize(s->spill_ev[slot]) != hipSuccess) return false;
+    s->spill_launch[slot] = -1;
+    uint64_t tot = 0;
+    for (size_t k = 0; k < s->groups.size(); ++k) tot += s->spill_host[slot * 256 + k];
+    if (tot <= std::max<uint64_t>(64, (uint64_t)s->total / 10000)) return false;
+    s->sort_interval = std::max<int64_t>(1, s->steps_since_sort - 3);
+    return true;
+}
+
+// Copy the spill count of group `group`'s tile launch to the host; the
+// launch counter advances after the last group.
+static int note_spills(mpm_sim *s, int group)
+{
+    if (!s->spill_host) {
+        MPM_HIP(hipHostMalloc((void **)&s->spill_host, mpm_sim::SPILL_SLOTS * 256 * sizeof(uint32_t),
+                              hipHostMallocDefault));
+        std::memset(s->spill_host, 0, mpm_sim::SPILL_SLOTS * 256 * sizeof(uint32_t));
+        for (int k = 0; k < mpm_sim::SPILL_SLOTS; ++k)
+            MPM_HIP(hipEventCreateWithFlags(&s->spill_ev[k], hipEventDisableTiming));
+    }
+    const int slot = (int)(s->tile_launches % mpm_sim::SPILL_SLOTS);
+    MPM_HIP(hipMemcpyAsync(s->spill_host + slot * 256 + group, s->counters + CT_SPILL, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s->stream));
+    if (group + 1 == (int)s->groups.size()) {
+        MPM_HIP(hipEventRecord(s->spill_ev[slot], s->stream));
+        s->spill_launch[slot] = s->tile_launches;
+        s->tile_launches += 1;
+    }
     return MPM_OK;
 }
 
 int sort_if_due(mpm_sim *s)
 {
-    if (!s->sorted_valid || s->steps_since_sort >= s->cfg.sort_every) return sort_all(s);
+    if (sort_due(s)) return sort_all(s);
     return MPM_OK;
 }
 
 int fast_prepare(mpm_sim *s)
 {
     int rc = MPM_OK;
-    if (!s->sorted_valid || s->steps_since_sort >= s->cfg.sort_every) {
+    if (sort_due(s)) {
         rc = sort_all(s);
         if (rc) return rc;
     }
@@ -918,7 +999,8 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags);
-    return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;
+    if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
+    return note_spills(s, group);
 }
 
 int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
@@ -943,8 +1025,10 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "g2p2g_block");
-    rc = launch_p2g_spill(s->stream, s->g, m, pv, dt_next, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags);
-    return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;
+    rc = launch_p2g_spill(s->stream, s->g, m, pv, dt_next, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp,
+                          s->flags);
+    if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
+    return note_spills(s, group);
 }
 
 // Stand-alone sort for the probe entry point (own temporaries).

@@ -75,6 +75,15 @@ enum Counter : int {
     CT_COUNT = 16
 };
 
+// Workgroups are dispatched to the 8 XCDs of the device round-robin
+// (workgroup id k runs on XCD k % 8), and each XCD has its own L2.  Work item
+// xcd_spread(k, n8) gives every XCD a contiguous eighth of [0, n8), so
+// neighbouring blocks / particle ranges (which share grid nodes) meet in the
+// same L2.  n8 = round_xcd(n); the map is a bijection of [0, n8).
+constexpr uint32_t NXCD = 8;
+__host__ __device__ inline uint32_t round_xcd(uint32_t n) { return (n + NXCD - 1) / NXCD * NXCD; }
+__host__ __device__ inline uint32_t xcd_spread(uint32_t k, uint32_t n8) { return (k % NXCD) * (n8 / NXCD) + k / NXCD; }
+
 __host__ __device__ inline int64_t node_slot(const GridP &g, int x, int y, int z)
 {
     int64_t b = ((int64_t)(z >> 2) * g.nby + (y >> 2)) * g.nbx + (x >> 2);
@@ -119,6 +128,17 @@ struct mpm_sim {
     float primed_dt = 0.f;                  // FUSED: dt the primed scatter used for the force term
     bool sorted_valid = false;
     int64_t steps_since_sort = 0;
+    // spill counts of the tile P2G launches, copied to pinned host memory;
+    // a re-sort is brought forward when they grow.  Two slots: the count of
+    // the launch two steps back is read (its copy has finished by then, and
+    // waiting for it keeps at most two steps queued ahead of the device).
+    static constexpr int SPILL_SLOTS = 2;
+    uint32_t *spill_host = nullptr;         // [slot][256 groups]
+    hipEvent_t spill_ev[SPILL_SLOTS] = {};
+    int64_t spill_launch[SPILL_SLOTS] = {-1, -1};   // tile launch number held by the slot
+    int64_t tile_launches = 0;
+    int64_t sorted_at_launch = 0;           // tile_launches when the last sort ran
+    int64_t sort_interval = 0;              // learned interval: spills ended the last one after this many steps
     int last_phase = -1;
     // warm-start MLP weights (device)
     float *mlp_w = nullptr;

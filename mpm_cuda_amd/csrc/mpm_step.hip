@@ -100,7 +100,10 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
     const uint32_t nact = counters[CT_ACTIVE];
     const int lane = threadIdx.x;
     const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-    for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
+    const uint32_t nact8 = round_xcd(nact);
+    for (uint32_t k8 = blockIdx.x; k8 < nact8; k8 += gridDim.x) {
+        const uint32_t k = xcd_spread(k8, nact8);
+        if (k >= nact) continue;
         const uint32_t b = __builtin_amdgcn_readfirstlane(list[k]);
         const int bx = (int)(b % (uint32_t)g.nbx);
         const int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
@@ -199,8 +202,10 @@ __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, flo
                                              const float4 *__restrict__ gv, int write_tau,
                                              int sweeps, uint32_t *counters)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pv.n) return;
+    const uint32_t nb = ((uint32_t)pv.n + blockDim.x - 1) / blockDim.x;
+    const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(nb));      // launched with round_xcd(nb) workgroups
+    const int i = (int)(blk * blockDim.x + threadIdx.x);
+    if (blk >= nb || i >= pv.n) return;
     G2POut o;
     GlobalFetch f{gv};
     uint32_t bits = g2p_core<MAT>(g, mat, pv, i, dt, f, write_tau != 0 && MAT != MPM_WATER, sweeps, counters, o);
@@ -308,7 +313,7 @@ int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv
                const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads)
 {
     if (pv.n <= 0) return 0;
-    dim3 gr(nblk(pv.n, threads)), bl(threads);
+    dim3 gr(round_xcd((uint32_t)nblk(pv.n, threads))), bl(threads);
     switch (m.type) {
     case MPM_JELLO: hipLaunchKernelGGL(k_g2p<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
     case MPM_SNOW: hipLaunchKernelGGL(k_g2p<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
