@@ -50,7 +50,8 @@ KERNEL_BYTES = {
     ("water", "fused"): {"g2p": (80, 28)},
 }
 KERNEL_NAME = {("plain", "p2g"): "k_p2g_atomic", ("plain", "g2p"): "k_g2p", ("fast", "p2g"): "k_p2g_block",
-               ("fast", "g2p"): "k_g2p", ("fused", "g2p"): "k_g2p2g_block"}
+               ("fast", "g2p"): "k_g2p", ("fused", "g2p"): "k_g2p2g_block", ("plain", "grid"): "k_grid_update",
+               ("fast", "grid"): "k_grid_update", ("fused", "grid"): "k_grid_update"}
 
 
 def parse():
@@ -102,6 +103,7 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, timing=True):
     sim.run(warmup, dt)
     sim.sync()
     sim.phase_times()          # drop warm-up records
+    sim.kernel_times()
     sim.health()
     sim.set_timing(timing)
     sorts0 = sim.stats()["sorts"]
@@ -110,25 +112,46 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, timing=True):
     sim.sync()
     t1 = time.perf_counter()
     ph = sim.phase_times()
+    kt = sim.kernel_times()
     health = sim.health()
     stats = sim.stats()
     stats["sorts_timed"] = stats["sorts"] - sorts0
     sim.close()
-    return {"n": n, "seconds": t1 - t0, "phases": ph, "health": health, "stats": stats, "info": info}
+    return {"n": n, "seconds": t1 - t0, "phases": ph, "kernels": kt, "health": health, "stats": stats, "info": info}
 
 
-def roofline(res, material, form, steps):
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same
+    counters on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_snow1m.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(res, material, form, steps, workload):
+    """Achieved = algorithmic bytes of the dominant kernel per launch / its
+    average launch time (HIP events around that launch alone)."""
     n = res["n"]
     ph = res["phases"]
+    kt = res["kernels"]
     nodes = max(ph["nodes"], 1)
     model = KERNEL_BYTES[(material, form)]
-    avg = {k: ph[k][0] / max(ph[k][1], 1) for k in ("sort", "p2g", "grid", "g2p")}   # ms per step-phase
+    avg = {k: kt[k][0] / max(kt[k][1], 1) for k in ("p2g", "grid", "g2p")}          # ms per launch
     bytes_per = {k: pb * n + nb * nodes for k, (pb, nb) in model.items()}
     dom = max(model, key=lambda k: avg[k])
     achieved = bytes_per[dom] / (avg[dom] * 1e-3) / 1e9
-    return {"kernel": KERNEL_NAME[(form, dom)], "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_phase_ms": {k: round(v, 4) for k, v in avg.items()},
+    kname = KERNEL_NAME[(form, dom)]
+    traffic = measured_traffic(kname) if workload == "snow_spheres" else None
+    return {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(traffic) if traffic else None,
+            "kernel_avg_ms": {KERNEL_NAME.get((form, k), k): round(v, 4) for k, v in avg.items() if v > 0},
+            "phase_avg_ms": {k: round(ph[k][0] / max(ph[k][1], 1), 4) for k in ("sort", "p2g", "grid", "g2p")},
             "bytes_per_launch": bytes_per[dom], "nodes_with_mass": ph["nodes"],
             "per_kernel_GBs": {KERNEL_NAME[(form, k)]: round(bytes_per[k] / (avg[k] * 1e-3) / 1e9, 1)
                                for k in bytes_per if avg[k] > 0}}
@@ -184,7 +207,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": res["info"]["desc"], "particles": n, "form": a.form, "warm_start": "none",
                    "sort_every": int(res["info"]["cfg"].sort_every), "tile": int(res["info"]["cfg"].tile)},
-        "roofline": roofline(res, material, a.form, a.steps),
+        "roofline": roofline(res, material, a.form, a.steps, a.workload),
         "health_bits": res["health"],
         "fp64_svd_fallbacks": res["stats"]["fp64_fallbacks"],
         "sorts_in_timed_steps": int(res["stats"]["sorts_timed"]),
@@ -198,7 +221,7 @@ def main():
             r = run_gpu(a.workload, form, ksteps, min(a.warmup, 5))
             v = r["n"] * ksteps / r["seconds"]
             out["variants"][form] = {"value": round(v, 1), "ms_per_step": round(r["seconds"] / ksteps * 1e3, 4),
-                                     "roofline": roofline(r, material, form, ksteps)}
+                                     "roofline": roofline(r, material, form, ksteps, a.workload)}
         if "plain" in out["variants"]:
             out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
     if not a.no_cpu_baseline:

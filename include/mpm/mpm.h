@@ -213,6 +213,11 @@ int mpm_synchronize(mpm_sim *sim);
  * with mass in the last grid update. */
 int mpm_set_timing(mpm_sim *sim, int32_t enable);
 int mpm_phase_times(mpm_sim *sim, double ms[4], int64_t counts[4], int64_t *stats_nodes);
+/* The main kernel of each phase alone, timed like the phases (events around
+ * the single launch): [0] P2G (k_p2g_block, or k_p2g_atomic in the PLAIN
+ * form), [1] k_grid_update, [2] G2P (k_g2p, or k_g2p2g_block in the FUSED
+ * form), [3] unused.  Accumulated since the previous call, then cleared. */
+int mpm_kernel_times(mpm_sim *sim, double ms[4], int64_t counts[4]);
 
 /* Simulation clock: current_time / current_frame (MPMSimulator.cuh:86-87). */
 double mpm_current_time(const mpm_sim *sim);

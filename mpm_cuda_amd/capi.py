@@ -81,6 +81,7 @@ _SIGS = {
     "mpm_synchronize": (C.c_int, [C.c_void_p]),
     "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mpm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mpm_current_time": (C.c_double, [C.c_void_p]),
     "mpm_current_frame": (C.c_int64, [C.c_void_p]),
     "mpm_write_bgeo": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64]),
@@ -258,6 +259,15 @@ class Sim:
         out = {names[k]: (ms[k], cnt[k]) for k in range(4)}
         out["nodes"] = nodes.value
         return out
+
+    def kernel_times(self) -> dict:
+        """{kernel: (total_ms, launches)} of the main kernel of each phase
+        (P2G, grid update, G2P) since the previous call."""
+        ms = (C.c_double * 4)()
+        cnt = (C.c_int64 * 4)()
+        check(lib().mpm_kernel_times(self._h, ms, cnt))
+        names = ("p2g", "grid", "g2p")
+        return {names[k]: (ms[k], cnt[k]) for k in range(3)}
 
     def set_warm_start_weights(self, w) -> None:
         w = _f32(w).reshape(-1)

@@ -524,23 +524,46 @@ int mpm_set_timing(mpm_sim *s, int32_t enable)
     return MPM_OK;
 }
 
-int mpm_phase_times(mpm_sim *s, double ms[4], int64_t counts[4], int64_t *stats_nodes)
+// Elapsed times of the recorded event pairs into the per-id accumulators.
+static int drain_timed(mpm_sim *s)
 {
-    if (!s || !ms || !counts) return set_error(MPM_ERR_INVALID, "null argument");
     MPM_HIP(hipStreamSynchronize(s->stream));
-    for (int k = 0; k < 4; ++k) { ms[k] = 0.0; counts[k] = 0; }
     for (auto &t : s->timed) {
         float e = 0.f;
         MPM_HIP(hipEventElapsedTime(&e, t.a, t.b));
-        if (t.phase >= 0 && t.phase < 4) { ms[t.phase] += e; counts[t.phase] += 1; }
+        if (t.phase >= 0 && t.phase < mpm_sim::TIMED_IDS) { s->tacc_ms[t.phase] += e; s->tacc_n[t.phase] += 1; }
         s->event_pool.push_back(t.a);
         s->event_pool.push_back(t.b);
     }
     s->timed.clear();
+    return MPM_OK;
+}
+
+int mpm_phase_times(mpm_sim *s, double ms[4], int64_t counts[4], int64_t *stats_nodes)
+{
+    if (!s || !ms || !counts) return set_error(MPM_ERR_INVALID, "null argument");
+    int rc = drain_timed(s);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k) {
+        ms[k] = s->tacc_ms[k]; counts[k] = s->tacc_n[k];
+        s->tacc_ms[k] = 0.0; s->tacc_n[k] = 0;
+    }
     if (stats_nodes) {
         uint32_t c[CT_COUNT];
         MPM_HIP(hipMemcpy(c, s->counters, sizeof c, hipMemcpyDeviceToHost));
         *stats_nodes = c[CT_NODES];
+    }
+    return MPM_OK;
+}
+
+int mpm_kernel_times(mpm_sim *s, double ms[4], int64_t counts[4])
+{
+    if (!s || !ms || !counts) return set_error(MPM_ERR_INVALID, "null argument");
+    int rc = drain_timed(s);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k) {
+        ms[k] = s->tacc_ms[4 + k]; counts[k] = s->tacc_n[4 + k];
+        s->tacc_ms[4 + k] = 0.0; s->tacc_n[4 + k] = 0;
     }
     return MPM_OK;
 }
@@ -586,7 +609,8 @@ void mpm_destroy(mpm_sim *s)
 namespace mpm {
 
 // Event pair around a phase when timing is on.
-static hipEvent_t take_event(mpm_sim *s)
+
+hipEvent_t take_event(mpm_sim *s)
 {
     hipEvent_t e = nullptr;
     if (!s->event_pool.empty()) {
@@ -597,23 +621,6 @@ static hipEvent_t take_event(mpm_sim *s)
     }
     return e;
 }
-
-struct PhaseTimer {
-    mpm_sim *s;
-    int id;
-    hipEvent_t a = nullptr;
-    PhaseTimer(mpm_sim *s_, int id_) : s(s_), id(id_)
-    {
-        if (s->timing && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
-    }
-    ~PhaseTimer()
-    {
-        if (!a) return;
-        hipEvent_t b = take_event(s);
-        if (b && hipEventRecord(b, s->stream) == hipSuccess) s->timed.push_back({id, a, b});
-        else { s->event_pool.push_back(a); if (b) s->event_pool.push_back(b); }
-    }
-};
 
 // One phase of MPMSimulator::step (MPMCUDA.cu:793-925).
 static int run_phase(mpm_sim *s, int phase, float dt)
@@ -634,7 +641,7 @@ static int run_phase(mpm_sim *s, int phase, float dt)
             PhaseTimer t(s, 1);
             for (size_t k = 0; k < s->groups.size() && !rc; ++k) rc = launch_p2g_tiles(s, (int)k, dt);
         } else {
-            PhaseTimer t(s, 1);
+            PhaseTimer t(s, 1), tk(s, TK_P2G);
             for (size_t k = 0; k < s->groups.size() && !rc; ++k)
                 rc = check_launch(launch_p2g_atomic(s->stream, s->g, s->groups[k].mat, view_of(s, (int)k), dt,
                                                     s->cfg.svd_sweeps, 0, s->grid_mp, s->flags, s->counters, thr),
@@ -648,6 +655,7 @@ static int run_phase(mpm_sim *s, int phase, float dt)
             rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
             if (rc) return rc;
             const bool staged = fast && s->blk_tile;
+            PhaseTimer tk(s, TK_GRID);
             rc = check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,
                                                  (int)s->nblocks, staged ? s->staging : nullptr,
                                                  staged ? s->blk_tile : nullptr, (int)s->groups.size(),
@@ -659,7 +667,7 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         return rc;
     }
     // G2P
-    PhaseTimer t(s, 3);
+    PhaseTimer t(s, 3), tk(s, TK_G2P);
     for (size_t k = 0; k < s->groups.size() && !rc; ++k) {
         if (fast) rc = launch_g2p_fast(s, (int)k, dt);
         else

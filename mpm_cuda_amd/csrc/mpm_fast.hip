@@ -991,10 +991,13 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
     if (rc) return hip_fail((hipError_t)rc, "zero");
     dim3 gr(wg), bl(TILE_THREADS);
+    {
+    PhaseTimer tk(s, TK_P2G);
     switch (m.type) {
     case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
     case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
     default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
@@ -1018,10 +1021,13 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
     if (rc) return hip_fail((hipError_t)rc, "zero");
     dim3 gr(wg), bl(TILE_THREADS);
+    {
+    PhaseTimer tk(s, TK_G2P);
     switch (m.type) {
     case MPM_JELLO: hipLaunchKernelGGL(k_g2p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
     case MPM_SNOW: hipLaunchKernelGGL(k_g2p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
     default: hipLaunchKernelGGL(k_g2p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "g2p2g_block");

@@ -143,7 +143,10 @@ struct mpm_sim {
     // warm-start MLP weights (device)
     float *mlp_w = nullptr;
     int64_t mlp_count = 0;
-    // phase timing (HIP events on `stream`)
+    // phase timing (HIP events on `stream`): ids 0-3 phases, 4-7 kernels
+    static constexpr int TIMED_IDS = 8;
+    double tacc_ms[TIMED_IDS] = {};
+    int64_t tacc_n[TIMED_IDS] = {};
     bool timing = false;
     struct Timed { int phase; hipEvent_t a, b; };
     std::vector<Timed> timed;
@@ -156,6 +159,32 @@ struct mpm_sim {
 };
 
 namespace mpm {
+
+// Timed ids: phases (sort, P2G, grid, G2P) and the main kernel of each.
+enum { TK_P2G = 4, TK_GRID = 5, TK_G2P = 6 };
+
+hipEvent_t take_event(mpm_sim *s);
+
+// Records a HIP event pair around its scope on the simulator's stream when
+// timing is on.
+struct PhaseTimer {
+    mpm_sim *s;
+    int id;
+    hipEvent_t a = nullptr;
+    PhaseTimer(mpm_sim *s_, int id_) : s(s_), id(id_)
+    {
+        if (s->timing && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
+    }
+    ~PhaseTimer()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(s);
+        if (b && hipEventRecord(b, s->stream) == hipSuccess) s->timed.push_back({id, a, b});
+        else { s->event_pool.push_back(a); if (b) s->event_pool.push_back(b); }
+    }
+};
+
+
 int set_error(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 }  // namespace mpm
