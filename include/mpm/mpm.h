@@ -227,8 +227,34 @@ int64_t mpm_current_frame(const mpm_sim *sim);
  * with one "position" attribute. */
 int mpm_write_bgeo(const char *path, const float *xyz, int64_t n);
 
-/* Load MLP warm-start weights (fp32, layout documented in mlp.h). */
+/* ---- learned warm start of the SVD (no reference counterpart; the
+ * reference's SVD always starts from the identity, svd3_cuda.h) ----
+ * A 9 -> 64 -> 64 -> 4 ReLU MLP maps F / |F|_F (row-major) to a quaternion
+ * q; V0 = R(q / |q|) starts the Jacobi SVD of the G2P update (snow trial F,
+ * jello F) with cfg.warm_sweeps sweeps.  A result whose columns of F V are
+ * not orthogonal to 1e-6, or whose F is ill conditioned, is replaced by the
+ * cold SVD (counted in mpm_stats [3]).  Products run in BF16 on the MFMA
+ * units with FP32 accumulation.  Parameters: mpm_mlp_param_count() FP32
+ * values, W1[9][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4], a layer being
+ * y_j = sum_i W[i][j] x_i + b_j. */
+int mpm_mlp_param_count(void);
+
+/* Use `weights` for the warm start (cfg.warm_start = MPM_WARM_MLP). */
 int mpm_set_warm_start_weights(mpm_sim *sim, const float *weights, int64_t count);
+
+/* Network output q[n][4] for F[n][9] (row-major). */
+int mpm_mlp_forward(const float *params, const float *F, int64_t n, float *q);
+
+/* Mean training loss over F[n][9] and its gradient with respect to the
+ * parameters.  loss = sum_{i!=j} (R^T A R)_ij^2 / |A|_F^2 with
+ * A = F^T F / |F|_F^2 and R = R(q / |q|). */
+int mpm_mlp_grad(const float *params, const float *F, int64_t n, float *grad, float *loss);
+
+/* Adam training on the device: `steps` minibatches of `batch` samples drawn
+ * (SplitMix64, `seed`) from F[n][9].  params: initial values in, trained
+ * values out; loss_history[steps] (may be NULL): mean loss per step. */
+int mpm_mlp_train(const float *F, int64_t n, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                  float *params, float *loss_history);
 
 /* MPMSimulator::~MPMSimulator (MPMSimulator.cpp:288-352). */
 void mpm_destroy(mpm_sim *sim);
@@ -248,6 +274,12 @@ int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, i
 /* Device SVD of n row-major 3x3 FP32 matrices: `sweeps` Jacobi sweeps plus
  * the FP64 fallback of the step kernels. U, V row-major, s[3] per matrix. */
 int mpm_probe_svd(const float *F, int64_t n, int32_t sweeps, float *U, float *s, float *V);
+
+/* Largest column orthogonality residual |b_p . b_q| / (|b_p| |b_q|) of
+ * B = F V after `sweeps` Jacobi sweeps started from V0 = R(q / |q|) (q[n][4];
+ * q == NULL: V0 = I), for n row-major FP32 matrices: the acceptance test of
+ * the warm start. */
+int mpm_probe_svd_residual(const float *F, const float *q, int64_t n, int32_t sweeps, float *off);
 
 /* Device stable LSD radix sort of n keys on their low `bits` bits; writes
  * the sorted keys and the permutation (sorted[i] = keys[perm[i]]). */

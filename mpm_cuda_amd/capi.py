@@ -81,6 +81,12 @@ _SIGS = {
     "mpm_synchronize": (C.c_int, [C.c_void_p]),
     "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mpm_mlp_param_count": (C.c_int, []),
+    "mpm_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "mpm_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_float)]),
+    "mpm_mlp_train": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.c_void_p,
+                                C.c_void_p]),
+    "mpm_probe_svd_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mpm_current_time": (C.c_double, [C.c_void_p]),
     "mpm_current_frame": (C.c_int64, [C.c_void_p]),
@@ -306,6 +312,56 @@ def probe_svd(F, sweeps: int = 4):
     check(lib().mpm_probe_svd(F.ctypes.data_as(C.c_void_p), n, int(sweeps), U.ctypes.data_as(C.c_void_p),
                               s.ctypes.data_as(C.c_void_p), V.ctypes.data_as(C.c_void_p)))
     return U.reshape(n, 3, 3), s, V.reshape(n, 3, 3)
+
+
+def probe_svd_residual(F, q=None, sweeps: int = 1):
+    """Column orthogonality residual of F V after `sweeps` Jacobi sweeps from
+    V0 = R(q) (identity when q is None)."""
+    F = _f32(F).reshape(-1, 9)
+    n = F.shape[0]
+    off = np.empty(n, np.float32)
+    qp = None
+    if q is not None:
+        q = _f32(q).reshape(n, 4)
+        qp = q.ctypes.data_as(C.c_void_p)
+    check(lib().mpm_probe_svd_residual(F.ctypes.data_as(C.c_void_p), qp, n, int(sweeps), off.ctypes.data_as(C.c_void_p)))
+    return off
+
+
+def mlp_param_count() -> int:
+    return int(lib().mpm_mlp_param_count())
+
+
+def mlp_forward(params, F):
+    """Warm-start network output q[n][4] (device, BF16 MFMA)."""
+    params = _f32(params).reshape(-1)
+    F = _f32(F).reshape(-1, 9)
+    n = F.shape[0]
+    q = np.empty((n, 4), np.float32)
+    check(lib().mpm_mlp_forward(params.ctypes.data_as(C.c_void_p), F.ctypes.data_as(C.c_void_p), n,
+                                q.ctypes.data_as(C.c_void_p)))
+    return q
+
+
+def mlp_grad(params, F):
+    """(mean loss, gradient) of the warm-start loss over F (device)."""
+    params = _f32(params).reshape(-1)
+    F = _f32(F).reshape(-1, 9)
+    g = np.empty(params.size, np.float32)
+    loss = C.c_float()
+    check(lib().mpm_mlp_grad(params.ctypes.data_as(C.c_void_p), F.ctypes.data_as(C.c_void_p), F.shape[0],
+                             g.ctypes.data_as(C.c_void_p), C.byref(loss)))
+    return float(loss.value), g
+
+
+def mlp_train(F, params, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0):
+    """Adam on the device; returns (trained params, per-step mean loss)."""
+    p = np.array(_f32(params).reshape(-1), copy=True)
+    F = _f32(F).reshape(-1, 9)
+    hist = np.empty(max(steps, 1), np.float32)
+    check(lib().mpm_mlp_train(F.ctypes.data_as(C.c_void_p), F.shape[0], int(steps), int(batch), float(lr), int(seed),
+                              p.ctypes.data_as(C.c_void_p), hist.ctypes.data_as(C.c_void_p)))
+    return p, hist[:steps]
 
 
 def probe_sort(keys, bits: int):

@@ -109,6 +109,7 @@ static MatP make_matp(const mpm_material &m, double mass, double vol)
     p.xi = (float)m.xi;
     p.eos_k = (float)m.eos_k;
     p.eos_gamma = (float)m.eos_gamma;
+    p.warm_sweeps = 0;
     return p;
 }
 
@@ -577,9 +578,12 @@ int mpm_write_bgeo(const char *path, const float *xyz, int64_t n)
     return write_bgeo(path, xyz, n);
 }
 
+extern "C" int mpm_mlp_param_count(void);
+
 int mpm_set_warm_start_weights(mpm_sim *s, const float *w, int64_t count)
 {
     if (!s || !w || count <= 0) return set_error(MPM_ERR_INVALID, "bad argument");
+    if (count != mpm_mlp_param_count()) return set_error(MPM_ERR_INVALID, "warm-start weights: wrong parameter count");
     if (s->mlp_w) { (void)hipFree(s->mlp_w); s->mlp_w = nullptr; }
     MPM_HIP(hipMalloc(&s->mlp_w, (size_t)count * 4));
     MPM_HIP(hipMemcpy(s->mlp_w, w, (size_t)count * 4, hipMemcpyHostToDevice));
@@ -620,6 +624,24 @@ hipEvent_t take_event(mpm_sim *s)
         e = nullptr;
     }
     return e;
+}
+
+int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv);
+
+// Warm start of the G2P SVDs: quaternions of the current F from the network
+// (snow and jello groups), and the sweep count for the G2P kernels.
+static int prepare_warm(mpm_sim *s)
+{
+    const bool on = s->cfg.warm_start == MPM_WARM_MLP && s->mlp_w && s->mlp_count == mpm_mlp_param_count();
+    for (size_t k = 0; k < s->groups.size(); ++k) {
+        MatP &m = s->groups[k].mat;
+        m.warm_sweeps = (on && m.type != MPM_WATER) ? std::max(1, s->cfg.warm_sweeps) : 0;
+        if (m.warm_sweeps > 0) {
+            int rc = check_launch(launch_mlp_infer(s->stream, s->mlp_w, view_of(s, (int)k)), "mlp_infer");
+            if (rc) return rc;
+        }
+    }
+    return MPM_OK;
 }
 
 // One phase of MPMSimulator::step (MPMCUDA.cu:793-925).
@@ -667,6 +689,8 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         return rc;
     }
     // G2P
+    rc = prepare_warm(s);
+    if (rc) return rc;
     PhaseTimer t(s, 3), tk(s, TK_G2P);
     for (size_t k = 0; k < s->groups.size() && !rc; ++k) {
         if (fast) rc = launch_g2p_fast(s, (int)k, dt);
@@ -707,6 +731,8 @@ static int fused_step(mpm_sim *s, float dt)
     }
     if (rc) return rc;
     s->last_phase = MPM_PHASE_G2P;
+    rc = prepare_warm(s);
+    if (rc) return rc;
     {
         PhaseTimer t(s, 3);
         for (size_t k = 0; k < s->groups.size() && !rc; ++k) rc = launch_g2p2g(s, (int)k, dt, dt);

@@ -255,14 +255,14 @@ __global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *val
 // Gathers every permuted SoA plane and the particle ids.
 __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, int64_t cap, int64_t off, int n,
                           const uint32_t *__restrict__ perm, const uint32_t *__restrict__ pid,
-                          uint32_t *__restrict__ pida, uint32_t plane_mask)
+                          uint32_t *__restrict__ pida, uint64_t plane_mask)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t src = perm[j];
     if (src >= (uint32_t)n) src = (uint32_t)j;    // never taken for a valid permutation
     for (int p = 0; p < PL_COUNT; ++p)
-        if ((plane_mask >> p) & 1u) Pa[(int64_t)p * cap + off + j] = P[(int64_t)p * cap + off + src];
+        if ((plane_mask >> p) & 1ull) Pa[(int64_t)p * cap + off + j] = P[(int64_t)p * cap + off + src];
     pida[off + j] = pid[off + src];
 }
 
@@ -860,8 +860,9 @@ static int sort_all(mpm_sim *s)
     hipStream_t st = s->stream;
     int bits = KEY_LOW + 1;
     while ((1ll << (bits - KEY_LOW)) < s->nblocks) ++bits;
-    uint32_t plane_mask = (1u << PL_COUNT) - 1u;
-    if (!s->tau_valid) plane_mask &= ~(((1u << 6) - 1u) << PL_TAU);
+    uint64_t plane_mask = (1ull << PL_COUNT) - 1ull;
+    if (!s->tau_valid) plane_mask &= ~(((1ull << 6) - 1ull) << PL_TAU);
+    plane_mask &= ~(((1ull << 4) - 1ull) << PL_Q);       // recomputed before every G2P
     const int nb = (int)s->nblocks;
     for (size_t k = 0; k < s->groups.size(); ++k) {
         PartView pv = view_of(s, (int)k);

@@ -170,12 +170,13 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
 #pragma unroll
         for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fn[k]);
         if (!(det3<float>(Fn) > 0.f)) bits |= MPM_H_DET_NONPOS;
-        if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters);
+        if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters, &pv, i);
         out.S = 1.0f;
     } else if (MAT == MPM_SNOW) {                                        // :620-709
         float Ft[9], U[9], sg[3], V[9];
         matmul3<float>(A, F, Ft);
-        svd_checked(Ft, U, sg, V, sweeps, counters);
+        if (mat.warm_sweeps > 0) svd_warm(pv, i, Ft, U, sg, V, mat.warm_sweeps, sweeps, counters);
+        else svd_checked(Ft, U, sg, V, sweeps, counters);
         float ratio = 1.0f;
         float sc[3];
 #pragma unroll

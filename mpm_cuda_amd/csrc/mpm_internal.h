@@ -23,7 +23,8 @@ enum Plane : int {
     PL_F = 15,    // 9: (elastic) deformation gradient, row-major
     PL_S = 24,    // 1: snow J_p = det F_p, water J
     PL_TAU = 25,  // 6: Kirchhoff stress (xx,yy,zz,xy,xz,yz) for the next P2G
-    PL_COUNT = 31
+    PL_Q = 31,    // 4: warm-start quaternion of the SVD (written before each G2P)
+    PL_COUNT = 35
 };
 
 // Grid nodes are stored block-major: 4x4x4-node blocks, block index
@@ -51,6 +52,7 @@ struct MatP {
     float mass, vol, mu, lam;
     float theta_c, theta_s, xi;
     float eos_k, eos_gamma;
+    int warm_sweeps;    // > 0: G2P SVDs start from the stored quaternion (PL_Q) with this many sweeps
 };
 
 // Per-launch view of one group's particles.

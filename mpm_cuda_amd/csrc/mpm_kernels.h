@@ -59,14 +59,41 @@ __device__ __forceinline__ void svd_checked(const float F[9], float U[9], float 
     }
 }
 
+// SVD started from the warm-start quaternion stored for particle i: V0 =
+// R(q / |q|), `warm_sweeps` Jacobi sweeps.  The result is accepted when the
+// columns of F V are orthogonal to 1e-6 (largest |cos| between columns) and
+// F is well conditioned; otherwise the cold SVD (with its FP64 fallback)
+// replaces it and the rejection is counted.
+constexpr float kWarmTol = 1e-6f;
+
+__device__ __forceinline__ void svd_warm(const PartView &pv, int i, const float F[9], float U[9], float s[3],
+                                         float V[9], int warm_sweeps, int sweeps, uint32_t *counters)
+{
+    float q[4] = {ld(pv, PL_Q, i), ld(pv, PL_Q + 1, i), ld(pv, PL_Q + 2, i), ld(pv, PL_Q + 3, i)};
+    const float qn2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (qn2 > 1e-20f && qn2 < 1e20f) {
+        const float inv = r_rsqrt(qn2);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] *= inv;
+        float V0[9];
+        quat_to_mat<float>(q, V0);
+        const float off = svd3<float>(F, U, s, V, warm_sweeps, V0);
+        if (off <= kWarmTol && s[2] > kFallbackRatio * s[0]) return;
+    }
+    atomicAdd(counters + CT_REJECT, 1u);
+    svd_checked(F, U, s, V, sweeps, counters);
+}
+
 // Kirchhoff stress of an elastic / snow particle from its deformation
 // gradient (reference P2G, MPMCUDA.cu:329-349).
 template <int MAT>
 __device__ __forceinline__ void stress_from_F(const MatP &m, const float F[9], float Jp, int sweeps,
-                                              float tau6[6], uint32_t *counters)
+                                              float tau6[6], uint32_t *counters, const PartView *warm_pv = nullptr,
+                                              int warm_i = 0)
 {
     float U[9], s[3], V[9];
-    svd_checked(F, U, s, V, sweeps, counters);
+    if (warm_pv && m.warm_sweeps > 0) svd_warm(*warm_pv, warm_i, F, U, s, V, m.warm_sweeps, sweeps, counters);
+    else svd_checked(F, U, s, V, sweeps, counters);
     float mu = m.mu, lam = m.lam;
     if (MAT == MPM_SNOW) {
         float hc = expf(m.xi * (1.0f - Jp));           // MPMCUDA.cu:345

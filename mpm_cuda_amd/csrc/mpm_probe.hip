@@ -31,6 +31,34 @@ __global__ void k_probe_svd(const float *F, int n, int sweeps, float *U, float *
     s[3 * i] = sg[0]; s[3 * i + 1] = sg[1]; s[3 * i + 2] = sg[2];
 }
 
+// Column orthogonality residual of F V after `sweeps` Jacobi sweeps from
+// V0 = R(q / |q|) (q == nullptr: V0 = I).
+__global__ void k_probe_svd_residual(const float *F, const float *q, int n, int sweeps, float *off)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float f[9], u[9], sg[3], v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k] = F[9 * i + k];
+    float r;
+    if (q) {
+        float qq[4] = {q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+        const float qn = sqrtf(qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2] + qq[3] * qq[3]);
+        if (qn > 0.f) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qq[k] /= qn;
+        } else {
+            qq[0] = 1.f; qq[1] = qq[2] = qq[3] = 0.f;
+        }
+        float V0[9];
+        quat_to_mat<float>(qq, V0);
+        r = svd3<float>(f, u, sg, v, sweeps, V0);
+    } else {
+        r = svd3<float>(f, u, sg, v, sweeps);
+    }
+    off[i] = r;
+}
+
 // RAII set of device temporaries for the probes.
 struct DevBufs {
     std::vector<void *> p;
@@ -101,6 +129,28 @@ int mpm_probe_svd(const float *F, int64_t n, int32_t sweeps, float *U, float *s,
     MPM_HIP(hipMemcpy(U, dU, n * 36, hipMemcpyDeviceToHost));
     MPM_HIP(hipMemcpy(V, dV, n * 36, hipMemcpyDeviceToHost));
     MPM_HIP(hipMemcpy(s, ds, n * 12, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_probe_svd_residual(const float *F, const float *q, int64_t n, int32_t sweeps, float *off)
+{
+    if (n > 0 && (!F || !off)) return set_error(MPM_ERR_INVALID, "null argument");
+    if (n <= 0) return MPM_OK;
+    if (sweeps < 0 || sweeps > 16) return set_error(MPM_ERR_INVALID, "sweeps out of range [0,16]");
+    if (n > (1ll << 26)) return set_error(MPM_ERR_INVALID, "too many matrices");
+    DevBufs b;
+    float *dF = nullptr, *dq = nullptr, *doff = nullptr;
+    MPM_HIP(b.alloc(&dF, n * 36));
+    MPM_HIP(b.alloc(&doff, n * 4));
+    MPM_HIP(hipMemcpy(dF, F, n * 36, hipMemcpyHostToDevice));
+    if (q) {
+        MPM_HIP(b.alloc(&dq, n * 16));
+        MPM_HIP(hipMemcpy(dq, q, n * 16, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_probe_svd_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dF, dq, (int)n,
+                       sweeps, doff);
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipMemcpy(off, doff, n * 4, hipMemcpyDeviceToHost));
     return MPM_OK;
 }
 
