@@ -267,6 +267,11 @@ void mpm_destroy(mpm_sim *sim);
  * min(*n, count) positions and stores the count. */
 int mpm_sample_object(const mpm_config *cfg, const mpm_object *obj, float *xyz, int64_t *n);
 
+/* The sampler's point-in-mesh test (ray parity along +z, the rule of the
+ * reference's point_inside_mesh, mesh_query0.1/mesh_query.cpp:287-308) for
+ * n points in the OBJ file's own coordinates; inside[i] = 1 or 0. */
+int mpm_probe_point_in_mesh(const char *obj_path, const double *pts, int64_t n, uint8_t *inside);
+
 /* Device FP32 base-node index max(0, floor((x - origin) * inv_h - 0.5)) of
  * n coordinates, as used by P2G / G2P / the sort keys. */
 int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, int32_t *out);

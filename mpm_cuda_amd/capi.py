@@ -87,6 +87,7 @@ _SIGS = {
     "mpm_mlp_train": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.c_void_p,
                                 C.c_void_p]),
     "mpm_probe_svd_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+    "mpm_probe_point_in_mesh": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mpm_current_time": (C.c_double, [C.c_void_p]),
     "mpm_current_frame": (C.c_int64, [C.c_void_p]),
@@ -326,6 +327,15 @@ def probe_svd_residual(F, q=None, sweeps: int = 1):
         qp = q.ctypes.data_as(C.c_void_p)
     check(lib().mpm_probe_svd_residual(F.ctypes.data_as(C.c_void_p), qp, n, int(sweeps), off.ctypes.data_as(C.c_void_p)))
     return off
+
+
+def probe_point_in_mesh(obj_path: str, pts) -> np.ndarray:
+    """The sampler's inside test for points in the OBJ file's coordinates (host only)."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    out = np.empty(pts.shape[0], np.uint8)
+    check(lib().mpm_probe_point_in_mesh(obj_path.encode(), pts.ctypes.data_as(C.c_void_p), pts.shape[0],
+                                        out.ctypes.data_as(C.c_void_p)))
+    return out.astype(bool)
 
 
 def mlp_param_count() -> int:

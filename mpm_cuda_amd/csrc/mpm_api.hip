@@ -131,6 +131,10 @@ static int fused_step(mpm_sim *s, float dt);
 
 using namespace mpm;
 
+namespace mpm {
+int points_in_mesh(const char *path, const double *pts, int64_t n, uint8_t *inside);
+}
+
 extern "C" {
 
 const char *mpm_last_error(void) { return g_err.c_str(); }
@@ -579,6 +583,12 @@ int mpm_write_bgeo(const char *path, const float *xyz, int64_t n)
 }
 
 extern "C" int mpm_mlp_param_count(void);
+
+int mpm_probe_point_in_mesh(const char *obj_path, const double *pts, int64_t n, uint8_t *inside)
+{
+    if (!obj_path || n < 0 || (n > 0 && (!pts || !inside))) return set_error(MPM_ERR_INVALID, "bad argument");
+    return mpm::points_in_mesh(obj_path, pts, n, inside);
+}
 
 int mpm_set_warm_start_weights(mpm_sim *s, const float *w, int64_t count)
 {

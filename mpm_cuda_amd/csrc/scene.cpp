@@ -197,6 +197,17 @@ static int load_obj(const char *path, Mesh &mesh)
     return MPM_OK;
 }
 
+// Inside test of the sampler for n points given in the OBJ file's own
+// coordinates (the frame in which sample_object tests M^-1 p).
+int points_in_mesh(const char *path, const double *pts, int64_t n, uint8_t *inside)
+{
+    Mesh mesh;
+    int rc = load_obj(path, mesh);
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; ++i) inside[i] = mesh.inside(pts + 3 * i) ? 1 : 0;
+    return MPM_OK;
+}
+
 // ---------------------------------------------------------------------------
 int sample_object(const mpm_config &cfg, const mpm_object &obj, std::vector<float> &xyz)
 {

@@ -47,12 +47,33 @@ class OrcParams(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("bc_pad", C.c_int32), ("bc_mode", C.c_int32)]
 
 
+REF_MESH_QUERY = os.path.join(_HERE, "_ref", "libmesh_query.so")
+_REF_SRC = "/root/reference/thirdparties/mesh_query0.1"
+
+
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile."""
+    """Compile liboracle.so with the committed Makefile, and the reference's
+    mesh_query into _ref/ when the reference sources are present."""
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
                        stdout=subprocess.DEVNULL)
+    if os.path.isdir(_REF_SRC) and (force or not os.path.exists(REF_MESH_QUERY)):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "ref"], check=True, stdout=subprocess.DEVNULL)
     return _LIB_PATH
+
+
+def mesh_query():
+    """ctypes handle of the reference's mesh_query (oracle/_ref), or None."""
+    if not os.path.exists(REF_MESH_QUERY):
+        return None
+    L = C.CDLL(REF_MESH_QUERY)
+    L.construct_mesh_object.restype = C.c_void_p
+    L.construct_mesh_object.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    L.destroy_mesh_object.restype = None
+    L.destroy_mesh_object.argtypes = [C.c_void_p]
+    L.point_inside_mesh.restype = C.c_bool
+    L.point_inside_mesh.argtypes = [C.c_void_p, C.c_void_p]
+    return L
 
 
 def lib():
