@@ -181,7 +181,8 @@ def roofline(res, material, form, steps, workload):
 
 
 def cpu_baseline(workload, steps):
-    """FP64 oracle, one host thread, on the same initial state."""
+    """The FP64 oracle on the host cores: OpenMP over all cores available to
+    this process (at most 16, the host share of one GPU), and one thread."""
     import oracle as O
     from mpm_cuda_amd import workloads
     if workload != "snow_spheres":
@@ -189,17 +190,28 @@ def cpu_baseline(workload, steps):
     cfg, mat, x, v = workloads.snow_spheres_state()
     h = cfg.h
     mu, lam = O.lame(mat.E, mat.nu)
-    g = O.Group(x.astype(np.float64), v.astype(np.float64), type=O.SNOW, mass=mat.rho * h ** 3, vol=h ** 3, mu=mu,
-                lam=lam, theta_c=mat.theta_c, theta_s=mat.theta_s, xi=mat.xi)
-    sim = O.Sim(cfg.nx, cfg.ny, cfg.nz, h, [g], gravity=tuple(cfg.gravity), bc_pad=cfg.bc_pad, bc_mode=cfg.bc_mode)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step(cfg.dt)
-    t1 = time.perf_counter()
     n = len(x)
-    return {"value": round(n * steps / (t1 - t0), 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle (oracle/mpm_oracle.c), all {n:,} particles of the same workload, {steps} steps, "
-                      f"dense FP64 grid as in the reference; {t1 - t0:.1f} s"}
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+
+    def rate(threads, nsteps):
+        O.set_threads(threads)
+        g = O.Group(x.astype(np.float64), v.astype(np.float64), type=O.SNOW, mass=mat.rho * h ** 3, vol=h ** 3,
+                    mu=mu, lam=lam, theta_c=mat.theta_c, theta_s=mat.theta_s, xi=mat.xi)
+        sim = O.Sim(cfg.nx, cfg.ny, cfg.nz, h, [g], gravity=tuple(cfg.gravity), bc_pad=cfg.bc_pad,
+                    bc_mode=cfg.bc_mode)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            sim.step(cfg.dt)
+        return n * nsteps / (time.perf_counter() - t0), time.perf_counter() - t0
+
+    multi, tm = rate(cores, max(steps, 2))
+    single, ts = rate(1, 1)
+    O.set_threads(1)
+    return {"value": round(multi, 1), "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (oracle/mpm_oracle.c, OpenMP, atomic P2G node sums) on all {n:,} particles of "
+                      f"the same workload, {max(steps, 2)} steps in {tm:.1f} s on {cores} threads; one thread: "
+                      f"{single:.0f} particle-steps/s (1 step, {ts:.1f} s); dense FP64 grid as in the reference",
+            "single_thread_value": round(single, 1)}
 
 
 def main():

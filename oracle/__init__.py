@@ -62,6 +62,11 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's P2G / grid / G2P loops (1 = sequential)."""
+    return int(lib().orc_set_threads(int(n)))
+
+
 def mesh_query():
     """ctypes handle of the reference's mesh_query (oracle/_ref), or None."""
     if not os.path.exists(REF_MESH_QUERY):
@@ -89,6 +94,8 @@ def lib():
         L.orc_weight_grad.restype = None; L.orc_weight_grad.argtypes = [_dp, _dp, C.c_double, _dp]
         L.orc_base_ref.restype = C.c_int; L.orc_base_ref.argtypes = [C.c_double, C.c_double, C.c_double]
         L.orc_base_f32.restype = C.c_int; L.orc_base_f32.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.orc_set_threads.restype = C.c_int; L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads(1)          # sequential and deterministic unless asked (set_threads)
         L.orc_block_key.restype = C.c_uint32
         L.orc_block_key.argtypes = [C.c_int] * 5
         L.orc_det3.restype = C.c_double; L.orc_det3.argtypes = [_dp]

@@ -11,6 +11,10 @@
  */
 #include "mpm_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <math.h>
 #include <string.h>
 
@@ -248,6 +252,9 @@ int orc_p2g(orc_grid *g, const orc_group *grp, double dt)
     int flags = 0;
     const orc_material *M = &grp->mat;
     const double h = g->h, m = M->mass, vol = M->vol;
+    /* with more than one thread (orc_set_threads) the node sums are atomic
+     * and their order, hence the last bits, varies between runs */
+#pragma omp parallel for schedule(static) reduction(| : flags)
     for (int64_t i = 0; i < grp->n; ++i) {
         const double *xp = grp->x + 3 * i, *vp = grp->v + 3 * i, *Cp = grp->C + 9 * i;
         double tau[9] = {0};
@@ -274,10 +281,12 @@ int orc_p2g(orc_grid *g, const orc_group *grp, double dt)
                     double xi[3] = {g->origin[0] + ix * h, g->origin[1] + iy * h, g->origin[2] + iz * h};
                     double w = orc_weight(xp, xi, h);
                     int64_t n = gidx(g, ix, iy, iz);
+#pragma omp atomic
                     g->m[n] += w * m;                                            /* :388 */
                     double d[3] = {xi[0] - xp[0], xi[1] - xp[1], xi[2] - xp[2]};
                     for (int c = 0; c < 3; ++c) {                                 /* :391-401 */
                         double cd = Cp[3 * c] * d[0] + Cp[3 * c + 1] * d[1] + Cp[3 * c + 2] * d[2];
+#pragma omp atomic
                         g->p[3 * n + c] += w * m * (vp[c] + cd);
                     }
                     tw += w;
@@ -290,7 +299,10 @@ int orc_p2g(orc_grid *g, const orc_group *grp, double dt)
                         for (int c = 0; c < 3; ++c)
                             f[c] = -vol * (tau[3 * c] * gw[0] + tau[3 * c + 1] * gw[1] + tau[3 * c + 2] * gw[2]);
                     }
-                    for (int c = 0; c < 3; ++c) g->p[3 * n + c] += dt * f[c];    /* :419-421 */
+                    for (int c = 0; c < 3; ++c) {                                 /* :419-421 */
+#pragma omp atomic
+                        g->p[3 * n + c] += dt * f[c];
+                    }
                 }
         if (fabs(tw - 1.0) >= 1e-3) flags |= ORC_FLAG_WEIGHT_SUM;                /* :431 */
     }
@@ -303,6 +315,7 @@ int orc_p2g(orc_grid *g, const orc_group *grp, double dt)
 int orc_grid_update(orc_grid *g, const orc_params *prm, double dt)
 {
     const int pad = prm->bc_pad;
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < g->nz; ++z)
         for (int y = 0; y < g->ny; ++y)
             for (int x = 0; x < g->nx; ++x) {
@@ -331,6 +344,7 @@ int orc_g2p(const orc_grid *g, orc_group *grp, double dt)
     int flags = 0;
     const orc_material *M = &grp->mat;
     const double h = g->h, h2i = 1.0 / (h * h);
+#pragma omp parallel for schedule(static) reduction(| : flags)
     for (int64_t i = 0; i < grp->n; ++i) {
         double *xp = grp->x + 3 * i, *vp = grp->v + 3 * i, *Cp = grp->C + 9 * i;
         int b[3];
@@ -406,4 +420,16 @@ int orc_step(orc_grid *g, orc_group *groups, int ngroups, const orc_params *prm,
     flags |= orc_grid_update(g, prm, dt);
     for (int k = 0; k < ngroups; ++k) flags |= orc_g2p(g, groups + k, dt);
     return flags;
+}
+
+/* Threads of the OpenMP loops (1 = the sequential oracle the tests use). */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+    return n > 0 ? n : 1;
+#else
+    (void)n;
+    return 1;
+#endif
 }

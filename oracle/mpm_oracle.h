@@ -99,6 +99,10 @@ int  orc_grid_update(orc_grid *g, const orc_params *prm, double dt);        /* :
 int  orc_g2p(const orc_grid *g, orc_group *grp, double dt);                 /* :474-741 */
 int  orc_step(orc_grid *g, orc_group *groups, int ngroups, const orc_params *prm, double dt); /* :793-925 */
 
+/* Threads of the OpenMP loops (P2G with atomic node sums, grid update,
+ * G2P); 1 by default.  Returns the count in effect. */
+int orc_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif
