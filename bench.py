@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the plain-form comparison run")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -141,6 +143,73 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0):
     kt = sim.kernel_times()
     sim.close()
     return {"n": n, "seconds": t1 - t0, "phases": ph, "kernels": kt, "health": health, "stats": stats, "info": info}
+
+
+def run_batch(steps, warmup, warm=False, autotune_reps=2, seq_steps=5):
+    """The batched workload: 64 scenes (log-uniform 10k-500k particles, mixed
+    jello / snow / water, 64^3-160^3 grids) advanced together, one launch per
+    stage replayed from hipGraphs, launch widths from the autotuner.  With
+    `warm`, snow scenes use the learned warm start.  Also times every scene
+    stepped on its own (the unbatched path) for comparison."""
+    from mpm_cuda_amd import capi, workloads, warm_start
+    kw = {"warm_start": capi.WARM_MLP, "warm_sweeps": 2} if warm else {}
+    scenes = workloads.batch_scenes(**kw)
+    sims = [s for s, _ in scenes]
+    if warm:
+        w = warm_start.load_weights()
+        for s, info in scenes:
+            if info["type"] == capi.SNOW:
+                s.set_warm_start_weights(w)
+    n = sum(s.total for s in sims)
+    b = capi.Batch(sims)
+    tune = b.autotune(reps=autotune_reps)
+    b.run(warmup)
+    b.sync()
+    st0 = [s.stats() for s in sims]
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.sync()
+    t1 = time.perf_counter()
+    ms = b.run(max(20, steps // 2), timed=True)
+    sh = b.shape
+    b.close()
+    st1 = [s.stats() for s in sims]
+    health = 0
+    for s in sims:
+        health |= s.health()
+    snow = [k for k, (_, info) in enumerate(scenes) if info["type"] == capi.SNOW]
+    n_snow = sum(sims[k].total for k in snow)
+    rejects = sum(st1[k]["warm_rejects"] - st0[k]["warm_rejects"] for k in snow)
+    # each scene on its own (mpm_run on its stream), same state
+    seq = []
+    for s in sims:
+        s.run(1)
+        s.sync()
+        a0 = time.perf_counter()
+        s.run(seq_steps)
+        s.sync()
+        seq.append((time.perf_counter() - a0) / seq_steps * 1e3)
+    for s in sims:
+        s.close()
+    seq = np.array(seq)
+    counts = np.array([info["n"] for _, info in scenes])
+    return {
+        "workload": f"{len(sims)} scenes, log-uniform {counts.min():,}-{counts.max():,} particles "
+                    f"({n:,} total), jello/snow/water, grids 64^3-160^3, FP32",
+        "particles": n, "scenes": len(sims),
+        "value": round(n * steps / (t1 - t0), 1), "unit": "particle-steps/s",
+        "ms_per_step": round((t1 - t0) / steps * 1e3, 4),
+        "step_ms_p50": round(float(np.percentile(ms, 50)), 4), "step_ms_p99": round(float(np.percentile(ms, 99)), 4),
+        "shape": {"p2g_workgroups": sh.p2g_workgroups, "grid_workgroups": sh.grid_workgroups,
+                  "g2p_threads": sh.g2p_threads, "sort_every": sh.sort_every, "graph": bool(sh.use_graph)},
+        "autotune_ms": {k: {str(c): round(v, 4) for c, v in d.items()} for k, d in tune.items()},
+        "warm_start": "snow scenes, 2 sweeps" if warm else "none",
+        "warm_rejected_fraction": round(rejects / max(1, n_snow * steps), 4) if warm else None,
+        "health_bits": health,
+        "unbatched": {"value": round(n / (seq.sum() * 1e-3), 1), "scene_step_ms_p50": round(float(np.percentile(seq, 50)), 4),
+                      "scene_step_ms_p99": round(float(np.percentile(seq, 99)), 4),
+                      "sum_ms_per_step": round(float(seq.sum()), 4)},
+    }
 
 
 def measured_traffic(kernel):
@@ -223,6 +292,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.batch_only:
+        print(json.dumps({"batched": run_batch(a.steps, a.warmup),
+                          "batched_warm": run_batch(a.steps, a.warmup, warm=True)}), flush=True)
+        return
     material = {"snow_spheres": "snow", "jello_cube": "jello", "water_dam": "water"}[a.workload]
     res = run_gpu(a.workload, a.form, a.steps, a.warmup, a.sort_every, a.tile)
     n = res["n"]
@@ -267,6 +340,9 @@ def main():
             "kernel_avg_ms": roofline(r, material, a.form, ksteps, a.workload)["kernel_avg_ms"]}
         if "plain" in out["variants"]:
             out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
+    if not a.no_batch:
+        out["batched"] = run_batch(a.steps, a.warmup)
+        out["batched_warm"] = run_batch(a.steps, a.warmup, warm=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -290,6 +290,52 @@ int mpm_probe_svd_residual(const float *F, const float *q, int64_t n, int32_t sw
  * the sorted keys and the permutation (sorted[i] = keys[perm[i]]). */
 int mpm_probe_sort(const uint32_t *keys, int64_t n, int32_t bits, uint32_t *sorted, uint32_t *perm);
 
+/* ---- Batches of independent scenes ------------------------------------
+ * No counterpart in the reference, which steps one MPMSimulator per process
+ * (main.cpp:202-221).  For callers with many small scenes: every scene of a
+ * batch advances one step (each at its own cfg.dt) with one launch per stage
+ * for the whole batch -- P2G tiles, spill fallback, block compaction, grid
+ * update, G2P -- replayed from a hipGraph; all scenes are re-sorted together
+ * every sort_every steps.  A scene is an mpm_sim in the FAST form holding
+ * exactly one particle group; it may still be read (mpm_read_particles,
+ * mpm_stats, ...) and stepped on its own between batch runs, which are
+ * ordered with its stream.  Destroy the batch before its scenes, and add no
+ * particles to a scene of a live batch. */
+typedef struct mpm_batch mpm_batch;
+
+typedef struct mpm_batch_shape {
+    int32_t p2g_workgroups;   /* width of the persistent P2G launch */
+    int32_t grid_workgroups;  /* width of the persistent grid-update launch */
+    int32_t g2p_threads;      /* G2P workgroup size: 64, 128, 192 or 256 */
+    int32_t sort_every;       /* steps between re-sorts of every scene */
+    int32_t use_graph;        /* 1: replay captured hipGraphs; 0: launch the stages directly */
+} mpm_batch_shape;
+
+void mpm_batch_shape_default(mpm_batch_shape *shape);
+
+/* Group n (1..256) scenes; shape NULL = defaults.  Sorts any scene that was
+ * never sorted and computes stale stresses; does not advance time. */
+int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *shape, mpm_batch **out);
+
+/* Times 4 launch-width candidates of P2G, grid update and G2P on live steps
+ * (advances every scene by 4 * reps steps) and keeps the fastest of each.
+ * ms_out (NULL or 12 floats): median ms per stage (P2G, grid, G2P) and
+ * candidate. */
+int mpm_batch_autotune(mpm_batch *b, int32_t reps, float *ms_out);
+
+int mpm_batch_get_shape(const mpm_batch *b, mpm_batch_shape *out);
+int mpm_batch_set_shape(mpm_batch *b, const mpm_batch_shape *shape);
+
+/* Advance every scene by nsteps steps (asynchronous unless step_ms is
+ * given).  step_ms (NULL or nsteps floats): device time of each batch step,
+ * a re-sort included in the step it precedes. */
+int mpm_batch_run(mpm_batch *b, int32_t nsteps, float *step_ms);
+
+int mpm_batch_synchronize(mpm_batch *b);
+int64_t mpm_batch_particles(const mpm_batch *b);
+int32_t mpm_batch_scenes(const mpm_batch *b);
+void mpm_batch_destroy(mpm_batch *b);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *mpm_last_error(void);
 

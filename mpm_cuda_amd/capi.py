@@ -51,6 +51,11 @@ class MpmObject(C.Structure):
                 ("material", MpmMaterial), ("max_particles", C.c_int64)]
 
 
+class MpmBatchShape(C.Structure):
+    _fields_ = [("p2g_workgroups", C.c_int32), ("grid_workgroups", C.c_int32), ("g2p_threads", C.c_int32),
+                ("sort_every", C.c_int32), ("use_graph", C.c_int32)]
+
+
 class MpmError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mpm error {code}: {msg}")
@@ -99,6 +104,17 @@ _SIGS = {
     "mpm_probe_base_index": (C.c_int, [C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
     "mpm_probe_svd": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_sort": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
+    "mpm_batch_shape_default": (None, [C.POINTER(MpmBatchShape)]),
+    "mpm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(MpmBatchShape),
+                                   C.POINTER(C.c_void_p)]),
+    "mpm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "mpm_batch_get_shape": (C.c_int, [C.c_void_p, C.POINTER(MpmBatchShape)]),
+    "mpm_batch_set_shape": (C.c_int, [C.c_void_p, C.POINTER(MpmBatchShape)]),
+    "mpm_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "mpm_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "mpm_batch_particles": (C.c_int64, [C.c_void_p]),
+    "mpm_batch_scenes": (C.c_int32, [C.c_void_p]),
+    "mpm_batch_destroy": (None, [C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -287,6 +303,85 @@ class Sim:
     @property
     def current_frame(self) -> int:
         return lib().mpm_current_frame(self._h)
+
+
+BATCH_STAGES = ("p2g", "grid", "g2p")
+BATCH_CANDIDATES = {"p2g": (768, 1536, 3072, 8192), "grid": (1024, 2048, 4096, 8192), "g2p": (64, 128, 192, 256)}
+
+
+def batch_shape(**kw) -> MpmBatchShape:
+    sh = MpmBatchShape()
+    lib().mpm_batch_shape_default(C.byref(sh))
+    for k, v in kw.items():
+        setattr(sh, k, v)
+    return sh
+
+
+class Batch:
+    """Owner of an ``mpm_batch*``: independent one-group FAST-form scenes
+    advanced together (the scenes stay owned by their ``Sim`` objects and
+    must outlive the batch)."""
+
+    def __init__(self, sims, shape: MpmBatchShape | None = None):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.handle for s in self.sims])
+        self._h = C.c_void_p()
+        check(lib().mpm_batch_create(arr, len(self.sims), C.byref(shape) if shape is not None else None,
+                                     C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().mpm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Median ms per stage and candidate; keeps the fastest candidate of each stage."""
+        ms = np.zeros(12, np.float32)
+        check(lib().mpm_batch_autotune(self._h, int(reps), ms.ctypes.data_as(C.c_void_p)))
+        ms = ms.reshape(3, 4)
+        return {st: dict(zip(BATCH_CANDIDATES[st], ms[i].tolist())) for i, st in enumerate(BATCH_STAGES)}
+
+    @property
+    def shape(self) -> MpmBatchShape:
+        sh = MpmBatchShape()
+        check(lib().mpm_batch_get_shape(self._h, C.byref(sh)))
+        return sh
+
+    @shape.setter
+    def shape(self, sh: MpmBatchShape) -> None:
+        check(lib().mpm_batch_set_shape(self._h, C.byref(sh)))
+
+    def run(self, nsteps: int, timed: bool = False):
+        """Advance every scene by nsteps; with timed, the device ms of each step."""
+        if not timed:
+            check(lib().mpm_batch_run(self._h, int(nsteps), None))
+            return None
+        ms = np.zeros(max(int(nsteps), 1), np.float32)
+        check(lib().mpm_batch_run(self._h, int(nsteps), ms.ctypes.data_as(C.c_void_p)))
+        return ms[:int(nsteps)]
+
+    def sync(self):
+        check(lib().mpm_batch_synchronize(self._h))
+
+    @property
+    def particles(self) -> int:
+        return lib().mpm_batch_particles(self._h)
+
+    @property
+    def scenes(self) -> int:
+        return lib().mpm_batch_scenes(self._h)
 
 
 def sample_object(cfg: MpmConfig, obj: MpmObject) -> np.ndarray:

@@ -672,6 +672,43 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
     __syncthreads();
 }
 
+// One tile (descriptor d = first particle, count, block, staging slot) of
+// the FAST-form P2G; returns health bits.  All threads of the workgroup.
+template <int MAT>
+__device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, const PartView &pv, const uint4 d,
+                                             uint32_t staging_cap, float dt, float4 *__restrict__ staging,
+                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, TileLds &L)
+{
+    uint32_t bits = 0;
+    const TileGeo t = tile_geo(g, d.z);
+    const bool staged = d.w < staging_cap;
+    tile_clear(L);
+    for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
+        const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
+        for (int s0 = (int)(threadIdx.x & ~63u); s0 < nc; s0 += TILE_THREADS) {
+            const int slot = s0 + (int)(threadIdx.x & 63u);
+            const bool active = slot < nc;
+            const int i = (int)(d.x + c0) + slot;
+            Stencil s;
+            s.b[0] = s.b[1] = s.b[2] = 0;
+            Scatter sc;
+            if (active) {
+                float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+                load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+                // issue all loads before the first use: under the kernel's
+                // register cap the scheduler would otherwise wait for each
+                // load in turn
+                __builtin_amdgcn_sched_group_barrier(0x0020, 21, 0);
+                bits |= make_stencil(g, x, s);
+            }
+            bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
+        }
+        scatter_chunk<MAT>(L, nc, g.inv_h, mat.mass);
+    }
+    tile_end(g, t, L, staging, d.w, staged, flags);
+    return bits;
+}
+
 template <int MAT>
 __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_block(GridP g, MatP mat, PartView pv,
                                                             const uint4 *__restrict__ desc,
@@ -687,36 +724,51 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
         const uint32_t tt = xcd_spread(k8, nt8);
         if (tt >= nt) continue;
-        const uint4 d = desc[tt];
-        const TileGeo t = tile_geo(g, d.z);
-        const bool staged = d.w < staging_cap;
-        tile_clear(L);
-        for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
-            const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
-            for (int s0 = (int)(threadIdx.x & ~63u); s0 < nc; s0 += TILE_THREADS) {
-                const int slot = s0 + (int)(threadIdx.x & 63u);
-                const bool active = slot < nc;
-                const int i = (int)(d.x + c0) + slot;
-                Stencil s;
-                s.b[0] = s.b[1] = s.b[2] = 0;
-                Scatter sc;
-                if (active) {
-                    float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
-                    load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
-                    // issue all loads before the first use: under the kernel's
-                    // register cap the scheduler would otherwise wait for each
-                    // load in turn
-                    __builtin_amdgcn_sched_group_barrier(0x0020, 21, 0);
-                    bits |= make_stencil(g, x, s);
-                }
-                bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
-            }
-            scatter_chunk<MAT>(L, nc, g.inv_h, mat.mass);
-        }
-        tile_end(g, t, L, staging, d.w, staged, flags);
+        bits |= p2g_tile<MAT>(g, mat, pv, desc[tt], staging_cap, dt, staging, spill, flags, counters, L);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
+}
+
+// Batched P2G: the tiles of every scene of a batch in one persistent launch.
+// Work index = the tiles of scene 0, then of scene 1, ...; workgroups walk
+// it with the XCD-aware stride of the single-scene kernel, and each tile's
+// scene selects parameters and material.
+__global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
+{
+    __shared__ TileLds L;
+    // scene map in LDS (registers are at the kernel's cap; 1 KB keeps
+    // three workgroups per CU)
+    __shared__ uint32_t s_end[BATCH_MAX];
+    __shared__ uint32_t s_total;
+    if (threadIdx.x < 64) {
+        SliceScan sc;
+        sc.build(nslices, [&](int s) { return min(*tab[s].tile_count, tab[s].desc_cap); });
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s_end[j * 64 + threadIdx.x] = sc.end[j];
+        if (threadIdx.x == 0) s_total = sc.total;
+    }
+    __syncthreads();
+    const uint32_t nt = s_total, nt8 = round_xcd(nt);
+    const int lane = threadIdx.x & 63;
+    for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
+        const uint32_t tt = xcd_spread(k8, nt8);
+        if (tt >= nt) continue;
+        uint32_t found = 0;
+        for (int j = 0; j * 64 < nslices; ++j) found += (uint32_t)__popcll(__ballot(s_end[j * 64 + lane] <= tt));
+        const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)found);
+        const BatchSlice &b = tab[sid];
+        const uint4 d = b.desc[tt - (sid ? s_end[sid - 1] : 0u)];
+        uint32_t bits;
+        switch (b.mat.type) {
+        case MPM_JELLO: bits = p2g_tile<MPM_JELLO>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L); break;
+        case MPM_SNOW: bits = p2g_tile<MPM_SNOW>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L); break;
+        default: bits = p2g_tile<MPM_WATER>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L); break;
+        }
+        if (bits) atomicOr(b.counters + CT_HEALTH, bits);
+    }
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < nslices; s += blockDim.x) tab[s].counters[CT_TILES] = min(*tab[s].tile_count, tab[s].desc_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -853,7 +905,7 @@ static int ensure_fast_buffers(mpm_sim *s)
     return MPM_OK;
 }
 
-static int sort_all(mpm_sim *s)
+int sort_all(mpm_sim *s)
 {
     int rc = ensure_fast_buffers(s);
     if (rc) return rc;
@@ -978,6 +1030,21 @@ int fast_prepare(mpm_sim *s)
     return MPM_OK;
 }
 
+// Sorted at least once and the stored stress current; no re-sort schedule.
+int fast_ready(mpm_sim *s)
+{
+    int rc = MPM_OK;
+    if (!s->sorted_valid && (rc = sort_all(s))) return rc;
+    if (!s->tau_valid) {
+        for (size_t k = 0; k < s->groups.size(); ++k) {
+            rc = launch_compute_tau(s->stream, s->groups[k].mat, view_of(s, (int)k), s->cfg.svd_sweeps, s->counters);
+            if (rc) return hip_fail((hipError_t)rc, "compute_tau");
+        }
+        s->tau_valid = true;
+    }
+    return MPM_OK;
+}
+
 int launch_p2g_tiles(mpm_sim *s, int group, float dt)
 {
     PartView pv = view_of(s, group);
@@ -1071,6 +1138,36 @@ int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, 
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
     return rc;
+}
+
+// Batch descriptor of a one-group simulator in its current particle order.
+void fill_batch_slice(mpm_sim *s, BatchSlice &b)
+{
+    memset(&b, 0, sizeof b);
+    b.g = s->g;
+    b.mat = s->groups[0].mat;
+    b.pv = view_of(s, 0);
+    b.dt = (float)s->cfg.dt;
+    b.sweeps = s->cfg.svd_sweeps;
+    b.grid_mp = s->grid_mp;
+    b.grid_v = s->grid_v;
+    b.staging = s->staging;
+    b.flags = s->flags;
+    b.list = s->list;
+    b.counters = s->counters;
+    b.spill = s->spill;
+    b.desc = (const uint4 *)s->tile_desc + s->group_tile_off[0];
+    b.tile_count = s->tile_counts;
+    b.blk_tile = s->blk_tile;
+    b.desc_cap = (uint32_t)tile_bound(s, s->groups[0].n);
+    b.staging_cap = (uint32_t)s->staging_cap;
+    b.nblocks = s->nblocks;
+}
+
+int launch_p2g_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs)
+{
+    hipLaunchKernelGGL(k_p2g_batch, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices);
+    return (int)hipGetLastError();
 }
 
 int launch_g2p_fast(mpm_sim *s, int group, float dt)

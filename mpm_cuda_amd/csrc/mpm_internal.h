@@ -99,6 +99,76 @@ struct GroupInfo {
     double mass = 0, vol = 0;
 };
 
+// One scene of a batch (a simulator with one particle group) as the batched
+// kernels read it from device memory: everything a single-scene launch
+// passes as arguments.
+constexpr int BATCH_MAX = 256;
+struct BatchSlice {
+    GridP g;
+    MatP mat;
+    PartView pv;
+    float dt;
+    int sweeps;
+    float4 *grid_mp, *grid_v, *staging;
+    uint32_t *flags, *list, *counters, *spill;
+    const uint4 *desc;              // P2G tiles of the last sort
+    const uint32_t *tile_count;
+    const int32_t *blk_tile;
+    uint32_t desc_cap, staging_cap;
+    int64_t nblocks;
+};
+
+// Inclusive prefix sums of per-scene work counts, held in one wave's
+// registers: lane l keeps scene l + 64 j in slot j (up to BATCH_MAX scenes).
+// Each wave that needs the map builds its own copy; find() turns a global
+// work index into its scene with four ballots.  All 64 lanes must be active.
+struct SliceScan {
+    uint32_t beg[4], end[4];
+    uint32_t total;
+    template <class Count>
+    __device__ __forceinline__ void build(int nslices, Count count)
+    {
+        const int lane = threadIdx.x & 63;
+        uint32_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = j * 64 + lane;
+            if (j * 64 >= nslices) {
+                beg[j] = carry;
+                end[j] = 0xFFFFFFFFu;
+                continue;
+            }
+            const uint32_t c = s < nslices ? count(s) : 0u;
+            uint32_t v = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(v, o);
+                if (lane >= o) v += t;
+            }
+            v += carry;
+            beg[j] = v - c;
+            end[j] = s < nslices ? v : 0xFFFFFFFFu;
+            carry = __shfl(v, 63);
+        }
+        total = carry;
+    }
+    // scene of work index t (t < total)
+    __device__ __forceinline__ uint32_t find(uint32_t t) const
+    {
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s += (uint32_t)__popcll(__ballot(end[j] <= t));
+        return s;
+    }
+    // first work index of scene sid
+    __device__ __forceinline__ uint32_t start(uint32_t sid) const
+    {
+        const uint32_t j = sid >> 6;
+        const uint32_t v = j == 0 ? beg[0] : (j == 1 ? beg[1] : (j == 2 ? beg[2] : beg[3]));
+        return __shfl(v, (int)(sid & 63u));
+    }
+};
+
 }  // namespace mpm
 
 struct mpm_sim {

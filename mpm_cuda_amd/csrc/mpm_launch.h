@@ -25,6 +25,20 @@ int fast_prepare(mpm_sim *s);                 // (re)sort particles and build P2
 int launch_p2g_tiles(mpm_sim *s, int group, float dt);
 int launch_g2p_fast(mpm_sim *s, int group, float dt);
 int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next);   // FUSED form
+int sort_all(mpm_sim *s);                     // sort every group now and rebuild the tiles
+int fast_ready(mpm_sim *s);                   // sorted once, stored stress current
+PartView view_of(mpm_sim *s, int g);
+
+// Batched stages over a BatchSlice table in device memory (mpm_step.hip,
+// mpm_fast.hip); host side in mpm_batch.hip.
+void fill_batch_slice(mpm_sim *s, BatchSlice &b);
+int launch_batch_zero(hipStream_t st, const BatchSlice *tab, int nslices);
+int launch_p2g_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
+int launch_spill_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
+int launch_compact_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
+int launch_grid_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
+int launch_g2p_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs, int threads);
+int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv);
 
 // Scene sampling and files (scene.cpp)
 int sample_object(const mpm_config &cfg, const mpm_object &obj, std::vector<float> &xyz);

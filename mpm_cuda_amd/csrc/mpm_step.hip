@@ -63,11 +63,12 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(GridP g, MatP mat, PartView 
 // ---------------------------------------------------------------------------
 // Active-block list: compacts the block flags written by P2G (and clears them).
 // One global atomic per workgroup (256 flags): wave counts are combined in LDS.
-__global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list,
-                                                        uint32_t *counters)
+// Thread t of the workgroup handles flag b0 + t.
+__device__ __forceinline__ void compact_flags(uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters,
+                                              int b0)
 {
     __shared__ uint32_t wave_cnt[4], wg_base;
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = b0 + (int)threadIdx.x;
     bool f = b < nblocks && flags[b] != 0u;
     if (f) flags[b] = 0u;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -85,6 +86,12 @@ __global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nbl
     if (f) list[off + __popcll(below)] = (uint32_t)b;
 }
 
+__global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list,
+                                                        uint32_t *counters)
+{
+    compact_flags(flags, nblocks, list, counters, (int)(blockIdx.x * blockDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // Grid velocity update over active blocks (MPMCUDA.cu:435-472): v = p / m,
 // gravity, walls.  One wave per active block, persistent over the active
@@ -92,76 +99,83 @@ __global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nbl
 // FAST-form spills) and, in the FAST form, of the staged P2G arenas of the
 // up to 2x2x2 particle blocks whose arenas cover the node.  grid_mp is
 // cleared for the next step; the result goes to grid_v.
+__device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b, float4 *gmp, float4 *gv,
+                                           const float4 *__restrict__ staging, const int32_t *__restrict__ blk_tile,
+                                           int ngroups, uint32_t staging_cap, int64_t nblocks)
+{
+    const int lane = threadIdx.x & 63;
+    const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
+    const int bx = (int)(b % (uint32_t)g.nbx);
+    const int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
+    const int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
+    const int64_t idx = (int64_t)b * BLK_NODES + lane;
+    float4 a = gmp[idx];
+    gmp[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (blk_tile) {
+        for (int gi = 0; gi < ngroups; ++gi) {
+            // lanes 0..26 look up the staging slots of the 27 neighbour
+            // blocks at once; the gathers below are then independent
+            const int32_t *bt = blk_tile + (int64_t)gi * nblocks;
+            int32_t my_t = -1;
+            if (lane < 27) {
+                const int px = bx + lane % 3 - 1, py = by + (lane / 3) % 3 - 1, pz = bz + lane / 9 - 1;
+                if (px >= 0 && py >= 0 && pz >= 0 && px < g.nbx && py < g.nby && pz < g.nbz) {
+                    const int32_t t = bt[((int64_t)pz * g.nby + py) * g.nbx + px];
+                    if (t >= 0 && (uint32_t)t < staging_cap) my_t = t;
+                }
+            }
+            // A node lies in exactly 8 of the 27 arenas: per axis its own
+            // block's (d = 0) and the one below (d = -1, local k <= 2) or
+            // above (d = +1, k = 3).  All 8 loads are issued before any is
+            // used.
+            const int sx = kx == 3 ? 1 : -1, sy = ky == 3 ? 1 : -1, sz = kz == 3 ? 1 : -1;
+            float4 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int dx = (e & 1) ? sx : 0, dy = (e & 2) ? sy : 0, dz = (e & 4) ? sz : 0;
+                const int32_t t = __shfl(my_t, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
+                const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
+                const int64_t ts = t >= 0 ? t : 0;
+                v[e] = staging[ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
+                if (t < 0) v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) { a.x += v[e].x; a.y += v[e].y; a.z += v[e].z; a.w += v[e].w; }
+        }
+    }
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!(fabsf(a.x) < 2.220446049250313e-16f)) {          // MPMCUDA.cu:444
+        const int ix = bx * BLK + kx, iy = by * BLK + ky, iz = bz * BLK + kz;
+        float vx = a.y / a.x + g.gx * dt;                  // MPMCUDA.cu:452-457
+        float vy = a.z / a.x + g.gy * dt;
+        float vz = a.w / a.x + g.gz * dt;
+        bool wx = ix <= g.pad || ix >= g.nx - g.pad;       // MPMCUDA.cu:463-469
+        bool wy = iy <= g.pad || iy >= g.ny - g.pad;
+        bool wz = iz <= g.pad || iz >= g.nz - g.pad;
+        if (g.bc_mode == MPM_BC_STICKY) {
+            if (wx || wy || wz) vx = vy = vz = 0.f;
+        } else {
+            if (wx) vx = 0.f;
+            if (wy) vy = 0.f;
+            if (wz) vz = 0.f;
+        }
+        out = make_float4(vx, vy, vz, a.x);
+    }
+    gv[idx] = out;
+}
+
 __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uint32_t *list, uint32_t *counters,
                                                     float4 *gmp, float4 *gv, const float4 *__restrict__ staging,
                                                     const int32_t *__restrict__ blk_tile, int ngroups,
                                                     uint32_t staging_cap, int64_t nblocks)
 {
     const uint32_t nact = counters[CT_ACTIVE];
-    const int lane = threadIdx.x;
-    const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
     const uint32_t nact8 = round_xcd(nact);
     for (uint32_t k8 = blockIdx.x; k8 < nact8; k8 += gridDim.x) {
         const uint32_t k = xcd_spread(k8, nact8);
         if (k >= nact) continue;
         const uint32_t b = __builtin_amdgcn_readfirstlane(list[k]);
-        const int bx = (int)(b % (uint32_t)g.nbx);
-        const int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
-        const int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
-        const int64_t idx = (int64_t)b * BLK_NODES + lane;
-        float4 a = gmp[idx];
-        gmp[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (blk_tile) {
-            for (int gi = 0; gi < ngroups; ++gi) {
-                // lanes 0..26 look up the staging slots of the 27 neighbour
-                // blocks at once; the gathers below are then independent
-                const int32_t *bt = blk_tile + (int64_t)gi * nblocks;
-                int32_t my_t = -1;
-                if (lane < 27) {
-                    const int px = bx + lane % 3 - 1, py = by + (lane / 3) % 3 - 1, pz = bz + lane / 9 - 1;
-                    if (px >= 0 && py >= 0 && pz >= 0 && px < g.nbx && py < g.nby && pz < g.nbz) {
-                        const int32_t t = bt[((int64_t)pz * g.nby + py) * g.nbx + px];
-                        if (t >= 0 && (uint32_t)t < staging_cap) my_t = t;
-                    }
-                }
-                // A node lies in exactly 8 of the 27 arenas: per axis its own
-                // block's (d = 0) and the one below (d = -1, local k <= 2) or
-                // above (d = +1, k = 3).  All 8 loads are issued before any is
-                // used.
-                const int sx = kx == 3 ? 1 : -1, sy = ky == 3 ? 1 : -1, sz = kz == 3 ? 1 : -1;
-                float4 v[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int dx = (e & 1) ? sx : 0, dy = (e & 2) ? sy : 0, dz = (e & 4) ? sz : 0;
-                    const int32_t t = __shfl(my_t, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
-                    const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
-                    const int64_t ts = t >= 0 ? t : 0;
-                    v[e] = staging[ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
-                    if (t < 0) v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-#pragma unroll
-                for (int e = 0; e < 8; ++e) { a.x += v[e].x; a.y += v[e].y; a.z += v[e].z; a.w += v[e].w; }
-            }
-        }
-        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!(fabsf(a.x) < 2.220446049250313e-16f)) {          // MPMCUDA.cu:444
-            const int ix = bx * BLK + kx, iy = by * BLK + ky, iz = bz * BLK + kz;
-            float vx = a.y / a.x + g.gx * dt;                  // MPMCUDA.cu:452-457
-            float vy = a.z / a.x + g.gy * dt;
-            float vz = a.w / a.x + g.gz * dt;
-            bool wx = ix <= g.pad || ix >= g.nx - g.pad;       // MPMCUDA.cu:463-469
-            bool wy = iy <= g.pad || iy >= g.ny - g.pad;
-            bool wz = iz <= g.pad || iz >= g.nz - g.pad;
-            if (g.bc_mode == MPM_BC_STICKY) {
-                if (wx || wy || wz) vx = vy = vz = 0.f;
-            } else {
-                if (wx) vx = 0.f;
-                if (wy) vy = 0.f;
-                if (wz) vz = 0.f;
-            }
-            out = make_float4(vx, vy, vz, a.x);
-        }
-        gv[idx] = out;
+        grid_block(g, dt, b, gmp, gv, staging, blk_tile, ngroups, staging_cap, nblocks);
     }
 }
 
@@ -198,6 +212,21 @@ int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters,
 // (MPMCUDA.cu:474-741).  With write_tau the Kirchhoff stress of the updated
 // state is stored for the next P2G, so a FAST-form step needs one SVD.
 template <int MAT>
+__device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, const PartView &pv, int i, float dt,
+                                             const float4 *__restrict__ gv, bool write_tau, int sweeps,
+                                             uint32_t *counters)
+{
+    G2POut o;
+    GlobalFetch f{gv};
+    uint32_t bits = g2p_core<MAT>(g, mat, pv, i, dt, f, write_tau && MAT != MPM_WATER, sweeps, counters, o);
+    if (write_tau && MAT != MPM_WATER) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
+    }
+    if (bits) atomicOr(counters + CT_HEALTH, bits);
+}
+
+template <int MAT>
 __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, float dt,
                                              const float4 *__restrict__ gv, int write_tau,
                                              int sweeps, uint32_t *counters)
@@ -206,19 +235,37 @@ __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, flo
     const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(nb));      // launched with round_xcd(nb) workgroups
     const int i = (int)(blk * blockDim.x + threadIdx.x);
     if (blk >= nb || i >= pv.n) return;
-    G2POut o;
-    GlobalFetch f{gv};
-    uint32_t bits = g2p_core<MAT>(g, mat, pv, i, dt, f, write_tau != 0 && MAT != MPM_WATER, sweeps, counters, o);
-    if (write_tau && MAT != MPM_WATER) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
-    }
-    if (bits) atomicOr(counters + CT_HEALTH, bits);
+    g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters);
 }
 
 // Particles whose stencil left their block arena (listed by the block
 // kernels): scatter with global atomics into grid_mp, which the grid update
 // adds in.  Persistent over the list.
+template <int MAT>
+__device__ __forceinline__ void spill_particle(const GridP &g, const MatP &mat, const PartView &pv, int i, float dt,
+                                               uint32_t *counters, float4 *grid, uint32_t *flags)
+{
+    float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
+    Stencil s;
+    make_stencil(g, x, s);
+    Scatter sc;
+    load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                float4 c = node_contrib(g, s, sc, ox, oy, oz);
+                float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
+                unsafeAtomicAdd(dst + 0, c.x);
+                unsafeAtomicAdd(dst + 1, c.y);
+                unsafeAtomicAdd(dst + 2, c.z);
+                unsafeAtomicAdd(dst + 3, c.w);
+            }
+    mark_blocks(g, s, flags);
+}
+
 template <int MAT>
 __global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView pv, float dt,
                                                    const uint32_t *__restrict__ list, uint32_t *counters,
@@ -228,25 +275,7 @@ __global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView p
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int i = (int)list[k];
         if (i < 0 || i >= pv.n) continue;
-        float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
-        Stencil s;
-        make_stencil(g, x, s);
-        Scatter sc;
-        load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
-#pragma unroll
-        for (int ox = 0; ox < 3; ++ox)
-#pragma unroll
-            for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-                for (int oz = 0; oz < 3; ++oz) {
-                    float4 c = node_contrib(g, s, sc, ox, oy, oz);
-                    float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
-                    unsafeAtomicAdd(dst + 0, c.x);
-                    unsafeAtomicAdd(dst + 1, c.y);
-                    unsafeAtomicAdd(dst + 2, c.z);
-                    unsafeAtomicAdd(dst + 3, c.w);
-                }
-        mark_blocks(g, s, flags);
+        spill_particle<MAT>(g, mat, pv, i, dt, counters, grid, flags);
     }
 }
 
@@ -264,6 +293,95 @@ __global__ void k_compute_tau(MatP mat, PartView pv, int sweeps, uint32_t *count
     stress_from_F<MAT>(mat, F, Jp, sweeps, t, counters);
 #pragma unroll
     for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, t[k]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched kernels: the scenes of a batch (BatchSlice table in device memory,
+// one particle group per scene) in one launch per stage.  A workgroup maps
+// its work index to a scene through a SliceScan of the per-scene counts,
+// reads that scene's parameters with uniform loads and runs the body of the
+// single-scene kernel.  Materials are selected per scene at run time.
+
+__global__ void k_batch_zero(const BatchSlice *__restrict__ tab, int nslices)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslices) return;
+    uint32_t *c = tab[s].counters;
+    c[CT_ACTIVE] = 0u;
+    c[CT_NODES] = 0u;
+    c[CT_SPILL] = 0u;
+}
+
+template <int MAT>
+__device__ __forceinline__ void spill_scene(const BatchSlice &b, uint32_t n, uint32_t tid, uint32_t stride)
+{
+    for (uint32_t k = tid; k < n; k += stride) {
+        const int i = (int)b.spill[k];
+        if (i < 0 || i >= b.pv.n) continue;
+        spill_particle<MAT>(b.g, b.mat, b.pv, i, b.dt, b.counters, b.grid_mp, b.flags);
+    }
+}
+
+// Spilled particles of every scene, scene after scene.
+__global__ void __launch_bounds__(256) k_spill_batch(const BatchSlice *__restrict__ tab, int nslices)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int sid = 0; sid < nslices; ++sid) {
+        const BatchSlice &b = tab[sid];
+        const uint32_t n = min(b.counters[CT_SPILL], (uint32_t)b.pv.n);
+        if (n == 0u) continue;
+        switch (b.mat.type) {
+        case MPM_JELLO: spill_scene<MPM_JELLO>(b, n, tid, stride); break;
+        case MPM_SNOW: spill_scene<MPM_SNOW>(b, n, tid, stride); break;
+        default: spill_scene<MPM_WATER>(b, n, tid, stride); break;
+        }
+    }
+}
+
+// Active-block lists: workgroup w compacts 256 flags of one scene.
+__global__ void __launch_bounds__(256) k_compact_batch(const BatchSlice *__restrict__ tab, int nslices)
+{
+    SliceScan sc;
+    sc.build(nslices, [&](int s) { return (uint32_t)((tab[s].nblocks + 255) / 256); });
+    if (blockIdx.x >= sc.total) return;
+    const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(blockIdx.x));
+    const BatchSlice &b = tab[sid];
+    compact_flags(b.flags, (int)b.nblocks, b.list, b.counters, (int)((blockIdx.x - sc.start(sid)) * 256u));
+}
+
+// Grid update over the active blocks of every scene (one wave per block).
+__global__ void __launch_bounds__(64) k_grid_batch(const BatchSlice *__restrict__ tab, int nslices)
+{
+    SliceScan sc;
+    sc.build(nslices, [&](int s) { return tab[s].counters[CT_ACTIVE]; });
+    const uint32_t n = sc.total, n8 = round_xcd(n);
+    for (uint32_t k8 = blockIdx.x; k8 < n8; k8 += gridDim.x) {
+        const uint32_t k = xcd_spread(k8, n8);
+        if (k >= n) continue;
+        const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(k));
+        const BatchSlice &b = tab[sid];
+        const uint32_t blk = (uint32_t)__builtin_amdgcn_readfirstlane((int)b.list[k - sc.start(sid)]);
+        grid_block(b.g, b.dt, blk, b.grid_mp, b.grid_v, b.staging, b.blk_tile, 1, b.staging_cap, b.nblocks);
+    }
+}
+
+// G2P of every particle of every scene; each scene's particles fill whole
+// workgroups.  Launched with round_xcd(sum of ceil(n / blockDim)) workgroups.
+__global__ void __launch_bounds__(256) k_g2p_batch(const BatchSlice *__restrict__ tab, int nslices)
+{
+    SliceScan sc;
+    sc.build(nslices, [&](int s) { return ((uint32_t)tab[s].pv.n + blockDim.x - 1) / blockDim.x; });
+    const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(sc.total));
+    if (blk >= sc.total) return;
+    const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(blk));
+    const BatchSlice &b = tab[sid];
+    const int i = (int)((blk - sc.start(sid)) * blockDim.x + threadIdx.x);
+    if (i >= b.pv.n) return;
+    switch (b.mat.type) {
+    case MPM_JELLO: g2p_particle<MPM_JELLO>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters); break;
+    case MPM_SNOW: g2p_particle<MPM_SNOW>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters); break;
+    default: g2p_particle<MPM_WATER>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters); break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -343,6 +461,36 @@ int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sw
         hipLaunchKernelGGL(k_compute_tau<MPM_JELLO>, gr, bl, 0, st, m, pv, sweeps, counters);
     else
         hipLaunchKernelGGL(k_compute_tau<MPM_SNOW>, gr, bl, 0, st, m, pv, sweeps, counters);
+    return (int)hipGetLastError();
+}
+
+int launch_batch_zero(hipStream_t st, const BatchSlice *tab, int nslices)
+{
+    hipLaunchKernelGGL(k_batch_zero, dim3(nblk(nslices, 64)), dim3(64), 0, st, tab, nslices);
+    return (int)hipGetLastError();
+}
+
+int launch_spill_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs)
+{
+    hipLaunchKernelGGL(k_spill_batch, dim3(wgs), dim3(256), 0, st, tab, nslices);
+    return (int)hipGetLastError();
+}
+
+int launch_compact_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs)
+{
+    hipLaunchKernelGGL(k_compact_batch, dim3(wgs), dim3(256), 0, st, tab, nslices);
+    return (int)hipGetLastError();
+}
+
+int launch_grid_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs)
+{
+    hipLaunchKernelGGL(k_grid_batch, dim3(wgs), dim3(64), 0, st, tab, nslices);
+    return (int)hipGetLastError();
+}
+
+int launch_g2p_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs, int threads)
+{
+    hipLaunchKernelGGL(k_g2p_batch, dim3(wgs), dim3(threads), 0, st, tab, nslices);
     return (int)hipGetLastError();
 }
 

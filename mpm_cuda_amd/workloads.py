@@ -12,6 +12,14 @@
   0.5 x 0.4 x 0.5 block in the lower-left corner (inside the 10-cell wall
   band), 2,000,000 particles, k = 1e4, gamma = 7, C = grad v, dt = 5e-5.
 
+* ``batch_scenes``  -- 64 independent scenes for the batched form: particle
+  counts log-uniform in [10,000, 500,000] (seeded), materials cycling jello /
+  snow / water, grids of 64^3 to 160^3 nodes (dx = 1/N, 3-cell walls).
+  Jello: a cube dropped at 2 m/s onto a sticky floor (E = 5e4, nu = 0.3);
+  snow: a ball thrown at (2, -3, 0) m/s (the snow material above); water: a
+  dam-break block in the lower corner (k = 1e4, gamma = 7, dt = 5e-5).  8
+  samples per cell, trimmed to the drawn count.
+
 Each builder returns ``(sim, info)`` where ``info`` holds the arrays needed
 to rebuild the same state elsewhere (positions, velocities, material).
 """
@@ -106,6 +114,59 @@ def water_dam(form=capi.FORM_FAST, n_grid=256, n_particles=2_000_000, **cfg_kw):
     sim.add_particles(mat, x, volume=cfg.h ** 3)
     return sim, {"cfg": cfg, "mat": mat, "x": x, "v": np.zeros_like(x), "name": "water_dam",
                  "desc": f"3D water dam break, {len(x):,} particles, {n_grid}^3 grid, dt=5e-5, FP32"}
+
+
+BATCH_GRIDS = (64, 80, 96, 112, 128, 144, 160)
+
+
+def batch_plan(n_scenes=64, seed=SEED0 + 5, n_min=10_000, n_max=500_000):
+    """(material, grid, particles) of every scene of the batched workload."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_scenes)).astype(np.int64)
+    plan = []
+    for k, n in enumerate(counts):
+        side = (n / 8.0) ** (1.0 / 3.0)                  # cells of a cube holding n at 8 per cell
+        fits = [N for N in BATCH_GRIDS if N - 8 >= 1.9 * side]
+        plan.append(((capi.JELLO, capi.SNOW, capi.WATER)[k % 3], int(rng.choice(fits)), int(n)))
+    return plan
+
+
+def batch_scene(type_, n_grid, n_particles, seed, form=capi.FORM_FAST, **cfg_kw):
+    h = 1.0 / n_grid
+    pad = 3
+    dt = 5e-5 if type_ == capi.WATER else 1e-4
+    cfg = capi.default_config(h=h, nx=n_grid, ny=n_grid, nz=n_grid, dt=dt, ppc=8, seed=seed, bc_pad=pad,
+                              bc_mode=capi.BC_STICKY if type_ == capi.JELLO else capi.BC_AXIS, form=form)
+    for k, val in cfg_kw.items():
+        setattr(cfg, k, val)
+    side = 1.12 * (n_particles / 8.0) ** (1.0 / 3.0) * h    # sampled a little larger, then trimmed
+    lo_wall = (pad + 2) * h
+    v = (0.0, 0.0, 0.0)
+    if type_ == capi.JELLO:
+        mat = capi.material(5e4, 0.3, 1e3, capi.JELLO)
+        lo = (0.5 - side / 2, lo_wall + 0.1 * (1 - side), 0.5 - side / 2)
+        x = _box(cfg, lo, tuple(a + side for a in lo), mat, max_particles=n_particles)
+        v = (0.0, -2.0, 0.0)
+    elif type_ == capi.SNOW:
+        mat = capi.material(1.4e5, 0.2, 400.0, capi.SNOW, theta_c=2.5e-2, theta_s=7.5e-3, xi=10.0)
+        r = side * (3.0 / (4.0 * np.pi)) ** (1.0 / 3.0) * 1.05
+        x = _sphere(cfg, (0.45, 0.55, 0.5), r, mat, n_particles)
+        v = (2.0, -3.0, 0.0)
+    else:
+        mat = capi.material(1e4, 0.2, 1e3, capi.WATER, eos_k=1e4, eos_gamma=7.0, water_c_mode=capi.WATER_C_GRADV)
+        # 0.5 : 0.4 : 0.5 block of the same volume
+        a = side / (0.1 ** (1.0 / 3.0))
+        lo = (lo_wall, lo_wall, lo_wall)
+        x = _box(cfg, lo, (lo_wall + 0.5 * a, lo_wall + 0.4 * a, lo_wall + 0.5 * a), mat, max_particles=n_particles)
+    sim = capi.Sim(cfg)
+    sim.add_particles(mat, x, v0=v, volume=cfg.h ** 3)
+    return sim, {"cfg": cfg, "mat": mat, "n": len(x), "type": type_, "grid": n_grid, "dt": dt}
+
+
+def batch_scenes(n_scenes=64, seed=SEED0 + 5, form=capi.FORM_FAST, **cfg_kw):
+    """The batched workload: a list of (sim, info)."""
+    return [batch_scene(t, N, n, seed + 17 * k, form=form, **cfg_kw)
+            for k, (t, N, n) in enumerate(batch_plan(n_scenes, seed))]
 
 
 WORKLOADS = {"snow_spheres": snow_spheres, "jello_cube": jello_cube, "water_dam": water_dam}
