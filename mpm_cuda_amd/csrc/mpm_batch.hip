@@ -4,9 +4,10 @@
 // Each scene is an mpm_sim with one particle group in the FAST form.  The
 // batch keeps a table of the scenes' parameters (BatchSlice) in device memory
 // and advances every scene by one step with one launch per stage for the
-// whole batch: counters, P2G tiles, atomic spill fallback, active-block
-// compaction, grid update, warm-start inference (one launch per warm scene)
-// and G2P.  That sequence is recorded once into a hipGraph and replayed.
+// whole batch -- counters, atomic spill fallback, active-block compaction,
+// grid update -- and one per stage and material for P2G tiles and G2P
+// (each a persistent launch over all scenes of that material), plus one
+// warm-start inference launch per warm scene.  That sequence is recorded once into a hipGraph and replayed.
 // Every `sort_every` steps all scenes are re-sorted by a second graph with
 // one branch per scene, on the scenes' own streams.  Sorting moves particle
 // storage between two buffers, so there is one step graph and one sort graph
@@ -101,11 +102,20 @@ void set_parity(mpm_batch *b, size_t k, int p)
     s->pid_alt = p ? b->pid0[k] : b->pid1[k];
 }
 
-int g2p_wgs(const mpm_batch *b, int threads)
+// G2P workgroups of the scenes of material `mat` (0: none of them).
+int g2p_wgs(const mpm_batch *b, int mat, int threads)
 {
     uint32_t w = 0;
-    for (mpm_sim *s : b->sims) w += (uint32_t)((s->groups[0].n + threads - 1) / threads);
+    for (mpm_sim *s : b->sims)
+        if (s->groups[0].mat.type == mat) w += (uint32_t)((s->groups[0].n + threads - 1) / threads);
     return (int)round_xcd(w);
+}
+
+bool has_material(const mpm_batch *b, int mat)
+{
+    for (mpm_sim *s : b->sims)
+        if (s->groups[0].mat.type == mat) return true;
+    return false;
 }
 
 int check(int rc, const char *what) { return rc ? hip_fail((hipError_t)rc, what) : MPM_OK; }
@@ -120,7 +130,8 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, hipEve
     int rc;
     if ((rc = check(launch_batch_zero(st, tab, n), "batch_zero"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[0], st));
-    if ((rc = check(launch_p2g_batch(st, tab, n, p2g), "p2g_batch"))) return rc;
+    for (int m = MPM_JELLO; m <= MPM_WATER; ++m)
+        if (has_material(b, m) && (rc = check(launch_p2g_batch(st, m, tab, n, p2g), "p2g_batch"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[1], st));
     if ((rc = check(launch_spill_batch(st, tab, n, SPILL_WGS), "spill_batch"))) return rc;
     if ((rc = check(launch_compact_batch(st, tab, n, b->compact_wgs), "compact_batch"))) return rc;
@@ -136,7 +147,10 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, hipEve
         if ((rc = check(launch_mlp_infer(st, s->mlp_w, pv), "mlp_infer"))) return rc;
     }
     if (ev) MPM_HIP(hipEventRecord(ev[4], st));
-    if ((rc = check(launch_g2p_batch(st, tab, n, g2p_wgs(b, g2p_threads), g2p_threads), "g2p_batch"))) return rc;
+    for (int m = MPM_JELLO; m <= MPM_WATER; ++m) {
+        const int wgs = g2p_wgs(b, m, g2p_threads);
+        if (wgs && (rc = check(launch_g2p_batch(st, m, tab, n, wgs, g2p_threads), "g2p_batch"))) return rc;
+    }
     if (ev) MPM_HIP(hipEventRecord(ev[5], st));
     return MPM_OK;
 }

@@ -730,10 +730,12 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
 }
 
-// Batched P2G: the tiles of every scene of a batch in one persistent launch.
-// Work index = the tiles of scene 0, then of scene 1, ...; workgroups walk
-// it with the XCD-aware stride of the single-scene kernel, and each tile's
-// scene selects parameters and material.
+// Batched P2G: the tiles of every scene of material MAT in a batch in one
+// persistent launch (one launch per material: a run-time material switch
+// over the three tile bodies doubles the tile time at this register cap).
+// Work index = the tiles of the first such scene, then of the next, ...;
+// workgroups walk it with the XCD-aware stride of the single-scene kernel.
+template <int MAT>
 __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
     __shared__ TileLds L;
@@ -743,7 +745,9 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     __shared__ uint32_t s_total;
     if (threadIdx.x < 64) {
         SliceScan sc;
-        sc.build(nslices, [&](int s) { return min(*tab[s].tile_count, tab[s].desc_cap); });
+        sc.build(nslices, [&](int s) {
+            return tab[s].mat.type == MAT ? min(*tab[s].tile_count, tab[s].desc_cap) : 0u;
+        });
 #pragma unroll
         for (int j = 0; j < 4; ++j) s_end[j * 64 + threadIdx.x] = sc.end[j];
         if (threadIdx.x == 0) s_total = sc.total;
@@ -759,16 +763,13 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
         const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)found);
         const BatchSlice &b = tab[sid];
         const uint4 d = b.desc[tt - (sid ? s_end[sid - 1] : 0u)];
-        uint32_t bits;
-        switch (b.mat.type) {
-        case MPM_JELLO: bits = p2g_tile<MPM_JELLO>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L); break;
-        case MPM_SNOW: bits = p2g_tile<MPM_SNOW>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L); break;
-        default: bits = p2g_tile<MPM_WATER>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L); break;
-        }
+        const uint32_t bits =
+            p2g_tile<MAT>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L);
         if (bits) atomicOr(b.counters + CT_HEALTH, bits);
     }
     if (blockIdx.x == 0)
-        for (int s = threadIdx.x; s < nslices; s += blockDim.x) tab[s].counters[CT_TILES] = min(*tab[s].tile_count, tab[s].desc_cap);
+        for (int s = threadIdx.x; s < nslices; s += blockDim.x)
+            if (tab[s].mat.type == MAT) tab[s].counters[CT_TILES] = min(*tab[s].tile_count, tab[s].desc_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1164,9 +1165,13 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
     b.nblocks = s->nblocks;
 }
 
-int launch_p2g_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs)
+int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, int nslices, int wgs)
 {
-    hipLaunchKernelGGL(k_p2g_batch, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices);
+    switch (mat) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_batch<MPM_JELLO>, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_batch<MPM_SNOW>, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices); break;
+    default: hipLaunchKernelGGL(k_p2g_batch<MPM_WATER>, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices); break;
+    }
     return (int)hipGetLastError();
 }
 

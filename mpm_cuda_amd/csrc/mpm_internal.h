@@ -73,7 +73,7 @@ enum Counter : int {
     CT_FALLBACK = 4,    // FP64 SVD fallbacks
     CT_REJECT = 5,      // warm-start rejections
     CT_TILES = 6,       // P2G tiles (FAST / FUSED forms)
-    CT_SPILLED = 7,     // particles that took the atomic fallback (accumulated)
+    CT_SPILLED = 7,     // reserved
     CT_COUNT = 16
 };
 

@@ -365,23 +365,24 @@ __global__ void __launch_bounds__(64) k_grid_batch(const BatchSlice *__restrict_
     }
 }
 
-// G2P of every particle of every scene; each scene's particles fill whole
-// workgroups.  Launched with round_xcd(sum of ceil(n / blockDim)) workgroups.
+// G2P of every particle of the scenes of material MAT (one launch per
+// material, as the batched P2G); each scene's particles fill whole
+// workgroups.  Launched with round_xcd(sum of ceil(n / blockDim)) workgroups
+// over those scenes.
+template <int MAT>
 __global__ void __launch_bounds__(256) k_g2p_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
     SliceScan sc;
-    sc.build(nslices, [&](int s) { return ((uint32_t)tab[s].pv.n + blockDim.x - 1) / blockDim.x; });
+    sc.build(nslices, [&](int s) {
+        return tab[s].mat.type == MAT ? ((uint32_t)tab[s].pv.n + blockDim.x - 1) / blockDim.x : 0u;
+    });
     const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(sc.total));
     if (blk >= sc.total) return;
     const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(blk));
     const BatchSlice &b = tab[sid];
     const int i = (int)((blk - sc.start(sid)) * blockDim.x + threadIdx.x);
     if (i >= b.pv.n) return;
-    switch (b.mat.type) {
-    case MPM_JELLO: g2p_particle<MPM_JELLO>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters); break;
-    case MPM_SNOW: g2p_particle<MPM_SNOW>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters); break;
-    default: g2p_particle<MPM_WATER>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters); break;
-    }
+    g2p_particle<MAT>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,9 +489,13 @@ int launch_grid_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wg
     return (int)hipGetLastError();
 }
 
-int launch_g2p_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs, int threads)
+int launch_g2p_batch(hipStream_t st, int mat, const BatchSlice *tab, int nslices, int wgs, int threads)
 {
-    hipLaunchKernelGGL(k_g2p_batch, dim3(wgs), dim3(threads), 0, st, tab, nslices);
+    switch (mat) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_g2p_batch<MPM_JELLO>, dim3(wgs), dim3(threads), 0, st, tab, nslices); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_g2p_batch<MPM_SNOW>, dim3(wgs), dim3(threads), 0, st, tab, nslices); break;
+    default: hipLaunchKernelGGL(k_g2p_batch<MPM_WATER>, dim3(wgs), dim3(threads), 0, st, tab, nslices); break;
+    }
     return (int)hipGetLastError();
 }
 
