@@ -5,9 +5,9 @@
 // batch keeps a table of the scenes' parameters (BatchSlice) in device memory
 // and advances every scene by one step with one launch per stage for the
 // whole batch -- counters, atomic spill fallback, active-block compaction,
-// grid update -- and one per stage and material for P2G tiles and G2P
-// (each a persistent launch over all scenes of that material), plus one
-// warm-start inference launch per warm scene.  That sequence is recorded once into a hipGraph and replayed.
+// grid update -- and one per stage and material for P2G tiles (per 64
+// scenes) and G2P, each a persistent launch over the scenes of that
+// material, plus one warm-start inference launch per warm scene.  That sequence is recorded once into a hipGraph and replayed.
 // Every `sort_every` steps all scenes are re-sorted by a second graph with
 // one branch per scene, on the scenes' own streams.  Sorting moves particle
 // storage between two buffers, so there is one step graph and one sort graph
@@ -64,6 +64,7 @@ struct mpm_batch {
     int64_t steps_since_sort = 0;
     int64_t total = 0;
     int compact_wgs = 0;
+    std::vector<int> types;             // material of each scene
     hipEvent_t ev_fork = nullptr, ev_done = nullptr;
     std::vector<hipEvent_t> ev_join;
     std::vector<hipEvent_t> tev;        // step timing events
@@ -111,12 +112,6 @@ int g2p_wgs(const mpm_batch *b, int mat, int threads)
     return (int)round_xcd(w);
 }
 
-bool has_material(const mpm_batch *b, int mat)
-{
-    for (mpm_sim *s : b->sims)
-        if (s->groups[0].mat.type == mat) return true;
-    return false;
-}
 
 int check(int rc, const char *what) { return rc ? hip_fail((hipError_t)rc, what) : MPM_OK; }
 
@@ -131,7 +126,7 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, hipEve
     if ((rc = check(launch_batch_zero(st, tab, n), "batch_zero"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[0], st));
     for (int m = MPM_JELLO; m <= MPM_WATER; ++m)
-        if (has_material(b, m) && (rc = check(launch_p2g_batch(st, m, tab, n, p2g), "p2g_batch"))) return rc;
+        if ((rc = check(launch_p2g_batch(st, m, tab, b->types.data(), n, p2g), "p2g_batch"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[1], st));
     if ((rc = check(launch_spill_batch(st, tab, n, SPILL_WGS), "spill_batch"))) return rc;
     if ((rc = check(launch_compact_batch(st, tab, n, b->compact_wgs), "compact_batch"))) return rc;
@@ -420,6 +415,7 @@ int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *sha
     }
     for (mpm_sim *s : b->sims)
         if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipErrorUnknown, "scene sync"));
+    for (mpm_sim *s : b->sims) b->types.push_back(s->groups[0].mat.type);
     for (mpm_sim *s : b->sims) {
         b->P0.push_back(s->P); b->P1.push_back(s->P_alt);
         b->pid0.push_back(s->pid); b->pid1.push_back(s->pid_alt);

@@ -401,10 +401,10 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
     if (sm) {
         const int leader = __ffsll((unsigned long long)sm) - 1;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(counters + CT_SPILL, (uint32_t)__popcll(sm));
+        if (lane == leader) base = gatomic_add(counters + CT_SPILL, (uint32_t)__popcll(sm));
         base = __shfl(base, leader);
         const uint64_t below = lane ? (sm & ((~0ull) >> (64 - lane))) : 0ull;
-        if (spill_me) spill[base + __popcll(below)] = (uint32_t)i;
+        if (spill_me) gp(spill)[base + __popcll(below)] = (uint32_t)i;
     }
     if (active) L.key[slot] = inside ? (uint8_t)(lx + 6 * (ly + 6 * lz)) : (uint8_t)NO_BIN;
     if (inside) {
@@ -653,7 +653,7 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 {
     __syncthreads();
     if (staged) {
-        float4 *dst = staging + (int64_t)slot * ARENA_NODES;
+        const int64_t dst = (int64_t)slot * ARENA_NODES;
         for (int k = threadIdx.x; k < ARENA_NODES; k += TILE_THREADS) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
             const int a = a_slot(kx, ky, kz);
@@ -663,10 +663,10 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
                 const float4 u = L.arena[w][a];
                 v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
             }
-            dst[k] = v;
+            gstore4(staging, dst + k, v);
             const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
             if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
-                flags[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
+                gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
         }
     }
     __syncthreads();
@@ -698,7 +698,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
                 // issue all loads before the first use: under the kernel's
                 // register cap the scheduler would otherwise wait for each
                 // load in turn
-                __builtin_amdgcn_sched_group_barrier(0x0020, 21, 0);
+                __builtin_amdgcn_sched_group_barrier(0x0020, MAT == MPM_WATER ? 16 : 21, 0);
                 bits |= make_stencil(g, x, s);
             }
             bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
@@ -730,26 +730,28 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
 }
 
-// Batched P2G: the tiles of every scene of material MAT in a batch in one
-// persistent launch (one launch per material: a run-time material switch
-// over the three tile bodies doubles the tile time at this register cap).
+// Batched P2G: the tiles of the scenes of material MAT among up to
+// P2G_CHUNK scenes of a batch in one persistent launch (one launch per
+// material -- a run-time material switch over the three tile bodies doubles
+// the tile time at this register cap -- and per chunk of P2G_CHUNK scenes).
 // Work index = the tiles of the first such scene, then of the next, ...;
 // workgroups walk it with the XCD-aware stride of the single-scene kernel.
+// The scene map (end of each scene's tile range) sits in LDS: 256 B more
+// than the tile's 52,976 B still fits three workgroups per CU (LDS is
+// allocated in granules; a 1 KB map leaves two, measured 40% slower).
+constexpr int P2G_CHUNK = 64;
 template <int MAT>
 __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
     __shared__ TileLds L;
-    // scene map in LDS (registers are at the kernel's cap; 1 KB keeps
-    // three workgroups per CU)
-    __shared__ uint32_t s_end[BATCH_MAX];
+    __shared__ uint32_t s_end[P2G_CHUNK];
     __shared__ uint32_t s_total;
     if (threadIdx.x < 64) {
         SliceScan sc;
         sc.build(nslices, [&](int s) {
-            return tab[s].mat.type == MAT ? min(*tab[s].tile_count, tab[s].desc_cap) : 0u;
+            return tab[s].mat.type == MAT ? min(*gp(tab[s].tile_count), tab[s].desc_cap) : 0u;
         });
-#pragma unroll
-        for (int j = 0; j < 4; ++j) s_end[j * 64 + threadIdx.x] = sc.end[j];
+        s_end[threadIdx.x] = sc.end[0];
         if (threadIdx.x == 0) s_total = sc.total;
     }
     __syncthreads();
@@ -758,18 +760,16 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
         const uint32_t tt = xcd_spread(k8, nt8);
         if (tt >= nt) continue;
-        uint32_t found = 0;
-        for (int j = 0; j * 64 < nslices; ++j) found += (uint32_t)__popcll(__ballot(s_end[j * 64 + lane] <= tt));
-        const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)found);
+        const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)__popcll(__ballot(s_end[lane] <= tt)));
         const BatchSlice &b = tab[sid];
-        const uint4 d = b.desc[tt - (sid ? s_end[sid - 1] : 0u)];
+        const uint4 d = gload_u4(b.desc, tt - (sid ? s_end[sid - 1] : 0u));
         const uint32_t bits =
             p2g_tile<MAT>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L);
-        if (bits) atomicOr(b.counters + CT_HEALTH, bits);
+        if (bits) gatomic_or(b.counters + CT_HEALTH, bits);
     }
     if (blockIdx.x == 0)
         for (int s = threadIdx.x; s < nslices; s += blockDim.x)
-            if (tab[s].mat.type == MAT) tab[s].counters[CT_TILES] = min(*tab[s].tile_count, tab[s].desc_cap);
+            if (tab[s].mat.type == MAT) gp(tab[s].counters)[CT_TILES] = min(*gp(tab[s].tile_count), tab[s].desc_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,14 +1165,24 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
     b.nblocks = s->nblocks;
 }
 
-int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, int nslices, int wgs)
+// Material `mat` of the scenes [0, nslices) of tab, in chunks of P2G_CHUNK.
+int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs)
 {
-    switch (mat) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_batch<MPM_JELLO>, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_batch<MPM_SNOW>, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices); break;
-    default: hipLaunchKernelGGL(k_p2g_batch<MPM_WATER>, dim3(wgs), dim3(TILE_THREADS), 0, st, tab, nslices); break;
+    dim3 gr(wgs), bl(TILE_THREADS);
+    for (int c0 = 0; c0 < nslices; c0 += P2G_CHUNK) {
+        const int n = std::min(P2G_CHUNK, nslices - c0);
+        bool any = false;
+        for (int k = c0; k < c0 + n; ++k) any = any || types[k] == mat;
+        if (!any) continue;
+        switch (mat) {
+        case MPM_JELLO: hipLaunchKernelGGL(k_p2g_batch<MPM_JELLO>, gr, bl, 0, st, tab + c0, n); break;
+        case MPM_SNOW: hipLaunchKernelGGL(k_p2g_batch<MPM_SNOW>, gr, bl, 0, st, tab + c0, n); break;
+        default: hipLaunchKernelGGL(k_p2g_batch<MPM_WATER>, gr, bl, 0, st, tab + c0, n); break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
     }
-    return (int)hipGetLastError();
+    return 0;
 }
 
 int launch_g2p_fast(mpm_sim *s, int group, float dt)

@@ -34,8 +34,8 @@ struct ArenaFetch {
     __device__ __forceinline__ float4 operator()(const GridP &g, int x, int y, int z) const
     {
         const unsigned lx = (unsigned)(x - ax), ly = (unsigned)(y - ay), lz = (unsigned)(z - az);
-        if (lx < (unsigned)ARENA && ly < (unsigned)ARENA && lz < (unsigned)ARENA) return lds[a_slot(lx, ly, lz)];
-        return gv[node_slot(g, x, y, z)];
+        if (lx < (unsigned)ARENA && ly < (unsigned)ARENA && lz < (unsigned)ARENA) return lload4(lds, a_slot(lx, ly, lz));
+        return gload4(gv, node_slot(g, x, y, z));
     }
     // all 27 stencil nodes of base b lie in the arena
     __device__ __forceinline__ bool covers(const int b[3]) const
@@ -52,7 +52,7 @@ struct ArenaOnlyFetch {
     int ax, ay, az;
     __device__ __forceinline__ float4 operator()(const GridP &, int x, int y, int z) const
     {
-        return lds[a_slot(x - ax, y - ay, z - az)];
+        return lload4(lds, a_slot(x - ax, y - ay, z - az));
     }
 };
 

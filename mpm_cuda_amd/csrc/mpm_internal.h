@@ -77,6 +77,60 @@ enum Counter : int {
     CT_COUNT = 16
 };
 
+// Memory accesses with their address space stated.  Pointers that reach
+// device code through memory (batch tables, fetch structs) are generic to
+// the compiler, which then emits FLAT instructions; on gfx950 a FLAT access
+// also counts against the LDS wait counter, so every LDS wait in the
+// surrounding code waits for it too.  Global buffers here come from
+// hipMalloc, LDS ones are __shared__ arrays.  (16-byte accesses go through
+// the native vector type: float4 / uint4 have no address-space-qualified
+// copies.)
+#define MPM_GLOBAL __attribute__((address_space(1)))
+#define MPM_LDS __attribute__((address_space(3)))
+typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int mpm_u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ MPM_GLOBAL T *gp(T *p)
+{
+    return (MPM_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ const MPM_GLOBAL T *gp(const T *p)
+{
+    return (const MPM_GLOBAL T *)p;
+}
+__device__ __forceinline__ float4 gload4(const float4 *p, int64_t i)
+{
+    const mpm_f32x4 v = ((const MPM_GLOBAL mpm_f32x4 *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore4(float4 *p, int64_t i, float4 v)
+{
+    mpm_f32x4 w;
+    w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
+    ((MPM_GLOBAL mpm_f32x4 *)p)[i] = w;
+}
+__device__ __forceinline__ uint4 gload_u4(const uint4 *p, int64_t i)
+{
+    const mpm_u32x4 v = ((const MPM_GLOBAL mpm_u32x4 *)p)[i];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 lload4(const float4 *p, int i)
+{
+    const mpm_f32x4 v = ((const MPM_LDS mpm_f32x4 *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gatomic_add(float *p, float v) { (void)__builtin_amdgcn_global_atomic_fadd_f32(gp(p), v); }
+__device__ __forceinline__ uint32_t gatomic_add(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gatomic_or(uint32_t *p, uint32_t v)
+{
+    (void)__hip_atomic_fetch_or(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Workgroups are dispatched to the 8 XCDs of the device round-robin
 // (workgroup id k runs on XCD k % 8), and each XCD has its own L2.  Work item
 // xcd_spread(k, n8) gives every XCD a contiguous eighth of [0, n8), so

@@ -10,11 +10,11 @@ constexpr float kFallbackRatio = 1e-4f;   // sigma_min / sigma_max below this ->
 
 __device__ __forceinline__ float ld(const PartView &pv, int plane, int i)
 {
-    return pv.P[(int64_t)plane * pv.cap + pv.off + i];
+    return gp(pv.P)[(int64_t)plane * pv.cap + pv.off + i];
 }
 __device__ __forceinline__ void st(const PartView &pv, int plane, int i, float v)
 {
-    pv.P[(int64_t)plane * pv.cap + pv.off + i] = v;
+    gp(pv.P)[(int64_t)plane * pv.cap + pv.off + i] = v;
 }
 
 // Stencil of one particle: base node (clamped so the 3x3x3 support stays in
@@ -55,7 +55,7 @@ __device__ __forceinline__ void svd_checked(const float F[9], float U[9], float 
 #pragma unroll
         for (int k = 0; k < 9; ++k) { U[k] = (float)Ud[k]; V[k] = (float)Vd[k]; }
         s[0] = (float)sd[0]; s[1] = (float)sd[1]; s[2] = (float)sd[2];
-        atomicAdd(counters + CT_FALLBACK, 1u);
+        gatomic_add(counters + CT_FALLBACK, 1u);
     }
 }
 
@@ -80,7 +80,7 @@ __device__ __forceinline__ void svd_warm(const PartView &pv, int i, const float 
         const float off = svd3<float>(F, U, s, V, warm_sweeps, V0);
         if (off <= kWarmTol && s[2] > kFallbackRatio * s[0]) return;
     }
-    atomicAdd(counters + CT_REJECT, 1u);
+    gatomic_add(counters + CT_REJECT, 1u);
     svd_checked(F, U, s, V, sweeps, counters);
 }
 

@@ -25,7 +25,7 @@ __device__ __forceinline__ void mark_blocks(const GridP &g, const Stencil &s, ui
     for (int bz = bz0; bz <= bz1; ++bz)
         for (int by = by0; by <= by1; ++by)
             for (int bx = bx0; bx <= bx1; ++bx)
-                flags[((int64_t)bz * g.nby + by) * g.nbx + bx] = 1u;
+                gp(flags)[((int64_t)bz * g.nby + by) * g.nbx + bx] = 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -69,21 +69,21 @@ __device__ __forceinline__ void compact_flags(uint32_t *flags, int nblocks, uint
 {
     __shared__ uint32_t wave_cnt[4], wg_base;
     int b = b0 + (int)threadIdx.x;
-    bool f = b < nblocks && flags[b] != 0u;
-    if (f) flags[b] = 0u;
+    bool f = b < nblocks && gp(flags)[b] != 0u;
+    if (f) gp(flags)[b] = 0u;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint64_t mask = __ballot(f);
     if (lane == 0) wave_cnt[w] = (uint32_t)__popcll(mask);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tot = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
-        wg_base = tot ? atomicAdd(counters + CT_ACTIVE, tot) : 0u;
+        wg_base = tot ? gatomic_add(counters + CT_ACTIVE, tot) : 0u;
     }
     __syncthreads();
     uint32_t off = wg_base;
     for (int k = 0; k < w; ++k) off += wave_cnt[k];
     uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
-    if (f) list[off + __popcll(below)] = (uint32_t)b;
+    if (f) gp(list)[off + __popcll(below)] = (uint32_t)b;
 }
 
 __global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list,
@@ -109,8 +109,8 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
     const int by = (int)((b / (uint32_t)g.nbx) % (uint32_t)g.nby);
     const int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
     const int64_t idx = (int64_t)b * BLK_NODES + lane;
-    float4 a = gmp[idx];
-    gmp[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 a = gload4(gmp, idx);
+    gstore4(gmp, idx, make_float4(0.f, 0.f, 0.f, 0.f));
     if (blk_tile) {
         for (int gi = 0; gi < ngroups; ++gi) {
             // lanes 0..26 look up the staging slots of the 27 neighbour
@@ -120,7 +120,7 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
             if (lane < 27) {
                 const int px = bx + lane % 3 - 1, py = by + (lane / 3) % 3 - 1, pz = bz + lane / 9 - 1;
                 if (px >= 0 && py >= 0 && pz >= 0 && px < g.nbx && py < g.nby && pz < g.nbz) {
-                    const int32_t t = bt[((int64_t)pz * g.nby + py) * g.nbx + px];
+                    const int32_t t = gp(bt)[((int64_t)pz * g.nby + py) * g.nbx + px];
                     if (t >= 0 && (uint32_t)t < staging_cap) my_t = t;
                 }
             }
@@ -136,7 +136,7 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
                 const int32_t t = __shfl(my_t, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
                 const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
                 const int64_t ts = t >= 0 ? t : 0;
-                v[e] = staging[ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax];
+                v[e] = gload4(staging, ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax);
                 if (t < 0) v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
@@ -161,7 +161,7 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
         }
         out = make_float4(vx, vy, vz, a.x);
     }
-    gv[idx] = out;
+    gstore4(gv, idx, out);
 }
 
 __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uint32_t *list, uint32_t *counters,
@@ -223,7 +223,7 @@ __device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, co
 #pragma unroll
         for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
     }
-    if (bits) atomicOr(counters + CT_HEALTH, bits);
+    if (bits) gatomic_or(counters + CT_HEALTH, bits);
 }
 
 template <int MAT>
@@ -258,10 +258,10 @@ __device__ __forceinline__ void spill_particle(const GridP &g, const MatP &mat, 
             for (int oz = 0; oz < 3; ++oz) {
                 float4 c = node_contrib(g, s, sc, ox, oy, oz);
                 float *dst = (float *)(grid + node_slot(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz));
-                unsafeAtomicAdd(dst + 0, c.x);
-                unsafeAtomicAdd(dst + 1, c.y);
-                unsafeAtomicAdd(dst + 2, c.z);
-                unsafeAtomicAdd(dst + 3, c.w);
+                gatomic_add(dst + 0, c.x);
+                gatomic_add(dst + 1, c.y);
+                gatomic_add(dst + 2, c.z);
+                gatomic_add(dst + 3, c.w);
             }
     mark_blocks(g, s, flags);
 }
@@ -307,16 +307,16 @@ __global__ void k_batch_zero(const BatchSlice *__restrict__ tab, int nslices)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslices) return;
     uint32_t *c = tab[s].counters;
-    c[CT_ACTIVE] = 0u;
-    c[CT_NODES] = 0u;
-    c[CT_SPILL] = 0u;
+    gp(c)[CT_ACTIVE] = 0u;
+    gp(c)[CT_NODES] = 0u;
+    gp(c)[CT_SPILL] = 0u;
 }
 
 template <int MAT>
 __device__ __forceinline__ void spill_scene(const BatchSlice &b, uint32_t n, uint32_t tid, uint32_t stride)
 {
     for (uint32_t k = tid; k < n; k += stride) {
-        const int i = (int)b.spill[k];
+        const int i = (int)gp(b.spill)[k];
         if (i < 0 || i >= b.pv.n) continue;
         spill_particle<MAT>(b.g, b.mat, b.pv, i, b.dt, b.counters, b.grid_mp, b.flags);
     }
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256) k_spill_batch(const BatchSlice *__restric
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
     for (int sid = 0; sid < nslices; ++sid) {
         const BatchSlice &b = tab[sid];
-        const uint32_t n = min(b.counters[CT_SPILL], (uint32_t)b.pv.n);
+        const uint32_t n = min(gp(b.counters)[CT_SPILL], (uint32_t)b.pv.n);
         if (n == 0u) continue;
         switch (b.mat.type) {
         case MPM_JELLO: spill_scene<MPM_JELLO>(b, n, tid, stride); break;
@@ -353,14 +353,14 @@ __global__ void __launch_bounds__(256) k_compact_batch(const BatchSlice *__restr
 __global__ void __launch_bounds__(64) k_grid_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
     SliceScan sc;
-    sc.build(nslices, [&](int s) { return tab[s].counters[CT_ACTIVE]; });
+    sc.build(nslices, [&](int s) { return gp(tab[s].counters)[CT_ACTIVE]; });
     const uint32_t n = sc.total, n8 = round_xcd(n);
     for (uint32_t k8 = blockIdx.x; k8 < n8; k8 += gridDim.x) {
         const uint32_t k = xcd_spread(k8, n8);
         if (k >= n) continue;
         const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(k));
         const BatchSlice &b = tab[sid];
-        const uint32_t blk = (uint32_t)__builtin_amdgcn_readfirstlane((int)b.list[k - sc.start(sid)]);
+        const uint32_t blk = (uint32_t)__builtin_amdgcn_readfirstlane((int)gp(b.list)[k - sc.start(sid)]);
         grid_block(b.g, b.dt, blk, b.grid_mp, b.grid_v, b.staging, b.blk_tile, 1, b.staging_cap, b.nblocks);
     }
 }

@@ -177,6 +177,29 @@ def test_batch_warm_start(gpu):
         _close(B)
 
 
+def test_batch_of_70_scenes(gpu):
+    """More scenes than one batched P2G launch maps (64): the P2G stage runs
+    in chunks; every scene still matches its twin stepped on its own."""
+    mats = (capi.JELLO, capi.SNOW, capi.WATER)
+    specs = [dict(type_=mats[k % 3], n_grid=24 + 4 * (k % 3), seed=100 + k, ppc=2,
+                  box=((0.35, 0.35, 0.35), (0.6, 0.55 + 0.01 * (k % 5), 0.6)), dt=5e-5 if k % 3 == 2 else 1e-4)
+             for k in range(70)]
+    A, B = _pairs(specs=specs), _pairs(specs=specs)
+    try:
+        with capi.Batch([p.gpu for p in A], capi.batch_shape(sort_every=4)) as b:
+            assert b.scenes == 70
+            b.run(9)
+        for p in B:
+            for _ in range(9):
+                p.gpu.step(p.dt)
+        for pa, pb in zip(A, B):
+            _compare(pa, pb)
+            assert pa.gpu.stats()["tiles"] > 0
+    finally:
+        _close(A)
+        _close(B)
+
+
 def test_batch_rejects_bad_scenes(gpu):
     plain = Pair(form=capi.FORM_PLAIN, **SPECS[0])
     fast = Pair(form=capi.FORM_FAST, **SPECS[0])
