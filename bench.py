@@ -69,7 +69,8 @@ def parse():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the plain-form comparison run")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="oracle steps for the CPU baseline and the FP32-vs-FP64 error")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -271,16 +272,79 @@ def cpu_baseline(workload, steps):
         t0 = time.perf_counter()
         for _ in range(nsteps):
             sim.step(cfg.dt)
-        return n * nsteps / (time.perf_counter() - t0), time.perf_counter() - t0
+        dt_s = time.perf_counter() - t0
+        return n * nsteps / dt_s, dt_s, sim
 
-    multi, tm = rate(cores, max(steps, 2))
-    single, ts = rate(1, 1)
+    multi, tm, orc = rate(cores, max(steps, 2))
+    single, ts, _ = rate(1, 1)
     O.set_threads(1)
     return {"value": round(multi, 1), "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle (oracle/mpm_oracle.c, OpenMP, atomic P2G node sums) on all {n:,} particles of "
                       f"the same workload, {max(steps, 2)} steps in {tm:.1f} s on {cores} threads; one thread: "
                       f"{single:.0f} particle-steps/s (1 step, {ts:.1f} s); dense FP64 grid as in the reference",
-            "single_thread_value": round(single, 1)}
+            "single_thread_value": round(single, 1)}, orc
+
+
+def _field_errors(g, ref, h):
+    out = {}
+    vscale = float(np.abs(ref["v"]).max()) / h          # scale of a velocity gradient
+    for k, b in ref.items():
+        a = np.asarray(g[k], np.float64).reshape(b.shape)
+        nb = np.linalg.norm(b)
+        e = {"rel_l2": float(f"{np.linalg.norm(a - b) / nb:.3e}") if nb > 0 else None,
+             "max_abs": float(f"{np.abs(a - b).max():.3e}")}
+        if k == "C":
+            e["max_abs_over_vmax_per_h"] = float(f"{np.abs(a - b).max() / max(vscale, 1e-300):.3e}")
+        out[k] = e
+    return out
+
+
+def _gpu_fields(sim):
+    from mpm_cuda_amd import capi
+    return {"x": sim.read(0, capi.P_POS), "v": sim.read(0, capi.P_VEL), "C": sim.read(0, capi.P_C),
+            "F": sim.read(0, capi.P_F), "J_p": sim.read(0, capi.P_JP)[:, 0]}
+
+
+def _orc_fields(orc):
+    o = orc.groups[0]
+    return {"x": o.x, "v": o.v, "C": o.C.reshape(-1, 9), "F": o.Fe.reshape(-1, 9), "J_p": np.linalg.det(o.Fp)}
+
+
+def fp32_vs_fp64(workload, orc, steps, restart_at=150):
+    """Per-field error of the device FP32 FAST form against the FP64 oracle:
+    (a) after `steps` steps from the same initial state (the oracle run of the
+    CPU baseline); (b) in the collision: the device state after `restart_at`
+    steps is copied into the oracle (J_p as F_p = J_p^(1/3) I, the only use
+    of F_p) and both advance `steps` more steps.  Relative L2 error
+    ||a - b|| / ||b|| and largest absolute difference per field; C also
+    against the velocity-gradient scale max|v| / h (C is ~0 before impact)."""
+    import oracle as O
+    sim, info = _make_sim(workload, "fast")
+    cfg, mat = info["cfg"], info["mat"]
+    h = cfg.h
+    sim.run(steps, cfg.dt)
+    sim.sync()
+    out = {"from_start": {"steps": steps, **_field_errors(_gpu_fields(sim), _orc_fields(orc), h)}}
+    sim.run(restart_at - steps, cfg.dt)
+    sim.sync()
+    g0 = _gpu_fields(sim)
+    mu, lam = O.lame(mat.E, mat.nu)
+    n = g0["x"].shape[0]
+    grp = O.Group(g0["x"].astype(np.float64), g0["v"].astype(np.float64), type=O.SNOW, mass=mat.rho * h ** 3,
+                  vol=h ** 3, mu=mu, lam=lam, theta_c=mat.theta_c, theta_s=mat.theta_s, xi=mat.xi,
+                  C=g0["C"].astype(np.float64).reshape(n, 3, 3), Fe=g0["F"].astype(np.float64).reshape(n, 3, 3),
+                  Fp=np.cbrt(g0["J_p"].astype(np.float64))[:, None, None] * np.eye(3))
+    o2 = O.Sim(cfg.nx, cfg.ny, cfg.nz, h, [grp], gravity=tuple(cfg.gravity), bc_pad=cfg.bc_pad, bc_mode=cfg.bc_mode)
+    O.set_threads(max(1, min(16, len(os.sched_getaffinity(0)))))
+    for _ in range(steps):
+        o2.step(cfg.dt)
+    O.set_threads(1)
+    sim.run(steps, cfg.dt)
+    sim.sync()
+    out["in_collision"] = {"start_step": restart_at, "steps": steps, **_field_errors(_gpu_fields(sim), _orc_fields(o2), h)}
+    out["health_bits"] = sim.health()
+    sim.close()
+    return out
 
 
 def main():
@@ -344,7 +408,10 @@ def main():
         out["batched"] = run_batch(a.steps, a.warmup)
         out["batched_warm"] = run_batch(a.steps, a.warmup, warm=True)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_steps)
+        cb = cpu_baseline(a.workload, a.cpu_steps)
+        if cb is not None:
+            out["cpu_baseline"], orc = cb
+            out["fp32_vs_fp64"] = fp32_vs_fp64(a.workload, orc, max(a.cpu_steps, 2))
     print(json.dumps(out), flush=True)
 
 
