@@ -72,6 +72,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="oracle steps for the CPU baseline and the FP32-vs-FP64 error")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D snow workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
@@ -285,6 +286,41 @@ def cpu_baseline(workload, steps):
             "single_thread_value": round(single, 1)}, orc
 
 
+def run_2d(oracle_steps=400):
+    """The reference's 2D snow step (MPMSnow.py) on its 9,000-particle
+    workload for its 2000 steps, timed on the device from the initial state
+    (hipGraph replays of 50 steps; the capture is inside the timed region),
+    and the FP64 2D oracle on one host thread over the first `oracle_steps`
+    steps of the same workload."""
+    import oracle as O
+    from mpm_cuda_amd import capi, workloads
+    warm, info = workloads.snow2d_squares()          # loads the kernels
+    warm.run(60)
+    warm.sync()
+    warm.close()
+    sim, info = workloads.snow2d_squares()
+    n, steps = sim.n, info["steps"]
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    health = sim.health()
+    sim.close()
+    c = info["cfg"]
+    o = O.Sim2D(info["x"].astype(np.float64), 0.0, n_grid=c.n_grid, wall=c.wall, E=c.E, nu=c.nu, rho=c.rho,
+                xi=c.xi, theta_c=c.theta_c, theta_s=c.theta_s, gravity=tuple(c.gravity))
+    a0 = time.perf_counter()
+    for _ in range(oracle_steps):
+        o.step(c.dt)
+    ta = time.perf_counter() - a0
+    return {"workload": info["desc"], "particles": n, "steps": steps, "value": round(n * steps / (t1 - t0), 1),
+            "unit": "particle-steps/s", "ms_per_step": round((t1 - t0) / steps * 1e3, 5),
+            "seconds": round(t1 - t0, 4), "health_bits": health,
+            "cpu_baseline": {"value": round(n * oracle_steps / ta, 1), "unit": "particle-steps/s", "cores": 1,
+                             "kind": "port", "sample": f"FP64 2D oracle (oracle/mpm_oracle2d.c), first "
+                                                       f"{oracle_steps} steps of the same workload, {ta:.1f} s"}}
+
+
 def _field_errors(g, ref, h):
     out = {}
     vscale = float(np.abs(ref["v"]).max()) / h          # scale of a velocity gradient
@@ -404,6 +440,8 @@ def main():
             "kernel_avg_ms": roofline(r, material, a.form, ksteps, a.workload)["kernel_avg_ms"]}
         if "plain" in out["variants"]:
             out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
+    if not a.no_2d:
+        out["snow2d"] = run_2d()
     if not a.no_batch:
         out["batched"] = run_batch(a.steps, a.warmup)
         out["batched_warm"] = run_batch(a.steps, a.warmup, warm=True)

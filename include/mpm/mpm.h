@@ -336,6 +336,46 @@ int64_t mpm_batch_particles(const mpm_batch *b);
 int32_t mpm_batch_scenes(const mpm_batch *b);
 void mpm_batch_destroy(mpm_batch *b);
 
+/* ---- 2D snow ----------------------------------------------------------
+ * The reference's 2D prototype MPMSnow.py (Taichi): substep() :77-150 --
+ * P2G with the FixedCorotatedStress :28-32 and hardening exp(xi (1 - J_p))
+ * :87, grid update with gravity and sticky walls :105-114, G2P with the
+ * snow clamp [1 - theta_c, 1 + theta_s] :141-145 -- and reset() :154-162
+ * (F_e = F_p = I, C = 0).  The grid is n_grid^2 nodes on the unit square,
+ * dx = 1 / n_grid, particle volume (dx/2)^2 (:8-11).  FP32 on the device. */
+typedef struct mpm2d_sim mpm2d_sim;
+
+enum { MPM2D_POS = 0, MPM2D_VEL = 1, MPM2D_C = 2, MPM2D_F = 3, MPM2D_JP = 4 };  /* widths 2, 2, 4, 4, 1 */
+
+typedef struct mpm2d_config {
+    int32_t n_grid;           /* nodes per axis (MPMSnow.py:7): 128 */
+    int32_t wall;             /* sticky walls: nodes i <= wall or i >= n - wall (:109): 5 */
+    double dt;                /* step: 1e-4 */
+    double E, nu, rho;        /* 1e3, 0.2 (:12), 1 (:10) */
+    double theta_c, theta_s;  /* clamp band: 2.5e-2, 7.5e-3 (the script's :141-142 use 8e-2, 1.5e-2) */
+    double xi;                /* hardening: 10 (:87) */
+    double gravity[2];        /* acceleration (:108): (0, -50) */
+    int32_t graph_steps;      /* steps per replayed hipGraph in mpm2d_run (0: plain launches): 50 */
+    int32_t reserved[3];
+} mpm2d_config;
+
+void mpm2d_config_default(mpm2d_config *cfg);
+int mpm2d_create(const mpm2d_config *cfg, mpm2d_sim **out);
+/* reset(), :154-162: n particles at xy[n][2] (inside [dx, 1 - 2 dx]),
+ * velocities vel[n][2] (NULL: 0); once per simulator. */
+int mpm2d_add_particles(mpm2d_sim *s, int64_t n, const float *xy, const float *vel);
+/* substep(), :77-150, nsteps times (asynchronous). */
+int mpm2d_run(mpm2d_sim *s, int32_t nsteps);
+/* Particle field (MPM2D_*) as n rows of its width, row-major 2x2. */
+int mpm2d_read(mpm2d_sim *s, int32_t field, float *out);
+/* Grid after the last update: mass[n_grid^2], vel[n_grid^2][2], node (i, j) at j * n_grid + i. */
+int mpm2d_read_grid(mpm2d_sim *s, float *mass, float *vel);
+int mpm2d_health(mpm2d_sim *s, uint32_t *bits);      /* MPM_H_* bits since creation */
+int64_t mpm2d_particles(const mpm2d_sim *s);
+int64_t mpm2d_steps(const mpm2d_sim *s);
+int mpm2d_synchronize(mpm2d_sim *s);
+void mpm2d_destroy(mpm2d_sim *s);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *mpm_last_error(void);
 

@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmpm_hip.so")
 ARCH = os.environ.get("MPM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mpm_api.hip", "mpm_step.hip", "mpm_fast.hip", "mpm_batch.hip", "mpm_mlp.hip", "mpm_probe.hip", "scene.cpp"]
+SOURCES = ["mpm_api.hip", "mpm_step.hip", "mpm_fast.hip", "mpm_batch.hip", "mpm2d.hip", "mpm_mlp.hip", "mpm_probe.hip",
+           "scene.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -56,6 +56,13 @@ class MpmBatchShape(C.Structure):
                 ("sort_every", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class Mpm2dConfig(C.Structure):
+    _fields_ = [("n_grid", C.c_int32), ("wall", C.c_int32), ("dt", C.c_double), ("E", C.c_double),
+                ("nu", C.c_double), ("rho", C.c_double), ("theta_c", C.c_double), ("theta_s", C.c_double),
+                ("xi", C.c_double), ("gravity", C.c_double * 2), ("graph_steps", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
+
+
 class MpmError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mpm error {code}: {msg}")
@@ -115,6 +122,17 @@ _SIGS = {
     "mpm_batch_particles": (C.c_int64, [C.c_void_p]),
     "mpm_batch_scenes": (C.c_int32, [C.c_void_p]),
     "mpm_batch_destroy": (None, [C.c_void_p]),
+    "mpm2d_config_default": (None, [C.POINTER(Mpm2dConfig)]),
+    "mpm2d_create": (C.c_int, [C.POINTER(Mpm2dConfig), C.POINTER(C.c_void_p)]),
+    "mpm2d_add_particles": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mpm2d_run": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mpm2d_read": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "mpm2d_read_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mpm2d_health": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "mpm2d_particles": (C.c_int64, [C.c_void_p]),
+    "mpm2d_steps": (C.c_int64, [C.c_void_p]),
+    "mpm2d_synchronize": (C.c_int, [C.c_void_p]),
+    "mpm2d_destroy": (None, [C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -382,6 +400,88 @@ class Batch:
     @property
     def scenes(self) -> int:
         return lib().mpm_batch_scenes(self._h)
+
+
+P2_POS, P2_VEL, P2_C, P2_F, P2_JP = 0, 1, 2, 3, 4
+_P2_WIDTH = {P2_POS: 2, P2_VEL: 2, P2_C: 4, P2_F: 4, P2_JP: 1}
+
+
+def config2d(**kw) -> Mpm2dConfig:
+    c = Mpm2dConfig()
+    lib().mpm2d_config_default(C.byref(c))
+    for k, v in kw.items():
+        if k == "gravity":
+            c.gravity = (C.c_double * 2)(*v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class Sim2D:
+    """Owner of an ``mpm2d_sim*``: the reference's 2D snow step (MPMSnow.py)."""
+
+    def __init__(self, cfg: Mpm2dConfig | None = None, **kw):
+        self.cfg = cfg if cfg is not None else config2d(**kw)
+        self._h = C.c_void_p()
+        check(lib().mpm2d_create(C.byref(self.cfg), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().mpm2d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_particles(self, xy, vel=None) -> None:
+        xy = _f32(xy).reshape(-1, 2)
+        vp = None
+        if vel is not None:
+            vel = _f32(np.broadcast_to(np.asarray(vel, np.float32), xy.shape))
+            vp = vel.ctypes.data_as(C.c_void_p)
+        check(lib().mpm2d_add_particles(self._h, xy.shape[0], xy.ctypes.data_as(C.c_void_p), vp))
+
+    def run(self, nsteps: int) -> None:
+        check(lib().mpm2d_run(self._h, int(nsteps)))
+
+    def sync(self) -> None:
+        check(lib().mpm2d_synchronize(self._h))
+
+    @property
+    def n(self) -> int:
+        return lib().mpm2d_particles(self._h)
+
+    @property
+    def steps(self) -> int:
+        return lib().mpm2d_steps(self._h)
+
+    def read(self, field: int) -> np.ndarray:
+        w = _P2_WIDTH[field]
+        out = np.empty((self.n, w), np.float32)
+        check(lib().mpm2d_read(self._h, int(field), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def grid(self):
+        """(mass [n*n], velocity [n*n][2]) after the last grid update; node (i, j) at j * n + i."""
+        nn = self.cfg.n_grid * self.cfg.n_grid
+        m = np.empty(nn, np.float32)
+        v = np.empty((nn, 2), np.float32)
+        check(lib().mpm2d_read_grid(self._h, m.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p)))
+        return m, v
+
+    def health(self) -> int:
+        b = C.c_uint32()
+        check(lib().mpm2d_health(self._h, C.byref(b)))
+        return b.value
 
 
 def sample_object(cfg: MpmConfig, obj: MpmObject) -> np.ndarray:

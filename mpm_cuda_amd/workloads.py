@@ -20,6 +20,14 @@
   dam-break block in the lower corner (k = 1e4, gamma = 7, dt = 5e-5).  8
   samples per cell, trimmed to the drawn count.
 
+* ``snow2d_squares`` -- the reference's 2D snow step (MPMSnow.py): 128^2 grid
+  (dx = 1/128), three 0.2-side squares of 3,000 seeded uniform-random particles
+  each, lower-left corners at (0.3 + 0.1 g, 0.05 + 0.32 g), starting at rest
+  (the script's (0, -6.4) m/s start is unstable with E = 1e3 at dt = 1e-4: the
+  FP64 oracle blows up at step ~630), E = 1e3, nu = 0.2, rho = 1,
+  theta_c = 2.5e-2, theta_s = 7.5e-3, xi = 10, gravity -50, dt = 1e-4, 2000
+  steps, FP32.
+
 Each builder returns ``(sim, info)`` where ``info`` holds the arrays needed
 to rebuild the same state elsewhere (positions, velocities, material).
 """
@@ -167,6 +175,21 @@ def batch_scenes(n_scenes=64, seed=SEED0 + 5, form=capi.FORM_FAST, **cfg_kw):
     """The batched workload: a list of (sim, info)."""
     return [batch_scene(t, N, n, seed + 17 * k, form=form, **cfg_kw)
             for k, (t, N, n) in enumerate(batch_plan(n_scenes, seed))]
+
+
+def snow2d_state(seed=SEED0 + 6, per_square=3000):
+    """Positions [9000][2] (FP32) of the three squares."""
+    rng = np.random.default_rng(seed)
+    sq = [rng.random((per_square, 2)) * 0.2 + np.array([0.3 + 0.1 * g, 0.05 + 0.32 * g]) for g in range(3)]
+    return np.concatenate(sq).astype(np.float32)
+
+
+def snow2d_squares(**cfg_kw):
+    x = snow2d_state()
+    sim = capi.Sim2D(**cfg_kw)
+    sim.add_particles(x)
+    return sim, {"x": x, "cfg": sim.cfg, "name": "snow2d_squares", "steps": 2000,
+                 "desc": f"2D snow, three 0.2 squares, {len(x):,} particles, 128^2 grid, dt=1e-4, FP32"}
 
 
 WORKLOADS = {"snow_spheres": snow_spheres, "jello_cube": jello_cube, "water_dam": water_dam}

@@ -47,6 +47,19 @@ class OrcParams(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("bc_pad", C.c_int32), ("bc_mode", C.c_int32)]
 
 
+_d2 = C.c_double * 2
+
+
+class Orc2Params(C.Structure):
+    _fields_ = [("n_grid", C.c_int32), ("wall", C.c_int32), ("dx", C.c_double), ("p_vol", C.c_double),
+                ("p_mass", C.c_double), ("mu0", C.c_double), ("lam0", C.c_double), ("xi", C.c_double),
+                ("theta_c", C.c_double), ("theta_s", C.c_double), ("gravity", C.c_double * 2)]
+
+
+class Orc2Particles(C.Structure):
+    _fields_ = [("n", C.c_int64), ("x", _dp), ("v", _dp), ("C", _dp), ("Fe", _dp), ("Fp", _dp)]
+
+
 REF_MESH_QUERY = os.path.join(_HERE, "_ref", "libmesh_query.so")
 _REF_SRC = "/root/reference/thirdparties/mesh_query0.1"
 
@@ -111,6 +124,12 @@ def lib():
         L.orc_g2p.restype = C.c_int; L.orc_g2p.argtypes = [C.POINTER(OrcGrid), C.POINTER(OrcGroup), C.c_double]
         L.orc_step.restype = C.c_int
         L.orc_step.argtypes = [C.POINTER(OrcGrid), C.POINTER(OrcGroup), C.c_int, C.POINTER(OrcParams), C.c_double]
+        L.orc2_N.restype = C.c_double; L.orc2_N.argtypes = [C.c_double]
+        L.orc2_dN.restype = C.c_double; L.orc2_dN.argtypes = [C.c_double]
+        L.orc2_svd.restype = None; L.orc2_svd.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc2_fcr_tau.restype = None; L.orc2_fcr_tau.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.orc2_step.restype = C.c_int
+        L.orc2_step.argtypes = [_dp, _dp, C.POINTER(Orc2Particles), C.POINTER(Orc2Params), C.c_double]
         del d3
         _lib = L
     return _lib
@@ -254,3 +273,35 @@ class Sim:
 
     def grid_vec(self):
         return self.p.reshape(self.nz, self.ny, self.nx, 3)
+
+
+class Sim2D:
+    """FP64 2D snow of MPMSnow.py:77-150 on an n x n grid (index j * n + i).
+    Particles in AoS FP64 (x, v: [n][2]; C, Fe, Fp: [n][2][2]); the script's
+    constants are keyword arguments (defaults: the 9,000-particle snow
+    workload -- E = 1e3, nu = 0.2, rho = 1, theta_c = 2.5e-2,
+    theta_s = 7.5e-3, xi = 10, gravity (0, -50), 5-node sticky walls)."""
+
+    def __init__(self, x, v, *, n_grid=128, wall=5, E=1e3, nu=0.2, rho=1.0, xi=10.0, theta_c=2.5e-2,
+                 theta_s=7.5e-3, gravity=(0.0, -50.0), C=None, Fe=None, Fp=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 2)
+        n = x.shape[0]
+        self.n = n
+        self.x = x.copy()
+        self.v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (n, 2))).copy()
+        eye = np.broadcast_to(np.eye(2), (n, 2, 2))
+        self.C = np.zeros((n, 2, 2)) if C is None else np.ascontiguousarray(C, np.float64).reshape(n, 2, 2).copy()
+        self.Fe = eye.copy() if Fe is None else np.ascontiguousarray(Fe, np.float64).reshape(n, 2, 2).copy()
+        self.Fp = eye.copy() if Fp is None else np.ascontiguousarray(Fp, np.float64).reshape(n, 2, 2).copy()
+        dx = 1.0 / n_grid
+        p_vol = (0.5 * dx) ** 2
+        mu, lam = lame(E, nu)
+        self.params = Orc2Params(n_grid, wall, dx, p_vol, p_vol * rho, mu, lam, xi, theta_c, theta_s,
+                                 _d2(*gravity))
+        self.n_grid = n_grid
+        self.grid_m = np.zeros(n_grid * n_grid)
+        self.grid_v = np.zeros(n_grid * n_grid * 2)
+
+    def step(self, dt):
+        ps = Orc2Particles(self.n, _ptr(self.x), _ptr(self.v), _ptr(self.C), _ptr(self.Fe), _ptr(self.Fp))
+        return lib().orc2_step(_ptr(self.grid_m), _ptr(self.grid_v), C.byref(ps), C.byref(self.params), float(dt))

@@ -103,6 +103,30 @@ int  orc_step(orc_grid *g, orc_group *groups, int ngroups, const orc_params *prm
  * G2P); 1 by default.  Returns the count in effect. */
 int orc_set_threads(int n);
 
+/* ---- 2D snow (MPMSnow.py:77-150, mpm_oracle2d.c) ---- */
+typedef struct orc2_params {
+    int32_t n_grid;            /* nodes per axis (MPMSnow.py:7) */
+    int32_t wall;              /* sticky band: i <= wall or i >= n - wall (:109) */
+    double dx;                 /* 1 / n_grid (:8) */
+    double p_vol, p_mass;      /* (dx/2)^2 and p_vol * rho (:10-11) */
+    double mu0, lam0;          /* Lame constants (:13) */
+    double xi;                 /* hardening h = exp(xi (1 - J_p)) (:87) */
+    double theta_c, theta_s;   /* clamp band [1 - theta_c, 1 + theta_s] (:141-142) */
+    double gravity[2];         /* acceleration added as dt * g (:108) */
+} orc2_params;
+
+typedef struct orc2_particles {
+    int64_t n;
+    double *x, *v;             /* [n][2] */
+    double *C, *Fe, *Fp;       /* [n][4], row-major 2x2 */
+} orc2_particles;
+
+double orc2_N(double x);                                            /* :36-45 */
+double orc2_dN(double x);                                           /* :55-65 */
+void orc2_svd(const double F[4], double U[4], double s[2], double V[4]);
+void orc2_fcr_tau(const double F[4], double mu, double lam, double tau[4]);   /* :28-32 */
+int orc2_step(double *grid_m, double *grid_v, orc2_particles *p, const orc2_params *prm, double dt);
+
 #ifdef __cplusplus
 }
 #endif
