@@ -327,6 +327,8 @@ constexpr int TILE_WAVES = 2;
 constexpr int TILE_THREADS = 64 * TILE_WAVES;
 static_assert(ARENA_NODES % TILE_THREADS == 0, "arena nodes per thread must be whole");
 constexpr int CHUNK = 384;          // particles per pass over a tile (a full 4^3-cell block holds ~320 at 5 per cell)
+constexpr int A_PASSES = CHUNK / TILE_THREADS;   // phase-A passes of a wave over a chunk
+static_assert(CHUNK % TILE_THREADS == 0, "whole phase-A passes");
 constexpr int ITEM_CAP = 4;         // particles per lane item
 constexpr int NBIN = 216;           // 6^3 base positions inside an arena
 constexpr uint32_t NO_BIN = 255u;
@@ -383,6 +385,29 @@ __device__ __forceinline__ TileGeo tile_geo(const GridP &g, uint32_t blk)
     const int by = (int)((blk / (uint32_t)g.nbx) % (uint32_t)g.nby);
     const int bz = (int)(blk / ((uint32_t)g.nbx * (uint32_t)g.nby));
     return TileGeo{bx * BLK - 1, by * BLK - 1, bz * BLK - 1};
+}
+
+// Loaded particle inputs parked in the particle's own record slot
+// (rec[k][slot], k < 21) between the loads and their use.
+template <int MAT>
+__device__ __forceinline__ void park_raw(TileLds &L, int slot, const RawP &r)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { L.rec[a][slot] = r.x[a]; L.rec[3 + a][slot] = r.v[a]; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L.rec[6 + k][slot] = r.C[k];
+#pragma unroll
+    for (int k = 0; k < (MAT == MPM_WATER ? 1 : 6); ++k) L.rec[15 + k][slot] = r.s[k];
+}
+template <int MAT>
+__device__ __forceinline__ void unpark_raw(const TileLds &L, int slot, RawP &r)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { r.x[a] = L.rec[a][slot]; r.v[a] = L.rec[3 + a][slot]; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.C[k] = L.rec[6 + k][slot];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.s[k] = (MAT == MPM_WATER && k > 0) ? 0.0f : L.rec[15 + k][slot];
 }
 
 // Phase A tail: bin the particle and write its record, or list it for the
@@ -683,23 +708,42 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     const TileGeo t = tile_geo(g, d.z);
     const bool staged = d.w < staging_cap;
     tile_clear(L);
+    const int wave0 = (int)(threadIdx.x & ~63u);
     for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
         const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
-        for (int s0 = (int)(threadIdx.x & ~63u); s0 < nc; s0 += TILE_THREADS) {
-            const int slot = s0 + (int)(threadIdx.x & 63u);
+        // A1: the loads of all passes (up to 3 x 21, within the wave's 63
+        // outstanding vector loads) are issued before any is used and parked
+        // in the lane's own record slots: one memory round trip per chunk
+        // instead of one per pass
+        {
+            // unconditional (one basic block): lanes past the chunk reload
+            // its last particle and park it in their own unused slot
+            RawP raw[A_PASSES];
+#pragma unroll
+            for (int it = 0; it < A_PASSES; ++it) {
+                const int slot = min(it * TILE_THREADS + (int)threadIdx.x, nc - 1);
+                load_raw<MAT>(pv, (int)(d.x + c0) + slot, raw[it]);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x0020, A_PASSES * raw_loads(MAT), 0);
+#pragma unroll
+            for (int it = 0; it < A_PASSES; ++it) park_raw<MAT>(L, it * TILE_THREADS + (int)threadIdx.x, raw[it]);
+        }
+        // A2: each lane reads back its own slots (no barrier needed), builds
+        // the stencil and record, and bins the particle
+#pragma unroll
+        for (int it = 0; it < A_PASSES; ++it) {
+            if (it * TILE_THREADS + wave0 >= nc) break;              // wave-uniform
+            const int slot = it * TILE_THREADS + (int)threadIdx.x;
             const bool active = slot < nc;
             const int i = (int)(d.x + c0) + slot;
             Stencil s;
             s.b[0] = s.b[1] = s.b[2] = 0;
             Scatter sc;
             if (active) {
-                float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
-                load_scatter<MAT>(pv, i, mat, dt, true, 0, sc, counters);
-                // issue all loads before the first use: under the kernel's
-                // register cap the scheduler would otherwise wait for each
-                // load in turn
-                __builtin_amdgcn_sched_group_barrier(0x0020, MAT == MPM_WATER ? 16 : 21, 0);
-                bits |= make_stencil(g, x, s);
+                RawP r;
+                unpark_raw<MAT>(L, slot, r);
+                scatter_from_raw<MAT>(r, mat, dt, sc);
+                bits |= make_stencil(g, r.x, s);
             }
             bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
         }

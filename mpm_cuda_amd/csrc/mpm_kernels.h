@@ -150,6 +150,61 @@ __device__ __forceinline__ void load_scatter(const PartView &pv, int i, const Ma
     }
 }
 
+// The FAST-form P2G inputs of one particle as loaded (x, v, C and the stored
+// stress; water: J in s[0]), so that a block kernel can issue the loads of
+// several particles before it uses any.
+struct RawP {
+    float x[3], v[3], C[9], s[6];
+};
+// Plane p of particle i as (uniform plane base) + (32-bit lane offset): the
+// loads then take the scalar-base form and share one offset register.
+__device__ __forceinline__ float ld_plane(const PartView &pv, int plane, uint32_t i)
+{
+    return gp(pv.P + (int64_t)plane * pv.cap + pv.off)[i];
+}
+template <int MAT>
+__device__ __forceinline__ void load_raw(const PartView &pv, int i, RawP &r)
+{
+    const uint32_t u = (uint32_t)i;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r.x[a] = ld_plane(pv, PL_X + a, u);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r.v[a] = ld_plane(pv, PL_V + a, u);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.C[k] = ld_plane(pv, PL_C + k, u);
+    if (MAT == MPM_WATER) {
+        r.s[0] = ld_plane(pv, PL_S, u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r.s[k] = ld_plane(pv, PL_TAU + k, u);
+    }
+}
+constexpr int raw_loads(int mat) { return mat == MPM_WATER ? 16 : 21; }
+
+// load_scatter's arithmetic on loaded inputs (stress stored).
+template <int MAT>
+__device__ __forceinline__ void scatter_from_raw(const RawP &r, const MatP &mat, float dt, Scatter &sc)
+{
+    sc.m = mat.mass;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sc.a[a] = mat.mass * r.v[a];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sc.B[k] = mat.mass * r.C[k];
+    if (MAT == MPM_WATER) {
+        const float J = r.s[0];
+        const float pr = mat.eos_k * (powf(J, -mat.eos_gamma) - 1.0f);   // MPMCUDA.cu:353
+        const float c = dt * mat.vol * pr * J;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sc.G[k] = (k % 4 == 0) ? c : 0.0f;
+    } else {
+        const float c = -dt * mat.vol;
+        sc.G[0] = c * r.s[0]; sc.G[4] = c * r.s[1]; sc.G[8] = c * r.s[2];
+        sc.G[1] = sc.G[3] = c * r.s[3];
+        sc.G[2] = sc.G[6] = c * r.s[4];
+        sc.G[5] = sc.G[7] = c * r.s[5];
+    }
+}
+
 // Contribution of one particle to stencil node (ox, oy, oz): out = (m, p).
 __device__ __forceinline__ float4 node_contrib(const GridP &g, const Stencil &s, const Scatter &sc,
                                                int ox, int oy, int oz)
