@@ -370,7 +370,8 @@ struct TileLds {
     uint32_t items[CHUNK];
     uint16_t perm[CHUNK];
     uint16_t rnk[CHUNK];
-    uint32_t cnt[NBIN];
+    uint16_t cnt[NBIN];         // phase B: wave 0's counts per bin
+    uint16_t cnt1[NBIN];        // phase B: wave 1's counts, then each bin's first sorted position
     uint8_t key[CHUNK];
     uint32_t nitems;
 };
@@ -453,74 +454,86 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
     return spill_me;
 }
 
-// Phase B (wave 0 only): stable counting sort of the chunk's nc particles by
-// bin and the item list.  Items pack bin | first sorted position << 8 |
-// count << 20.
+// Phase B (both waves): counting sort of the chunk's nc particles by bin and
+// the item list.  Wave w counts the 64-particle batches w, w + 2, ... into
+// its own counts; wave 0 then builds the items from the summed counts; both
+// waves place their particles.  Within a bin, wave 0's particles precede
+// wave 1's, each in slot order, so the order (and every sum) is the same in
+// every run.  Items pack bin | first sorted position << 8 | count << 20.
 __device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
 {
-    const int lane = threadIdx.x & 63;
+    static_assert(TILE_WAVES == 2, "two counting waves");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    for (int b = lane; b < NBIN; b += 64) L.cnt[b] = 0u;
+    uint16_t *cw = wave ? L.cnt1 : L.cnt;
+    for (int b = lane; b < NBIN; b += 64) cw[b] = 0;
     __asm__ volatile("" ::: "memory");
-    for (int s0 = 0; s0 < nc; s0 += 64) {
+    for (int s0 = wave * 64; s0 < nc; s0 += TILE_THREADS) {
         const int sl = s0 + lane;
         const uint32_t key = sl < nc ? (uint32_t)L.key[sl] : NO_BIN;
         const bool v = key != NO_BIN;
         const uint64_t m = match_key8(key) & __ballot(v);
         const uint32_t below = (uint32_t)__popcll(m & lt);
         uint32_t prev = 0;
-        if (v) prev = L.cnt[key];
-        if (v && below == 0) L.cnt[key] = prev + (uint32_t)__popcll(m);
+        if (v) prev = cw[key];
+        if (v && below == 0) cw[key] = (uint16_t)(prev + (uint32_t)__popcll(m));
         if (v) L.rnk[sl] = (uint16_t)(prev + below);
         __asm__ volatile("" ::: "memory");
     }
-    // Lane l owns bins 4l .. 4l+3.  A bin of c particles gives c / ITEM_CAP
-    // full items and one item of the remainder.  Full items come first, then
-    // remainders by decreasing size, so a batch of 64 items mostly holds
-    // items of equal length (fewer idle lanes per round) and the items of one
-    // bin mostly fall into different batches (fewer turns in the flush).
-    uint32_t c[4], nfull = 0, csum = 0, nrem[ITEM_CAP];
+    __syncthreads();
+    if (wave == 0) {
+        // Lane l owns bins 4l .. 4l+3.  A bin of c particles gives c / ITEM_CAP
+        // full items and one item of the remainder.  Full items come first,
+        // then remainders by decreasing size, so a batch of 64 items mostly
+        // holds items of equal length (fewer idle lanes per round) and the
+        // items of one bin mostly fall into different batches (fewer turns in
+        // the flush).
+        uint32_t c[4], nfull = 0, csum = 0, nrem[ITEM_CAP];
 #pragma unroll
-    for (int r = 0; r < ITEM_CAP; ++r) nrem[r] = 0;
+        for (int r = 0; r < ITEM_CAP; ++r) nrem[r] = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int b = 4 * lane + j;
-        c[j] = b < NBIN ? L.cnt[b] : 0u;
-        csum += c[j];
-        nfull += c[j] / ITEM_CAP;
+        for (int j = 0; j < 4; ++j) {
+            const int b = 4 * lane + j;
+            c[j] = b < NBIN ? (uint32_t)L.cnt[b] + (uint32_t)L.cnt1[b] : 0u;
+            csum += c[j];
+            nfull += c[j] / ITEM_CAP;
 #pragma unroll
-        for (int r = 1; r < ITEM_CAP; ++r) nrem[r] += (c[j] % ITEM_CAP) == (uint32_t)r ? 1u : 0u;
+            for (int r = 1; r < ITEM_CAP; ++r) nrem[r] += (c[j] % ITEM_CAP) == (uint32_t)r ? 1u : 0u;
+        }
+        uint32_t crun = wave_incl_scan(csum) - csum;
+        const uint32_t fincl = wave_incl_scan(nfull);
+        uint32_t frun = fincl - nfull;
+        uint32_t total = __shfl(fincl, 63);
+        uint32_t rrun[ITEM_CAP];
+#pragma unroll
+        for (int r = ITEM_CAP - 1; r >= 1; --r) {
+            const uint32_t inc = wave_incl_scan(nrem[r]);
+            rrun[r] = total + inc - nrem[r];
+            total += __shfl(inc, 63);
+        }
+        if (lane == 0) L.nitems = total;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = 4 * lane + j;
+            if (b < NBIN) L.cnt1[b] = (uint16_t)crun;      // now the bin's first sorted position
+            const uint32_t nf = c[j] / ITEM_CAP, rm = c[j] % ITEM_CAP;
+            for (uint32_t q = 0; q < nf; ++q)
+                L.items[frun + q] = (uint32_t)b | ((crun + q * ITEM_CAP) << 8) | ((uint32_t)ITEM_CAP << 20);
+            frun += nf;
+#pragma unroll
+            for (int r = 1; r < ITEM_CAP; ++r)
+                if (rm == (uint32_t)r) L.items[rrun[r]++] = (uint32_t)b | ((crun + nf * ITEM_CAP) << 8) | (rm << 20);
+            crun += c[j];
+        }
     }
-    uint32_t crun = wave_incl_scan(csum) - csum;
-    const uint32_t fincl = wave_incl_scan(nfull);
-    uint32_t frun = fincl - nfull;
-    uint32_t total = __shfl(fincl, 63);
-    uint32_t rrun[ITEM_CAP];
-#pragma unroll
-    for (int r = ITEM_CAP - 1; r >= 1; --r) {
-        const uint32_t inc = wave_incl_scan(nrem[r]);
-        rrun[r] = total + inc - nrem[r];
-        total += __shfl(inc, 63);
-    }
-    if (lane == 0) L.nitems = total;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int b = 4 * lane + j;
-        if (b < NBIN) L.cnt[b] = crun;          // now the bin's first sorted position
-        const uint32_t nf = c[j] / ITEM_CAP, rm = c[j] % ITEM_CAP;
-        for (uint32_t q = 0; q < nf; ++q)
-            L.items[frun + q] = (uint32_t)b | ((crun + q * ITEM_CAP) << 8) | ((uint32_t)ITEM_CAP << 20);
-        frun += nf;
-#pragma unroll
-        for (int r = 1; r < ITEM_CAP; ++r)
-            if (rm == (uint32_t)r) L.items[rrun[r]++] = (uint32_t)b | ((crun + nf * ITEM_CAP) << 8) | (rm << 20);
-        crun += c[j];
-    }
-    __asm__ volatile("" ::: "memory");
-    for (int s0 = 0; s0 < nc; s0 += 64) {
+    __syncthreads();
+    for (int s0 = wave * 64; s0 < nc; s0 += TILE_THREADS) {
         const int sl = s0 + lane;
         const uint32_t key = sl < nc ? (uint32_t)L.key[sl] : NO_BIN;
-        if (key != NO_BIN) L.perm[L.cnt[key] + L.rnk[sl]] = (uint16_t)sl;
+        if (key != NO_BIN) {
+            const uint32_t off = wave ? (uint32_t)L.cnt[key] : 0u;     // wave 1 after wave 0's particles
+            L.perm[(uint32_t)L.cnt1[key] + off + L.rnk[sl]] = (uint16_t)sl;
+        }
     }
 }
 
@@ -657,7 +670,7 @@ template <int MAT>
 __device__ __forceinline__ void scatter_chunk(TileLds &L, int nc, float inv_h, float m)
 {
     __syncthreads();
-    if ((threadIdx.x >> 6) == 0) sort_chunk(L, nc);
+    sort_chunk(L, nc);
     __syncthreads();
     const int wave = threadIdx.x >> 6;
     const int nb = (int)((L.nitems + 63u) >> 6);
