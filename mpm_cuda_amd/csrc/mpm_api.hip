@@ -598,6 +598,8 @@ int mpm_set_warm_start_weights(mpm_sim *s, const float *w, int64_t count)
     MPM_HIP(hipMalloc(&s->mlp_w, (size_t)count * 4));
     MPM_HIP(hipMemcpy(s->mlp_w, w, (size_t)count * 4, hipMemcpyHostToDevice));
     s->mlp_count = count;
+    s->mlp_host.assign(w, w + count);
+    s->mlp_version += 1;
     return MPM_OK;
 }
 

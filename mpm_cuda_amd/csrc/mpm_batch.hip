@@ -7,7 +7,8 @@
 // whole batch -- counters, atomic spill fallback, active-block compaction,
 // grid update -- and one per stage and material for P2G tiles (per 64
 // scenes) and G2P, each a persistent launch over the scenes of that
-// material, plus one warm-start inference launch per warm scene.  That sequence is recorded once into a hipGraph and replayed.
+// material, plus one warm-start inference launch per set of scenes with
+// equal network weights.  That sequence is recorded once into a hipGraph and replayed.
 // Every `sort_every` steps all scenes are re-sorted by a second graph with
 // one branch per scene, on the scenes' own streams.  Sorting moves particle
 // storage between two buffers, so there is one step graph and one sort graph
@@ -36,13 +37,14 @@ struct SceneSig {
     float *P0, *P1;
     uint32_t *pid0, *pid1;
     float *mlp_w;
+    int64_t mlp_version;
     int warm;
     double dt;
     int sweeps;
     bool operator==(const SceneSig &o) const
     {
         return P0 == o.P0 && P1 == o.P1 && pid0 == o.pid0 && pid1 == o.pid1 && mlp_w == o.mlp_w &&
-               warm == o.warm && dt == o.dt && sweeps == o.sweeps;
+               mlp_version == o.mlp_version && warm == o.warm && dt == o.dt && sweeps == o.sweeps;
     }
 };
 
@@ -65,6 +67,10 @@ struct mpm_batch {
     int64_t total = 0;
     int compact_wgs = 0;
     std::vector<int> types;             // material of each scene
+    // warm-start weight sets: scenes with equal weights share one inference launch
+    std::vector<const float *> warm_params; // device weights of each set
+    std::vector<int64_t> warm_chunks;       // 64-particle chunks of each set
+    std::vector<int> warm_group;            // per scene, -1: cold
     hipEvent_t ev_fork = nullptr, ev_done = nullptr;
     std::vector<hipEvent_t> ev_join;
     std::vector<hipEvent_t> tev;        // step timing events
@@ -81,8 +87,8 @@ bool warm_on(const mpm_sim *s)
 SceneSig sig_of(const mpm_batch *b, size_t k)
 {
     const mpm_sim *s = b->sims[k];
-    return SceneSig{b->P0[k], b->P1[k], b->pid0[k], b->pid1[k], s->mlp_w, warm_on(s) ? 1 : 0, s->cfg.dt,
-                    s->cfg.svd_sweeps};
+    return SceneSig{b->P0[k], b->P1[k], b->pid0[k], b->pid1[k], s->mlp_w, s->mlp_version, warm_on(s) ? 1 : 0,
+                    s->cfg.dt, s->cfg.svd_sweeps};
 }
 
 int parity_of(const mpm_batch *b, size_t k)
@@ -133,14 +139,10 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, hipEve
     if (ev) MPM_HIP(hipEventRecord(ev[2], st));
     if ((rc = check(launch_grid_batch(st, tab, n, grid), "grid_batch"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[3], st));
-    for (size_t k = 0; k < b->sims.size(); ++k) {
-        mpm_sim *s = b->sims[k];
-        if (!warm_on(s)) continue;
-        PartView pv = view_of(s, 0);
-        pv.P = p ? b->P1[k] : b->P0[k];
-        pv.pid = p ? b->pid1[k] : b->pid0[k];
-        if ((rc = check(launch_mlp_infer(st, s->mlp_w, pv), "mlp_infer"))) return rc;
-    }
+    for (size_t w = 0; w < b->warm_params.size(); ++w)
+        if ((rc = check(launch_mlp_infer_batch(st, b->warm_params[w], tab, n, (int)w, b->warm_chunks[w]),
+                        "mlp_infer_batch")))
+            return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[4], st));
     for (int m = MPM_JELLO; m <= MPM_WATER; ++m) {
         const int wgs = g2p_wgs(b, m, g2p_threads);
@@ -215,10 +217,35 @@ int capture(mpm_batch *b, hipGraphExec_t *out, bool sort, int p)
 }
 
 // Device tables for both parities; graphs when the shape asks for them.
+// Warm scenes grouped by identical weights.
+void group_warm(mpm_batch *b)
+{
+    b->warm_params.clear();
+    b->warm_chunks.clear();
+    b->warm_group.assign(b->sims.size(), -1);
+    std::vector<size_t> rep;                        // first scene of each set
+    for (size_t k = 0; k < b->sims.size(); ++k) {
+        mpm_sim *s = b->sims[k];
+        if (!warm_on(s)) continue;
+        int g = -1;
+        for (size_t r = 0; r < rep.size() && g < 0; ++r)
+            if (b->sims[rep[r]]->mlp_host == s->mlp_host) g = (int)r;
+        if (g < 0) {
+            g = (int)rep.size();
+            rep.push_back(k);
+            b->warm_params.push_back(s->mlp_w);
+            b->warm_chunks.push_back(0);
+        }
+        b->warm_group[k] = g;
+        b->warm_chunks[g] += (s->groups[0].n + 63) / 64;
+    }
+}
+
 int build(mpm_batch *b)
 {
     const size_t n = b->sims.size();
     MPM_HIP(hipStreamSynchronize(b->stream));       // no replay still reads the tables
+    group_warm(b);
     for (int p = 0; p < 2; ++p) {
         std::vector<BatchSlice> h(n);
         for (size_t k = 0; k < n; ++k) {
@@ -228,6 +255,7 @@ int build(mpm_batch *b)
             set_parity(b, k, p);
             fill_batch_slice(s, h[k]);
             h[k].mat.warm_sweeps = warm_on(s) ? std::max(1, s->cfg.warm_sweeps) : 0;
+            h[k].warm_group = b->warm_group[k];
             s->P = P; s->P_alt = Pa; s->pid = id; s->pid_alt = ida;
         }
         MPM_HIP(hipMemcpy(b->tab[p], h.data(), n * sizeof(BatchSlice), hipMemcpyHostToDevice));

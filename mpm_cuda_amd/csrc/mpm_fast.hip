@@ -1207,6 +1207,7 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
     b.pv = view_of(s, 0);
     b.dt = (float)s->cfg.dt;
     b.sweeps = s->cfg.svd_sweeps;
+    b.warm_group = -1;
     b.grid_mp = s->grid_mp;
     b.grid_v = s->grid_v;
     b.staging = s->staging;

@@ -163,6 +163,8 @@ struct BatchSlice {
     PartView pv;
     float dt;
     int sweeps;
+    int warm_group;                 // warm-start weight set of the scene, -1: cold start
+    int reserved;
     float4 *grid_mp, *grid_v, *staging;
     uint32_t *flags, *list, *counters, *spill;
     const uint4 *desc;              // P2G tiles of the last sort
@@ -266,9 +268,12 @@ struct mpm_sim {
     int64_t sorted_at_launch = 0;           // tile_launches when the last sort ran
     int64_t sort_interval = 0;              // learned interval: spills ended the last one after this many steps
     int last_phase = -1;
-    // warm-start MLP weights (device)
+    // warm-start MLP weights (device), a host copy and a version that
+    // changes with every mpm_set_warm_start_weights
     float *mlp_w = nullptr;
     int64_t mlp_count = 0;
+    std::vector<float> mlp_host;
+    int64_t mlp_version = 0;
     // phase timing (HIP events on `stream`): ids 0-3 phases, 4-7 kernels
     static constexpr int TIMED_IDS = 8;
     double tacc_ms[TIMED_IDS] = {};

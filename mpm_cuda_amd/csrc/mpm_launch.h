@@ -39,6 +39,8 @@ int launch_compact_batch(hipStream_t st, const BatchSlice *tab, int nslices, int
 int launch_grid_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
 int launch_g2p_batch(hipStream_t st, int mat, const BatchSlice *tab, int nslices, int wgs, int threads);
 int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv);
+int launch_mlp_infer_batch(hipStream_t st, const float *params, const BatchSlice *tab, int nslices, int group,
+                           int64_t chunks);
 
 // Scene sampling and files (scene.cpp)
 int sample_object(const mpm_config &cfg, const mpm_object &obj, std::vector<float> &xyz);

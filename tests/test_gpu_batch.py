@@ -154,16 +154,19 @@ def test_autotune_keeps_results(gpu):
 
 
 def test_batch_warm_start(gpu):
-    """Snow scenes with the trained warm start (two sweeps, cold fallback) in
-    a batch match the same scenes with the warm start stepped one by one."""
+    """Snow and jello scenes with the learned warm start (two sweeps, cold
+    fallback) in a batch match the same scenes with the warm start stepped
+    one by one.  Two weight sets (the trained one and a perturbed copy) give
+    two batched inference launches."""
     w = warm_start.load_weights()
+    w2 = (w + 0.01 * np.random.default_rng(40).normal(size=w.size)).astype(np.float32)
     specs = [SPECS[1], SPECS[3], SPECS[0]]
     extra = {"warm_start": capi.WARM_MLP, "warm_sweeps": 2}
     A = _pairs(specs=specs, extra_cfg=extra)
     B = _pairs(specs=specs, extra_cfg=extra)
     try:
-        for p in A + B:
-            p.gpu.set_warm_start_weights(w)
+        for k, p in enumerate(A + B):
+            p.gpu.set_warm_start_weights(w2 if k % len(specs) == 1 else w)
         with capi.Batch([p.gpu for p in A], capi.batch_shape(sort_every=6)) as b:
             b.run(10)
         for p in B:
