@@ -1043,9 +1043,9 @@ static bool sort_due(mpm_sim *s)
     return true;
 }
 
-// Copy the spill count of group `group`'s tile launch to the host; the
-// launch counter advances after the last group.
-static int note_spills(mpm_sim *s, int group)
+// Pinned host word that receives the spill count of group `group`'s tile
+// launch (written by the spill kernel itself: no copy in the stream).
+static int spill_host_word(mpm_sim *s, int group, uint32_t **word)
 {
     if (!s->spill_host) {
         MPM_HIP(hipHostMalloc((void **)&s->spill_host, mpm_sim::SPILL_SLOTS * 256 * sizeof(uint32_t),
@@ -1055,8 +1055,15 @@ static int note_spills(mpm_sim *s, int group)
             MPM_HIP(hipEventCreateWithFlags(&s->spill_ev[k], hipEventDisableTiming));
     }
     const int slot = (int)(s->tile_launches % mpm_sim::SPILL_SLOTS);
-    MPM_HIP(hipMemcpyAsync(s->spill_host + slot * 256 + group, s->counters + CT_SPILL, sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s->stream));
+    *word = s->spill_host + slot * 256 + group;
+    return MPM_OK;
+}
+
+// After group `group`'s tile launch: the launch counter advances after the
+// last group, with an event that marks its spill counts as written.
+static int note_spills(mpm_sim *s, int group)
+{
+    const int slot = (int)(s->tile_launches % mpm_sim::SPILL_SLOTS);
     if (group + 1 == (int)s->groups.size()) {
         MPM_HIP(hipEventRecord(s->spill_ev[slot], s->stream));
         s->spill_launch[slot] = s->tile_launches;
@@ -1114,8 +1121,13 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     const uint32_t *tc = s->tile_counts + group;
     const uint32_t cap = (uint32_t)bound;
     const uint32_t scap = (uint32_t)s->staging_cap;
-    int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
-    if (rc) return hip_fail((hipError_t)rc, "zero");
+    int rc = MPM_OK;
+    if (group > 0) {        // the step's counter reset cleared group 0's count
+        rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
+        if (rc) return hip_fail((hipError_t)rc, "zero");
+    }
+    uint32_t *hw = nullptr;
+    if ((rc = spill_host_word(s, group, &hw))) return rc;
     dim3 gr(wg), bl(TILE_THREADS);
     {
     PhaseTimer tk(s, TK_P2G);
@@ -1127,7 +1139,8 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
-    rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags);
+    rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags,
+                          hw);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }
@@ -1146,6 +1159,8 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     const int sw = s->cfg.svd_sweeps;
     int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
     if (rc) return hip_fail((hipError_t)rc, "zero");
+    uint32_t *hw = nullptr;
+    if ((rc = spill_host_word(s, group, &hw))) return rc;
     dim3 gr(wg), bl(TILE_THREADS);
     {
     PhaseTimer tk(s, TK_G2P);
@@ -1158,7 +1173,7 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "g2p2g_block");
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt_next, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp,
-                          s->flags);
+                          s->flags, hw);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }

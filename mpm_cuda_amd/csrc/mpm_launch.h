@@ -18,7 +18,8 @@ int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv
                const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads);
 int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);
 int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
-                     const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags);
+                     const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
+                     uint32_t *host_count);
 
 // FAST form (mpm_fast.hip)
 int fast_prepare(mpm_sim *s);                 // (re)sort particles and build P2G tiles when due
