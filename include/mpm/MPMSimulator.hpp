@@ -86,7 +86,8 @@ struct ParticleGroup {
 
 class MPMSimulator {
 public:
-    double adp_dt = 0.0;        // MPMSimulator.cuh:85 (the reference never adapts it)
+    double max_dt = 0.0;        // MPMSimulator.cuh:70
+    double adp_dt = 0.0;        // :85 (the reference never adapts it; with a dt controller the last dt)
     double current_time = 0.0;  // :86
     int current_frame = 0;      // :87
     std::vector<ParticleGroup> mParticlesGroupsVec;  // :89
@@ -103,7 +104,8 @@ public:
         c.h = gap; c.dt = dt;
         c.nx = (int)nodeNumDimX; c.ny = (int)nodeNumDimY; c.nz = (int)nodeNumDimZ;
         c.ppc = (int)particleNumPerCell;
-        adp_dt = dt;
+        max_dt = adp_dt = dt;
+        adaptive_ = c.dt_mode != MPM_DT_FIXED;
         mGrid.h = gap;
         mGrid.nodeNumDimX = nodeNumDimX; mGrid.nodeNumDimY = nodeNumDimY; mGrid.nodeNumDimZ = nodeNumDimZ;
         for (int k = 0; k < 3; ++k) mGrid.originCorner[k] = c.origin[k];
@@ -147,7 +149,8 @@ public:
     // MPMCUDA.cu:793-925
     void step()
     {
-        throw_if(mpm_step(sim_, adp_dt));
+        throw_if(mpm_step(sim_, adaptive_ ? max_dt : adp_dt));
+        if (adaptive_) adp_dt = mpm_last_dt(sim_);
         current_time = mpm_current_time(sim_);
         current_frame = (int)mpm_current_frame(sim_);
     }
@@ -165,6 +168,7 @@ public:
 
 private:
     mpm_sim *sim_ = nullptr;
+    bool adaptive_ = false;
 };
 
 // FilesIO.cpp:6-23

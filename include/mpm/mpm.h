@@ -60,6 +60,22 @@ enum { MPM_FORM_PLAIN = 0, MPM_FORM_FAST = 1, MPM_FORM_FUSED = 2 };
 /* Warm start for the polar decomposition in G2P. */
 enum { MPM_WARM_NONE = 0, MPM_WARM_MLP = 1 };
 
+/* Step-size control.  The reference computes a deformation factor
+ * min(10, max_p max(J, 1/J)^3) after every step and never applies it
+ * (MPMCUDA.cu:613-619, 700-708, 915-920): adp_dt stays max_dt.
+ *   FIXED       every step takes the dt given to mpm_step (the reference);
+ *   CFL         dt = min(max_dt, cfl h / max_p(|v_p| + c_p)), c_p the
+ *               particle's elastic wave speed (jello sqrt((lam + 2 mu) / rho),
+ *               snow with the hardening factor exp(xi (1 - J_p)), water
+ *               sqrt(k gamma J^(1 - gamma) / rho));
+ *   CFL_DEFORM  CFL, and at most max_dt divided by the reference's factor
+ *               clamped to [1, 10] (jello det F, snow det F_p; water
+ *               particles do not enter it, as in the reference).
+ * The maxima are reduced on the device by each step's G2P and read by the
+ * next step on the device (no host round trip); mpm_last_dt and
+ * mpm_current_time then synchronize.  PLAIN and FAST forms only. */
+enum { MPM_DT_FIXED = 0, MPM_DT_CFL = 1, MPM_DT_CFL_DEFORM = 2 };
+
 /* Particle volume: reference quirk V_p = h^3 (MPMSimulator.cpp:263, occupancy
  * counts particles), or h^3 / ppc.  Dynamics do not depend on the choice. */
 enum { MPM_VOLUME_REF = 0, MPM_VOLUME_PPC = 1 };
@@ -83,7 +99,9 @@ typedef struct mpm_config {
     int32_t warm_sweeps;      /* Jacobi sweeps after a warm start */
     int32_t tile;             /* FAST: particles per P2G tile, 0 = tuned default */
     int32_t threads;          /* threads per workgroup for particle kernels, 0 = default */
-    int32_t reserved[5];
+    int32_t dt_mode;          /* MPM_DT_*; the dt given to mpm_step is max_dt */
+    float cfl;                /* CFL number of the controller, in (0, 1] (0.5) */
+    int32_t reserved[3];
 } mpm_config;
 
 typedef struct mpm_material {
@@ -219,9 +237,14 @@ int mpm_phase_times(mpm_sim *sim, double ms[4], int64_t counts[4], int64_t *stat
  * form), [3] unused.  Accumulated since the previous call, then cleared. */
 int mpm_kernel_times(mpm_sim *sim, double ms[4], int64_t counts[4]);
 
-/* Simulation clock: current_time / current_frame (MPMSimulator.cuh:86-87). */
+/* Simulation clock: current_time / current_frame (MPMSimulator.cuh:86-87).
+ * With a dt controller the time accumulates on the device and reading it
+ * synchronizes the simulator's stream. */
 double mpm_current_time(const mpm_sim *sim);
 int64_t mpm_current_frame(const mpm_sim *sim);
+/* adp_dt (MPMSimulator.cuh:85): the dt of the last step (0 before the first);
+ * synchronizes with a dt controller. */
+double mpm_last_dt(const mpm_sim *sim);
 
 /* OutputBego (FilesIO.h:10): write positions as a Houdini BGEO particle file
  * with one "position" attribute. */

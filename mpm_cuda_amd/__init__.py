@@ -99,7 +99,8 @@ class MPMSimulator:
                                 nz=int(nodeNumDimZ), ppc=int(particleNumPerCell), **cfg)
         self._sim = capi.Sim(c)
         self.max_dt = float(dt)
-        self.adp_dt = float(dt)       # MPMSimulator.cpp:184-185 (the controller never changes it)
+        self.adp_dt = float(dt)       # MPMSimulator.cpp:184-185 (the reference never changes it)
+        self._adaptive = int(cfg.get("dt_mode", capi.DT_FIXED)) != capi.DT_FIXED
         for info in objInitInfoVec:
             self._sim.add_object(info.c_struct())
 
@@ -116,8 +117,14 @@ class MPMSimulator:
         return self._sim.current_frame
 
     def step(self) -> None:
-        """One step of adp_dt (MPMCUDA.cu:793-925)."""
-        self._sim.step(self.adp_dt)
+        """One step of adp_dt (MPMCUDA.cu:793-925).  With a dt controller
+        (``dt_mode``) the step takes at most max_dt and adp_dt reports the dt
+        the controller chose."""
+        if self._adaptive:
+            self._sim.step(self.max_dt)
+            self.adp_dt = self._sim.last_dt
+        else:
+            self._sim.step(self.adp_dt)
 
     def totalParticlesNum(self) -> int:
         return self._sim.total

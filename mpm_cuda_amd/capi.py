@@ -20,6 +20,7 @@ BC_AXIS, BC_STICKY = 0, 1
 WATER_C_APIC, WATER_C_GRADV = 0, 1
 FORM_PLAIN, FORM_FAST, FORM_FUSED = 0, 1, 2
 WARM_NONE, WARM_MLP = 0, 1
+DT_FIXED, DT_CFL, DT_CFL_DEFORM = 0, 1, 2
 VOLUME_REF, VOLUME_PPC = 0, 1
 SHAPE_MESH, SHAPE_BOX, SHAPE_SPHERE = 0, 1, 2
 P_POS, P_VEL, P_C, P_F, P_JP, P_TAU = 0, 1, 2, 3, 4, 5
@@ -35,7 +36,8 @@ class MpmConfig(C.Structure):
                 ("bc_pad", C.c_int32), ("bc_mode", C.c_int32), ("ppc", C.c_int32), ("volume_mode", C.c_int32),
                 ("seed", C.c_uint64), ("form", C.c_int32), ("sort_every", C.c_int32),
                 ("svd_sweeps", C.c_int32), ("warm_start", C.c_int32), ("warm_sweeps", C.c_int32),
-                ("tile", C.c_int32), ("threads", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("tile", C.c_int32), ("threads", C.c_int32), ("dt_mode", C.c_int32), ("cfl", C.c_float),
+                ("reserved", C.c_int32 * 3)]
 
 
 class MpmMaterial(C.Structure):
@@ -103,6 +105,7 @@ _SIGS = {
     "mpm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mpm_current_time": (C.c_double, [C.c_void_p]),
     "mpm_current_frame": (C.c_int64, [C.c_void_p]),
+    "mpm_last_dt": (C.c_double, [C.c_void_p]),
     "mpm_write_bgeo": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64]),
     "mpm_set_warm_start_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "mpm_destroy": (None, [C.c_void_p]),
@@ -321,6 +324,11 @@ class Sim:
     @property
     def current_frame(self) -> int:
         return lib().mpm_current_frame(self._h)
+
+    @property
+    def last_dt(self) -> float:
+        """dt of the last step (adp_dt); synchronizes with a dt controller."""
+        return lib().mpm_last_dt(self._h)
 
 
 BATCH_STAGES = ("p2g", "grid", "g2p")

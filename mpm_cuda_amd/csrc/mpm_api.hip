@@ -110,6 +110,11 @@ static MatP make_matp(const mpm_material &m, double mass, double vol)
     p.eos_k = (float)m.eos_k;
     p.eos_gamma = (float)m.eos_gamma;
     p.warm_sweeps = 0;
+    // squared wave speed at rest for the step-size controller: P waves
+    // (lam + 2 mu) / rho, water's bulk modulus k gamma over rho
+    const double rho = mass / vol;
+    const double mu = m.E / (2.0 * (1.0 + m.nu)), lam = m.E * m.nu / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu));
+    p.c2 = (float)((m.type == MPM_WATER ? m.eos_k * m.eos_gamma : lam + 2.0 * mu) / rho);
     return p;
 }
 
@@ -159,6 +164,8 @@ void mpm_config_default(mpm_config *c)
     c->warm_sweeps = 1;
     c->tile = 0;
     c->threads = 256;
+    c->dt_mode = MPM_DT_FIXED;   // adp_dt = max_dt (MPMSimulator.cpp:184-185)
+    c->cfl = 0.5f;
 }
 
 int mpm_material_init(mpm_material *m, double E, double nu, double rho, int type)
@@ -186,6 +193,12 @@ int mpm_create(const mpm_config *cfg, mpm_sim **out)
         return set_error(MPM_ERR_INVALID, "unknown form");
     if (cfg->bc_mode != MPM_BC_AXIS && cfg->bc_mode != MPM_BC_STICKY) return set_error(MPM_ERR_INVALID, "unknown bc_mode");
     if (cfg->svd_sweeps < 1 || cfg->svd_sweeps > 16) return set_error(MPM_ERR_INVALID, "svd_sweeps out of range [1,16]");
+    if (cfg->dt_mode < MPM_DT_FIXED || cfg->dt_mode > MPM_DT_CFL_DEFORM) return set_error(MPM_ERR_INVALID, "unknown dt_mode");
+    if (cfg->dt_mode != MPM_DT_FIXED && cfg->form == MPM_FORM_FUSED)
+        return set_error(MPM_ERR_INVALID, "the FUSED form scatters the next step before its dt is known; "
+                                          "use dt_mode FIXED or the FAST form");
+    if (cfg->dt_mode != MPM_DT_FIXED && !(cfg->cfl > 0.0f && cfg->cfl <= 1.0f))
+        return set_error(MPM_ERR_INVALID, "cfl must be in (0, 1]");
     int threads = cfg->threads ? cfg->threads : 256;
     if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
         return set_error(MPM_ERR_INVALID, "threads must be 64, 128, 256 or 512");
@@ -203,6 +216,7 @@ int mpm_create(const mpm_config *cfg, mpm_sim **out)
     g.ox = (float)cfg->origin[0]; g.oy = (float)cfg->origin[1]; g.oz = (float)cfg->origin[2];
     g.gx = (float)cfg->gravity[0]; g.gy = (float)cfg->gravity[1]; g.gz = (float)cfg->gravity[2];
     g.pad = cfg->bc_pad; g.bc_mode = cfg->bc_mode;
+    g.dtp = nullptr;
     s->nblocks = (int64_t)g.nbx * g.nby * g.nbz;
     int rc = MPM_OK;
     auto fail = [&](hipError_t e, const char *w) { rc = hip_fail(e, w); };
@@ -218,6 +232,11 @@ int mpm_create(const mpm_config *cfg, mpm_sim **out)
     if (!rc && (e = hipMemsetAsync(s->grid_v, 0, gbytes, s->stream)) != hipSuccess) fail(e, "memset");
     if (!rc && (e = hipMemsetAsync(s->flags, 0, (size_t)s->nblocks * 4, s->stream)) != hipSuccess) fail(e, "memset");
     if (!rc && (e = hipMemsetAsync(s->counters, 0, CT_COUNT * 4, s->stream)) != hipSuccess) fail(e, "memset");
+    if (!rc && cfg->dt_mode != MPM_DT_FIXED) {
+        if ((e = hipMalloc(&s->dtc, sizeof(DtCtl))) != hipSuccess) fail(e, "hipMalloc dt state");
+        else if ((e = hipMemsetAsync(s->dtc, 0, sizeof(DtCtl), s->stream)) != hipSuccess) fail(e, "memset");
+        else g.dtp = &s->dtc->dt;
+    }
     if (!rc && (e = hipStreamSynchronize(s->stream)) != hipSuccess) fail(e, "sync");
     if (rc) { mpm_destroy(s); return rc; }
     *out = s;
@@ -276,6 +295,7 @@ int mpm_add_particles(mpm_sim *s, const mpm_material *mat, int64_t n, const floa
     s->tau_valid = false;
     s->sorted_valid = false;
     s->primed = false;
+    s->dt_valid = false;
     if (group) *group = (int32_t)(s->groups.size() - 1);
     return MPM_OK;
 }
@@ -299,7 +319,33 @@ int mpm_step_phase(mpm_sim *s, int32_t phase, double dt)
     if (phase < MPM_PHASE_P2G || phase > MPM_PHASE_G2P) return set_error(MPM_ERR_INVALID, "unknown phase");
     float fdt = (float)(dt > 0 ? dt : s->cfg.dt);
     s->primed = false;
-    return run_phase(s, phase, fdt);
+    // single phases always take the given dt; the controller re-reduces its
+    // maxima at the next full step
+    s->dt_valid = false;
+    const float *dtp = s->g.dtp;
+    s->g.dtp = nullptr;
+    int rc = run_phase(s, phase, fdt);
+    s->g.dtp = dtp;
+    return rc;
+}
+
+// Step-size control at the start of a step: the maxima of the current state
+// (reduced by the previous G2P, or scanned now) give the step's dt on the
+// device.
+static int dt_control(mpm_sim *s, double max_dt)
+{
+    int rc = MPM_OK;
+    if (!s->dt_valid) {
+        rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_SPEED, 2), "zero");
+        for (size_t k = 0; k < s->groups.size() && !rc; ++k)
+            rc = check_launch(launch_dt_scan(s->stream, s->groups[k].mat, view_of(s, (int)k), s->counters), "dt_scan");
+        if (rc) return rc;
+    }
+    rc = check_launch(launch_dt_control(s->stream, s->dtc, s->counters, (float)max_dt,
+                                        (float)(s->cfg.cfl * s->cfg.h), s->cfg.dt_mode),
+                      "dt_control");
+    if (!rc) s->dt_valid = true;        // this step's G2P reduces the maxima of the state it writes
+    return rc;
 }
 
 int mpm_step(mpm_sim *s, double dt)
@@ -307,6 +353,10 @@ int mpm_step(mpm_sim *s, double dt)
     if (!s) return set_error(MPM_ERR_INVALID, "sim is null");
     double d = dt > 0 ? dt : s->cfg.dt;
     float fdt = (float)d;
+    if (s->dtc) {
+        int rc = dt_control(s, d);
+        if (rc) return rc;
+    }
     if (s->cfg.form == MPM_FORM_FUSED) {
         int rc = fused_step(s, fdt);
         if (rc) return rc;
@@ -317,7 +367,10 @@ int mpm_step(mpm_sim *s, double dt)
         }
     }
     s->steps += 1;          // ++current_frame (MPMCUDA.cu:923)
-    s->time += d;           // current_time += adp_dt (MPMCUDA.cu:924)
+    if (!s->dtc) {
+        s->time += d;       // current_time += adp_dt (MPMCUDA.cu:924); with a controller the device adds it
+        s->last_dt = d;
+    }
     return MPM_OK;
 }
 
@@ -413,6 +466,7 @@ int mpm_write_particles(mpm_sim *s, int32_t group, int32_t field, const float *i
     if (field == MPM_P_F || field == MPM_P_JP) s->tau_valid = false;
     if (field == MPM_P_POS) s->sorted_valid = false;
     s->primed = false;
+    s->dt_valid = false;
     return MPM_OK;
 }
 
@@ -516,7 +570,7 @@ int mpm_stats(mpm_sim *s, double st[8])
     st[2] = c[CT_FALLBACK];
     st[3] = c[CT_REJECT];
     st[4] = (double)s->sorts;
-    st[5] = s->time;
+    st[5] = mpm_current_time(s);
     st[6] = c[CT_TILES];
     st[7] = c[CT_SPILL];
     return MPM_OK;
@@ -573,7 +627,30 @@ int mpm_kernel_times(mpm_sim *s, double ms[4], int64_t counts[4])
     return MPM_OK;
 }
 
-double mpm_current_time(const mpm_sim *s) { return s ? s->time : 0.0; }
+// With a controller the clock lives on the device; reading it synchronizes
+// the stream (the handle's host copy is refreshed).
+static bool read_dt_state(const mpm_sim *s, DtCtl *out)
+{
+    if (!s->dtc) return false;
+    if (hipMemcpyAsync(out, s->dtc, sizeof *out, hipMemcpyDeviceToHost, s->stream) != hipSuccess) return false;
+    return hipStreamSynchronize(s->stream) == hipSuccess;
+}
+
+double mpm_current_time(const mpm_sim *s)
+{
+    if (!s) return 0.0;
+    DtCtl c;
+    if (read_dt_state(s, &c)) const_cast<mpm_sim *>(s)->time = c.time;
+    return s->time;
+}
+
+double mpm_last_dt(const mpm_sim *s)
+{
+    if (!s) return 0.0;
+    DtCtl c;
+    if (read_dt_state(s, &c)) const_cast<mpm_sim *>(s)->last_dt = c.last;
+    return s->last_dt;
+}
 int64_t mpm_current_frame(const mpm_sim *s) { return s ? s->steps : 0; }
 
 int mpm_write_bgeo(const char *path, const float *xyz, int64_t n)
@@ -610,7 +687,7 @@ void mpm_destroy(mpm_sim *s)
     void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
                     s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
                     s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
-                    s->staging, s->blk_tile, s->spill};
+                    s->staging, s->blk_tile, s->spill, s->dtc};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (s->spill_host) (void)hipHostFree(s->spill_host);
     for (hipEvent_t e : s->spill_ev) if (e) (void)hipEventDestroy(e);

@@ -357,6 +357,7 @@ void after_step(mpm_batch *b)
     for (mpm_sim *s : b->sims) {
         s->steps += 1;
         s->time += s->cfg.dt;
+        s->last_dt = s->cfg.dt;
         s->steps_since_sort += 1;
         s->tau_valid = true;
         s->last_phase = MPM_PHASE_G2P;
@@ -428,6 +429,8 @@ int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *sha
         mpm_sim *s = sims[k];
         if (!s) return set_error(MPM_ERR_INVALID, "scene is null");
         if (s->cfg.form != MPM_FORM_FAST) return set_error(MPM_ERR_INVALID, "batched scenes use the FAST form");
+        if (s->cfg.dt_mode != MPM_DT_FIXED)
+            return set_error(MPM_ERR_INVALID, "batched scenes step with their fixed dt (dt_mode FIXED)");
         if (s->groups.size() != 1 || s->groups[0].n <= 0)
             return set_error(MPM_ERR_INVALID, "a batched scene holds exactly one non-empty particle group");
         for (int j = 0; j < k; ++j)

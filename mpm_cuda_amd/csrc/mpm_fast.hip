@@ -778,6 +778,7 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
     const uint32_t nt8 = round_xcd(nt);
+    dt = step_dt(g, dt);
     for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
         const uint32_t tt = xcd_spread(k8, nt8);
         if (tt >= nt) continue;

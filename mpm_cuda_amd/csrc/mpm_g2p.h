@@ -118,8 +118,24 @@ __device__ __forceinline__ void gather_grid(const GridP &g, const Stencil &s, co
 
 // Updated particle state kept in registers for a following scatter.
 struct G2POut {
-    float x[3], v[3], C[9], tau[6], S;
+    float x[3], v[3], C[9], tau[6];
+    float S;            // snow J_p, water J, jello det F
 };
+
+// Step-size controller inputs of one particle: |v| + c, with c the elastic
+// wave speed at its deformation (snow: hardened by exp(xi (1 - J_p)); water:
+// bulk modulus k gamma J^-gamma at density rho / J), and max(J, 1/J), the
+// measure of the reference's unused factor (MPMCUDA.cu:613-619, 700-708;
+// jello det F, snow det F_p; the reference never sets it for water, :710-713).
+template <int MAT>
+__device__ __forceinline__ void dt_measures(const MatP &m, const float v[3], float J, float &speed, float &deform)
+{
+    float c2 = m.c2;
+    if (MAT == MPM_SNOW) c2 *= expf(m.xi * (1.0f - J));
+    if (MAT == MPM_WATER) c2 *= powf(J, 1.0f - m.eos_gamma);
+    speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + sqrtf(fmaxf(c2, 0.0f));
+    deform = MAT == MPM_WATER ? 0.0f : (J > 0.0f ? fmaxf(J, 1.0f / J) : 1e30f);
+}
 
 template <int MAT, class Fetch>
 __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, const PartView &pv, int i, float dt,
@@ -169,9 +185,10 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         matmul3<float>(A, F, Fn);
 #pragma unroll
         for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fn[k]);
-        if (!(det3<float>(Fn) > 0.f)) bits |= MPM_H_DET_NONPOS;
+        const float Jn = det3<float>(Fn);
+        if (!(Jn > 0.f)) bits |= MPM_H_DET_NONPOS;
         if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters, &pv, i);
-        out.S = 1.0f;
+        out.S = Jn;
     } else if (MAT == MPM_SNOW) {                                        // :620-709
         float Ft[9], U[9], sg[3], V[9];
         matmul3<float>(A, F, Ft);

@@ -45,6 +45,7 @@ struct GridP {
     float ox, oy, oz;
     float gx, gy, gz;
     int pad, bc_mode;
+    const float *dtp;   // step-size controller on: the step's dt is read here (device memory), else null
 };
 
 struct MatP {
@@ -53,6 +54,7 @@ struct MatP {
     float theta_c, theta_s, xi;
     float eos_k, eos_gamma;
     int warm_sweeps;    // > 0: G2P SVDs start from the stored quaternion (PL_Q) with this many sweeps
+    float c2;           // squared elastic wave speed at rest: (lam + 2 mu) / rho, water k gamma / rho
 };
 
 // Per-launch view of one group's particles.
@@ -74,7 +76,17 @@ enum Counter : int {
     CT_REJECT = 5,      // warm-start rejections
     CT_TILES = 6,       // P2G tiles (FAST / FUSED forms)
     CT_SPILLED = 7,     // reserved
+    CT_SPEED = 8,       // step-size controller: max_p |v_p| + c_p (float bits), since the last control
+    CT_DEFORM = 9,      // step-size controller: max_p max(J, 1/J) (float bits)
     CT_COUNT = 16
+};
+
+// Device state of the step-size controller (one per simulator).
+struct DtCtl {
+    float dt;           // dt of the current step, read by the step's kernels
+    float pad;
+    double time;        // simulated time after the last step
+    double last;        // dt of the last step
 };
 
 // Memory accesses with their address space stated.  Pointers that reach
@@ -129,6 +141,20 @@ __device__ __forceinline__ uint32_t gatomic_add(uint32_t *p, uint32_t v)
 __device__ __forceinline__ void gatomic_or(uint32_t *p, uint32_t v)
 {
     (void)__hip_atomic_fetch_or(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The step's dt: the controller's value in device memory when it is on.
+__device__ __forceinline__ float step_dt(const GridP &g, float dt) { return g.dtp ? *gp(g.dtp) : dt; }
+
+// Largest v over the wave (all 64 lanes take part; NaN loses), merged into
+// *dst with one atomic max.  Non-negative floats order like their bit
+// patterns as unsigned integers.
+__device__ __forceinline__ void wave_max_to(uint32_t *dst, float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
+    if ((threadIdx.x & 63) == 0 && v > 0.f)
+        (void)__hip_atomic_fetch_max(gp(dst), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Workgroups are dispatched to the 8 XCDs of the device round-robin
@@ -274,6 +300,10 @@ struct mpm_sim {
     int64_t mlp_count = 0;
     std::vector<float> mlp_host;
     int64_t mlp_version = 0;
+    // step-size controller (cfg.dt_mode != MPM_DT_FIXED)
+    mpm::DtCtl *dtc = nullptr;
+    bool dt_valid = false;                  // counters hold the maxima of the current state
+    double last_dt = 0;                     // fixed mode: dt of the last step
     // phase timing (HIP events on `stream`): ids 0-3 phases, 4-7 kernels
     static constexpr int TIMED_IDS = 8;
     double tacc_ms[TIMED_IDS] = {};

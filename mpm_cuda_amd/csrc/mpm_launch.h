@@ -20,6 +20,8 @@ int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sw
 int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                      const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
                      uint32_t *host_count);
+int launch_dt_scan(hipStream_t st, const MatP &m, const PartView &pv, uint32_t *counters);
+int launch_dt_control(hipStream_t st, DtCtl *c, uint32_t *counters, float max_dt, float cfl_h, int mode);
 
 // FAST form (mpm_fast.hip)
 int fast_prepare(mpm_sim *s);                 // (re)sort particles and build P2G tiles when due

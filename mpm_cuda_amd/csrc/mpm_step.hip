@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(GridP g, MatP mat, PartView 
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pv.n) return;
+    dt = step_dt(g, dt);
     float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
     Stencil s;
     uint32_t bits = make_stencil(g, x, s);
@@ -171,6 +172,7 @@ __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uin
 {
     const uint32_t nact = counters[CT_ACTIVE];
     const uint32_t nact8 = round_xcd(nact);
+    dt = step_dt(g, dt);
     for (uint32_t k8 = blockIdx.x; k8 < nact8; k8 += gridDim.x) {
         const uint32_t k = xcd_spread(k8, nact8);
         if (k >= nact) continue;
@@ -211,10 +213,12 @@ int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters,
 // G2P, deformation update and advection, one thread per particle
 // (MPMCUDA.cu:474-741).  With write_tau the Kirchhoff stress of the updated
 // state is stored for the next P2G, so a FAST-form step needs one SVD.
+// With sd, the step-size controller's measures of the updated particle are
+// returned in sd[0] (speed) and sd[1] (deformation).
 template <int MAT>
 __device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, const PartView &pv, int i, float dt,
                                              const float4 *__restrict__ gv, bool write_tau, int sweeps,
-                                             uint32_t *counters)
+                                             uint32_t *counters, float *sd = nullptr)
 {
     G2POut o;
     GlobalFetch f{gv};
@@ -224,8 +228,12 @@ __device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, co
         for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
     }
     if (bits) gatomic_or(counters + CT_HEALTH, bits);
+    if (sd) dt_measures<MAT>(mat, o.v, o.S, sd[0], sd[1]);
 }
 
+// With the step-size controller on (g.dtp), every wave also merges its
+// particles' maxima into counters[CT_SPEED], counters[CT_DEFORM] for the
+// next step's control.
 template <int MAT>
 __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, float dt,
                                              const float4 *__restrict__ gv, int write_tau,
@@ -233,9 +241,17 @@ __global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, flo
 {
     const uint32_t nb = ((uint32_t)pv.n + blockDim.x - 1) / blockDim.x;
     const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(nb));      // launched with round_xcd(nb) workgroups
+    if (blk >= nb) return;                                             // whole workgroup
     const int i = (int)(blk * blockDim.x + threadIdx.x);
-    if (blk >= nb || i >= pv.n) return;
-    g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters);
+    if (!g.dtp) {
+        if (i < pv.n) g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters);
+        return;
+    }
+    dt = step_dt(g, dt);
+    float sd[2] = {0.0f, 0.0f};
+    if (i < pv.n) g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters, sd);
+    wave_max_to(counters + CT_SPEED, sd[0]);
+    wave_max_to(counters + CT_DEFORM, sd[1]);
 }
 
 // Particles whose stencil left their block arena (listed by the block
@@ -276,6 +292,7 @@ __global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView p
     const uint32_t listed = counters[CT_SPILL];
     if (host_count && blockIdx.x == 0 && threadIdx.x == 0) *host_count = listed;
     const uint32_t n = min(listed, list_cap);
+    dt = step_dt(g, dt);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int i = (int)list[k];
         if (i < 0 || i >= pv.n) continue;
@@ -501,6 +518,72 @@ int launch_g2p_batch(hipStream_t st, int mat, const BatchSlice *tab, int nslices
     case MPM_SNOW: hipLaunchKernelGGL(k_g2p_batch<MPM_SNOW>, dim3(wgs), dim3(threads), 0, st, tab, nslices); break;
     default: hipLaunchKernelGGL(k_g2p_batch<MPM_WATER>, dim3(wgs), dim3(threads), 0, st, tab, nslices); break;
     }
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Step-size controller (no counterpart in the reference, which computes
+// min(10, max_p max(J, 1/J)^3) after every step and never uses it,
+// MPMCUDA.cu:915-920).  k_dt_scan reduces the controller's maxima over the
+// current state (first step, or after the host rewrote particles); after
+// that every G2P reduces them over the state it writes.  k_dt_control runs
+// first in each step: it turns the maxima into the step's dt, advances the
+// clock and clears the maxima for this step's G2P.
+template <int MAT>
+__global__ void __launch_bounds__(256) k_dt_scan(MatP mat, PartView pv, uint32_t *counters)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    float sd[2] = {0.0f, 0.0f};
+    if (i < pv.n) {
+        const float v[3] = {ld(pv, PL_V, i), ld(pv, PL_V + 1, i), ld(pv, PL_V + 2, i)};
+        float J;
+        if (MAT == MPM_JELLO) {
+            float F[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] = ld(pv, PL_F + k, i);
+            J = det3<float>(F);
+        } else {
+            J = ld(pv, PL_S, i);
+        }
+        dt_measures<MAT>(mat, v, J, sd[0], sd[1]);
+    }
+    wave_max_to(counters + CT_SPEED, sd[0]);
+    wave_max_to(counters + CT_DEFORM, sd[1]);
+}
+
+__global__ void __launch_bounds__(64) k_dt_control(DtCtl *c, uint32_t *counters, float max_dt, float cfl_h, int mode)
+{
+    if (threadIdx.x != 0) return;
+    const float speed = __uint_as_float(gp(counters)[CT_SPEED]);
+    const float deform = __uint_as_float(gp(counters)[CT_DEFORM]);
+    float dt = max_dt;
+    if (speed > 0.0f) dt = fminf(dt, cfl_h / speed);
+    if (mode == MPM_DT_CFL_DEFORM) {
+        const float f = fminf(10.0f, fmaxf(1.0f, deform * deform * deform));
+        dt = fminf(dt, max_dt / f);
+    }
+    gp(c)->dt = dt;
+    gp(c)->last = (double)dt;
+    gp(c)->time = gp(c)->time + (double)dt;
+    gp(counters)[CT_SPEED] = 0u;
+    gp(counters)[CT_DEFORM] = 0u;
+}
+
+int launch_dt_scan(hipStream_t st, const MatP &m, const PartView &pv, uint32_t *counters)
+{
+    if (pv.n <= 0) return 0;
+    dim3 gr(nblk(pv.n, 256)), bl(256);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_dt_scan<MPM_JELLO>, gr, bl, 0, st, m, pv, counters); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_dt_scan<MPM_SNOW>, gr, bl, 0, st, m, pv, counters); break;
+    default: hipLaunchKernelGGL(k_dt_scan<MPM_WATER>, gr, bl, 0, st, m, pv, counters); break;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_dt_control(hipStream_t st, DtCtl *c, uint32_t *counters, float max_dt, float cfl_h, int mode)
+{
+    hipLaunchKernelGGL(k_dt_control, dim3(1), dim3(64), 0, st, c, counters, max_dt, cfl_h, mode);
     return (int)hipGetLastError();
 }
 

@@ -275,6 +275,54 @@ class Sim:
         return self.p.reshape(self.nz, self.ny, self.nx, 3)
 
 
+DT_FIXED, DT_CFL, DT_CFL_DEFORM = 0, 1, 2
+
+
+def controller_measures(g: Group):
+    """Per-particle inputs of the step-size controller for one group's FP64
+    state: |v| + c (c the elastic wave speed at the particle's deformation:
+    P waves sqrt((lam + 2 mu) / rho), hardened by exp(xi (1 - det F_p)) for
+    snow; water sqrt(k gamma J^(1 - gamma) / rho)) and the deformation
+    measure max(J, 1/J) of the reference's unused time-step factor
+    (MPMCUDA.cu:613-619 jello det F, :700-708 snow det F_p; never set for
+    water, :710-713, so 0 here)."""
+    m = g.mat
+    rho = m.mass / m.vol
+    if m.type == WATER:
+        J = g.J
+        c2 = m.eos_k * m.eos_gamma / rho * np.power(np.where(J > 0, J, np.nan), 1.0 - m.eos_gamma)
+        d = np.zeros(g.n)
+    else:
+        J = np.linalg.det(g.Fp if m.type == SNOW else g.Fe)
+        c2 = (m.lam + 2.0 * m.mu) / rho * np.ones(g.n)
+        if m.type == SNOW:
+            c2 = c2 * np.exp(m.xi * (1.0 - J))
+        d = np.where(J > 0, np.maximum(J, 1.0 / np.where(J > 0, J, 1.0)), 1e30)
+    c = np.sqrt(np.maximum(np.nan_to_num(c2, nan=0.0), 0.0))
+    return np.linalg.norm(g.v, axis=1) + c, d
+
+
+def controller_dt(groups, h: float, max_dt: float, cfl: float, mode: int) -> float:
+    """dt the controller gives the next step: min(max_dt, cfl h / max(|v|+c))
+    and, with DT_CFL_DEFORM, at most max_dt / clamp(max(J, 1/J)^3, 1, 10)
+    (the reference's factor min(10, max^3), MPMCUDA.cu:915-920, which it
+    computes but never applies)."""
+    if mode == DT_FIXED:
+        return float(max_dt)
+    speed, deform = 0.0, 0.0
+    for g in groups:
+        s, d = controller_measures(g)
+        if g.n:
+            speed = max(speed, float(s.max()))
+            deform = max(deform, float(d.max()))
+    dt = float(max_dt)
+    if speed > 0:
+        dt = min(dt, cfl * h / speed)
+    if mode == DT_CFL_DEFORM:
+        dt = min(dt, max_dt / min(10.0, max(1.0, deform ** 3)))
+    return dt
+
+
 class Sim2D:
     """FP64 2D snow of MPMSnow.py:77-150 on an n x n grid (index j * n + i).
     Particles in AoS FP64 (x, v: [n][2]; C, Fe, Fp: [n][2][2]); the script's
