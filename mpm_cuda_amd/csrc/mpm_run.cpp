@@ -6,6 +6,8 @@
 //
 //   mpm_run --scene snow|jello|water|obj [--obj path] [--time 1.0] [--fps 36]
 //           [--out Animations] [--form fast|plain] [--grid N]
+//           [--dt-mode fixed|cfl|cfl_deform] [--cfl 0.5]
+// With a dt controller the scene's dt is the largest step (max_dt).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -34,12 +36,16 @@ int main(int argc, char **argv)
     std::string out = arg(argc, argv, "--out", "Animations");
     std::string form = arg(argc, argv, "--form", "fast");
     int n = atoi(arg(argc, argv, "--grid", "128"));
+    std::string dt_mode = arg(argc, argv, "--dt-mode", "fixed");
     if (fps <= 0 || n < 16) { fprintf(stderr, "bad --fps or --grid\n"); return 2; }
+    if (dt_mode != "fixed" && dt_mode != "cfl" && dt_mode != "cfl_deform") { fprintf(stderr, "bad --dt-mode\n"); return 2; }
     mkdir(out.c_str(), 0755);
 
     mpm_config cfg;
     mpm_config_default(&cfg);
     cfg.form = form == "plain" ? MPM_FORM_PLAIN : MPM_FORM_FAST;
+    cfg.dt_mode = dt_mode == "cfl" ? MPM_DT_CFL : (dt_mode == "cfl_deform" ? MPM_DT_CFL_DEFORM : MPM_DT_FIXED);
+    cfg.cfl = (float)atof(arg(argc, argv, "--cfl", "0.5"));
     double h = 1.0 / n, dt = 1e-4;
     unsigned ppc = 8;
     std::vector<ObjInitInfo> infos;
