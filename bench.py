@@ -420,6 +420,11 @@ def main():
         "fp64_svd_fallbacks": res["stats"]["fp64_fallbacks"],
         "sorts_in_timed_steps": int(res["stats"]["sorts_timed"]),
     }
+    # the achievable bandwidth beside the published peak
+    from mpm_cuda_amd import capi
+    cp = round(capi.probe_copy_bandwidth(1 << 30, 10), 1)
+    out["roofline"]["copy_GBs"] = cp
+    out["roofline"]["frac_of_copy"] = round(out["roofline"]["achieved"] / cp, 4)
     if not a.no_variants:
         ksteps = max(10, a.steps // 4)
         out["variants"] = {}

@@ -295,6 +295,11 @@ int mpm_sample_object(const mpm_config *cfg, const mpm_object *obj, float *xyz, 
  * n points in the OBJ file's own coordinates; inside[i] = 1 or 0. */
 int mpm_probe_point_in_mesh(const char *obj_path, const double *pts, int64_t n, uint8_t *inside);
 
+/* Achievable HBM bandwidth: a streaming device copy of `bytes` (multiple of
+ * 16, 1 MiB to 8 GiB), read + write counted, median over `reps` timed
+ * launches after one warm-up, in GB/s. */
+int mpm_probe_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs);
+
 /* Device FP32 base-node index max(0, floor((x - origin) * inv_h - 0.5)) of
  * n coordinates, as used by P2G / G2P / the sort keys. */
 int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, int32_t *out);

@@ -114,6 +114,7 @@ _SIGS = {
     "mpm_probe_base_index": (C.c_int, [C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
     "mpm_probe_svd": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_sort": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
+    "mpm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "mpm_batch_shape_default": (None, [C.POINTER(MpmBatchShape)]),
     "mpm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(MpmBatchShape),
                                    C.POINTER(C.c_void_p)]),
@@ -575,6 +576,13 @@ def mlp_train(F, params, steps: int, batch: int = 4096, lr: float = 1e-3, seed: 
     check(lib().mpm_mlp_train(F.ctypes.data_as(C.c_void_p), F.shape[0], int(steps), int(batch), float(lr), int(seed),
                               p.ctypes.data_as(C.c_void_p), hist.ctypes.data_as(C.c_void_p)))
     return p, hist[:steps]
+
+
+def probe_copy_bandwidth(bytes_: int = 1 << 30, reps: int = 10) -> float:
+    """GB/s of a streaming device copy (read + write), median of reps."""
+    g = C.c_double()
+    check(lib().mpm_probe_copy_bandwidth(int(bytes_), int(reps), C.byref(g)))
+    return g.value
 
 
 def probe_sort(keys, bits: int):

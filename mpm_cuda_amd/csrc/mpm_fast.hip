@@ -305,18 +305,19 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
 //
 // A tile is one node block's particles (grouped by the last sort).  It is
 // processed in chunks of at most CHUNK particles:
-//   A. one lane per particle (dense and coalesced): load the particle (FAST)
-//      or run its G2P update (FUSED), find its base node and write a record
-//      (below) to LDS.  The base, relative to the block's 8^3-node arena
-//      (the block's 4 nodes + 1 below + 3 above per axis), is one of 6^3
-//      bins.
-//   B. wave 0 sorts the chunk by bin (stable counting sort in LDS) and cuts
-//      every bin into items of at most ITEM_CAP particles.
+//   A. one lane per particle (dense and coalesced): FAST loads the position,
+//      FUSED runs the particle's G2P update and writes its record (below)
+//      to LDS; both find the base node.  The base, relative to the block's
+//      8^3-node arena (the block's 4 nodes + 1 below + 3 above per axis),
+//      is one of 6^3 bins.
+//   B. both waves sort the chunk by bin (stable counting sort in LDS) and
+//      cut every bin into items of at most ITEM_CAP particles.
 //   C. one lane per item: the item's particles share their base node, so the
-//      lane sums their 27 node contributions in registers and adds them to
-//      its wave's LDS arena once, in a fixed node order.  In each of the 27
-//      read-modify-writes the lanes of different bins touch different slots;
-//      lanes of one bin (a bin cut into several items) take turns.
+//      lane sums their 27 node contributions in registers (FAST reloads each
+//      particle and builds its record in registers; FUSED reads the LDS
+//      record) and adds them to its wave's LDS arena once, in a fixed node
+//      order.  In each of the 27 read-modify-writes the lanes of different
+//      bins touch different slots; lanes of one bin take turns.
 // The waves' arenas are summed into the tile's staging slot, which the grid
 // update gathers.  Particles whose stencil leaves the arena (or whose tile
 // has no staging slot) are listed for the atomic fallback kernel.
@@ -376,6 +377,19 @@ struct TileLds {
     uint32_t nitems;
 };
 
+// LDS of a FAST-form tile: bin keys, the sort and the arenas (20.7 KB; the
+// particle records stay in HBM / cache).
+struct TileKeys {
+    float4 arena[TILE_WAVES][A_SLOTS];
+    uint32_t items[CHUNK];
+    uint16_t perm[CHUNK];
+    uint16_t rnk[CHUNK];
+    uint16_t cnt[NBIN];
+    uint16_t cnt1[NBIN];
+    uint8_t key[CHUNK];
+    uint32_t nitems;
+};
+
 struct TileGeo {
     int ax, ay, az;     // arena origin (node index of arena slot 0)
 };
@@ -388,35 +402,11 @@ __device__ __forceinline__ TileGeo tile_geo(const GridP &g, uint32_t blk)
     return TileGeo{bx * BLK - 1, by * BLK - 1, bz * BLK - 1};
 }
 
-// Loaded particle inputs parked in the particle's own record slot
-// (rec[k][slot], k < 21) between the loads and their use.
-template <int MAT>
-__device__ __forceinline__ void park_raw(TileLds &L, int slot, const RawP &r)
-{
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { L.rec[a][slot] = r.x[a]; L.rec[3 + a][slot] = r.v[a]; }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) L.rec[6 + k][slot] = r.C[k];
-#pragma unroll
-    for (int k = 0; k < (MAT == MPM_WATER ? 1 : 6); ++k) L.rec[15 + k][slot] = r.s[k];
-}
-template <int MAT>
-__device__ __forceinline__ void unpark_raw(const TileLds &L, int slot, RawP &r)
-{
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { r.x[a] = L.rec[a][slot]; r.v[a] = L.rec[3 + a][slot]; }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) r.C[k] = L.rec[6 + k][slot];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) r.s[k] = (MAT == MPM_WATER && k > 0) ? 0.0f : L.rec[15 + k][slot];
-}
-
-// Phase A tail: bin the particle and write its record, or list it for the
-// atomic fallback.  Returns true when the particle was listed.
-template <int MAT>
-__device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const TileGeo &t, bool active, int slot,
-                                             int i, const Stencil &s, const Scatter &sc, bool staged,
-                                             uint32_t *spill, uint32_t *counters)
+// Phase A tail: the particle's bin key, or NO_BIN and an entry in the list
+// of the atomic fallback.  Returns true when the particle was listed.
+template <class TL>
+__device__ __forceinline__ bool bin_key(const GridP &g, TL &L, const TileGeo &t, bool active, int slot, int i,
+                                        const Stencil &s, bool staged, uint32_t *spill, uint32_t *counters)
 {
     const int lane = threadIdx.x & 63;
     const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
@@ -433,23 +423,43 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
         if (spill_me) gp(spill)[base + __popcll(below)] = (uint32_t)i;
     }
     if (active) L.key[slot] = inside ? (uint8_t)(lx + 6 * (ly + 6 * lz)) : (uint8_t)NO_BIN;
-    if (inside) {
-        const float h = 1.0f / g.inv_h;
+    return spill_me;
+}
+
+// The particle's P2G record from its stencil and scatter quantities:
+// fx, q0 = m v - h m C fx, hB = h m C, G (xx, yy, zz, xy, xz, yz).
+template <int MAT>
+__device__ __forceinline__ void make_record(const GridP &g, const Stencil &s, const Scatter &sc, float q0[3],
+                                            float hB[9], float Gs[6])
+{
+    const float h = 1.0f / g.inv_h;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) L.rec[a][slot] = s.fx[a];
+    for (int c = 0; c < 3; ++c) {
+        const float bf = sc.B[3 * c] * s.fx[0] + sc.B[3 * c + 1] * s.fx[1] + sc.B[3 * c + 2] * s.fx[2];
+        q0[c] = sc.a[c] - h * bf;
+    }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float bf = sc.B[3 * c] * s.fx[0] + sc.B[3 * c + 1] * s.fx[1] + sc.B[3 * c + 2] * s.fx[2];
-            L.rec[3 + c][slot] = sc.a[c] - h * bf;
-        }
+    for (int k = 0; k < 9; ++k) hB[k] = h * sc.B[k];
+    Gs[0] = sc.G[0]; Gs[1] = sc.G[4]; Gs[2] = sc.G[8]; Gs[3] = sc.G[1]; Gs[4] = sc.G[2]; Gs[5] = sc.G[5];
+}
+
+// FUSED form phase A tail: bin the particle and write its record to LDS, or
+// list it for the atomic fallback.  Returns true when it was listed.
+template <int MAT>
+__device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const TileGeo &t, bool active, int slot,
+                                             int i, const Stencil &s, const Scatter &sc, bool staged,
+                                             uint32_t *spill, uint32_t *counters)
+{
+    const bool spill_me = bin_key(g, L, t, active, slot, i, s, staged, spill, counters);
+    if (active && !spill_me) {
+        float q0[3], hB[9], Gs[6];
+        make_record<MAT>(g, s, sc, q0, hB, Gs);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) L.rec[6 + k][slot] = h * sc.B[k];
-        if (MAT == MPM_WATER) {
-            L.rec[15][slot] = sc.G[0];
-        } else {
-            L.rec[15][slot] = sc.G[0]; L.rec[16][slot] = sc.G[4]; L.rec[17][slot] = sc.G[8];
-            L.rec[18][slot] = sc.G[1]; L.rec[19][slot] = sc.G[2]; L.rec[20][slot] = sc.G[5];
-        }
+        for (int a = 0; a < 3; ++a) { L.rec[a][slot] = s.fx[a]; L.rec[3 + a][slot] = q0[a]; }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) L.rec[6 + k][slot] = hB[k];
+#pragma unroll
+        for (int k = 0; k < (MAT == MPM_WATER ? 1 : 6); ++k) L.rec[15 + k][slot] = Gs[k];
     }
     return spill_me;
 }
@@ -460,7 +470,8 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
 // waves place their particles.  Within a bin, wave 0's particles precede
 // wave 1's, each in slot order, so the order (and every sum) is the same in
 // every run.  Items pack bin | first sorted position << 8 | count << 20.
-__device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
+template <class TL>
+__device__ __forceinline__ void sort_chunk(TL &L, int nc)
 {
     static_assert(TILE_WAVES == 2, "two counting waves");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -537,8 +548,10 @@ __device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
     }
 }
 
-// Phase C, one particle: add its 27 node contributions to acc (node
-// ox + 3 oy + 9 oz).  The sum is factored per axis:
+// Phase C, one particle given its record in registers (fx, q0, hB and G as
+// xx, yy, zz, xy, xz, yz; water: the diagonal value in Gs[0]): add its 27
+// node contributions to acc (node ox + 3 oy + 9 oz).  The sum is factored
+// per axis:
 //   p_c(o) = Nz (alpha_c + oz beta_c) + Dz gamma_c
 // with alpha, beta, gamma depending on (ox, oy) only, N the B-spline weights
 // and D = dN/dx.  Offsets enter through kadd/kmul (no multiply by a
@@ -546,25 +559,19 @@ __device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
 __device__ __forceinline__ float kadd(int k, float a, float b) { return k == 0 ? a : (k == 1 ? a + b : a + (b + b)); }
 
 template <int MAT>
-__device__ __forceinline__ void accum_particle(const TileLds &L, int slot, float inv_h, float m, float4 acc[27])
+__device__ __forceinline__ void accum_regs(const float fx[3], const float q0[3], const float hB[9], const float Gs[6],
+                                           float inv_h, float m, float4 acc[27])
 {
     constexpr bool W = MAT == MPM_WATER;
-    float fx[3], q0[3], hB[9], G[9];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) fx[a] = L.rec[a][slot];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) q0[c] = L.rec[3 + c][slot];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) hB[k] = L.rec[6 + k][slot];
+    float G[9];
     if (W) {
-        const float gw = L.rec[15][slot];
-        G[0] = G[4] = G[8] = gw;
+        G[0] = G[4] = G[8] = Gs[0];
         G[1] = G[2] = G[3] = G[5] = G[6] = G[7] = 0.0f;     // never read
     } else {
-        G[0] = L.rec[15][slot]; G[4] = L.rec[16][slot]; G[8] = L.rec[17][slot];
-        G[1] = G[3] = L.rec[18][slot];
-        G[2] = G[6] = L.rec[19][slot];
-        G[5] = G[7] = L.rec[20][slot];
+        G[0] = Gs[0]; G[4] = Gs[1]; G[8] = Gs[2];
+        G[1] = G[3] = Gs[3];
+        G[2] = G[6] = Gs[4];
+        G[5] = G[7] = Gs[5];
     }
     // G_ck is structurally zero for water off the diagonal
 #define GZ(c, k) (W && (c) != (k))
@@ -623,23 +630,29 @@ __device__ __forceinline__ void accum_particle(const TileLds &L, int slot, float
 #undef GZ
 }
 
-// Phase C for one wave: items [batch * 64, ...) of the chunk.
+// The same from the particle's LDS record (FUSED form).
 template <int MAT>
-__device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int batch, float inv_h, float m)
+__device__ __forceinline__ void accum_particle(const TileLds &L, int slot, float inv_h, float m, float4 acc[27])
+{
+    float fx[3], q0[3], hB[9], Gs[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fx[a] = L.rec[a][slot];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q0[c] = L.rec[3 + c][slot];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hB[k] = L.rec[6 + k][slot];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Gs[k] = (MAT == MPM_WATER && k > 0) ? 0.0f : L.rec[15 + k][slot];
+    accum_regs<MAT>(fx, q0, hB, Gs, inv_h, m, acc);
+}
+
+// Phase C flush: the lane's 27 accumulated nodes are added to its wave's
+// arena in a fixed order (compiler barrier; LDS executes one wave's
+// operations in order).  Lanes of different bins touch different slots in
+// each instruction; lanes of the same bin take turns.
+__device__ __forceinline__ void flush_item(float4 *arena, bool has, uint32_t bin, const float4 acc[27])
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t it = (uint32_t)(batch * 64 + lane);
-    const bool has = it < L.nitems;
-    const uint32_t item = has ? L.items[it] : 0u;
-    const uint32_t bin = has ? (item & 255u) : NO_BIN;
-    const uint32_t first = (item >> 8) & 4095u, cnt = has ? (item >> 20) : 0u;
-    float4 acc[27];
-#pragma unroll
-    for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t rounds = wave_max_u32(cnt);
-    for (uint32_t r = 0; r < rounds; ++r)
-        if (r < cnt) accum_particle<MAT>(L, L.perm[first + r], inv_h, m, acc);
-    // ordered read-modify-write of the 27 nodes; same-bin lanes take turns
     const int bx = (int)(bin % 6u), by = (int)((bin / 6u) % 6u), bz = (int)(bin / 36u);
     uint64_t pending = __ballot(has);
     const uint64_t same = match_key8(bin);
@@ -665,6 +678,58 @@ __device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int bat
     }
 }
 
+// Phase C for one wave, FUSED form: items [batch * 64, ...) of the chunk,
+// records from LDS.
+template <int MAT>
+__device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int batch, float inv_h, float m)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t it = (uint32_t)(batch * 64 + lane);
+    const bool has = it < L.nitems;
+    const uint32_t item = has ? L.items[it] : 0u;
+    const uint32_t bin = has ? (item & 255u) : NO_BIN;
+    const uint32_t first = (item >> 8) & 4095u, cnt = has ? (item >> 20) : 0u;
+    float4 acc[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t rounds = wave_max_u32(cnt);
+    for (uint32_t r = 0; r < rounds; ++r)
+        if (r < cnt) accum_particle<MAT>(L, L.perm[first + r], inv_h, m, acc);
+    flush_item(arena, has, bin, acc);
+}
+
+// Phase C for one wave, FAST form: the item's particles are loaded again
+// from the particle planes (the tile's lines are in cache from phase A or
+// arrive now; no LDS records) and their records built in registers.
+template <int MAT>
+__device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena, int batch, const GridP &g,
+                                                     const MatP &mat, const PartView &pv, int i0, float dt)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t it = (uint32_t)(batch * 64 + lane);
+    const bool has = it < L.nitems;
+    const uint32_t item = has ? L.items[it] : 0u;
+    const uint32_t bin = has ? (item & 255u) : NO_BIN;
+    const uint32_t first = (item >> 8) & 4095u, cnt = has ? (item >> 20) : 0u;
+    float4 acc[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t rounds = wave_max_u32(cnt);
+    for (uint32_t r = 0; r < rounds; ++r)
+        if (r < cnt) {
+            RawP raw;
+            load_raw<MAT>(pv, i0 + (int)L.perm[first + r], raw);
+            Scatter sc;
+            scatter_from_raw<MAT>(raw, mat, dt, sc);
+            Stencil st;
+            make_stencil(g, raw.x, st);
+            float q0[3], hB[9], Gs[6];
+            make_record<MAT>(g, st, sc, q0, hB, Gs);
+            accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
+        }
+    flush_item(arena, has, bin, acc);
+}
+
 // Phases B and C of one chunk (all threads; ends synchronized).
 template <int MAT>
 __device__ __forceinline__ void scatter_chunk(TileLds &L, int nc, float inv_h, float m)
@@ -678,7 +743,8 @@ __device__ __forceinline__ void scatter_chunk(TileLds &L, int nc, float inv_h, f
     __syncthreads();
 }
 
-__device__ __forceinline__ void tile_clear(TileLds &L)
+template <class TL>
+__device__ __forceinline__ void tile_clear(TL &L)
 {
     for (int k = threadIdx.x; k < TILE_WAVES * A_SLOTS; k += TILE_THREADS)
         L.arena[k / A_SLOTS][k % A_SLOTS] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -686,7 +752,8 @@ __device__ __forceinline__ void tile_clear(TileLds &L)
 
 // Sum the waves' arenas into staging slot `slot` (logical node order) and
 // mark the node blocks that received mass.
-__device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const TileLds &L, float4 *staging,
+template <class TL>
+__device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const TL &L, float4 *staging,
                                          uint32_t slot, bool staged, uint32_t *flags)
 {
     __syncthreads();
@@ -712,55 +779,51 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 
 // One tile (descriptor d = first particle, count, block, staging slot) of
 // the FAST-form P2G; returns health bits.  All threads of the workgroup.
+// Phase A loads only positions (bin keys); phase C reloads x, v, C and the
+// stored stress of each item's particles.  Without LDS records a workgroup
+// needs 20.7 KB instead of 53 KB, so the register file, not LDS, bounds the
+// workgroups per CU (four instead of three: 90 vs 106 us per 1M-particle
+// snow step, measured).
 template <int MAT>
 __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, const PartView &pv, const uint4 d,
                                              uint32_t staging_cap, float dt, float4 *__restrict__ staging,
-                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, TileLds &L)
+                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, TileKeys &L)
 {
     uint32_t bits = 0;
     const TileGeo t = tile_geo(g, d.z);
     const bool staged = d.w < staging_cap;
     tile_clear(L);
     const int wave0 = (int)(threadIdx.x & ~63u);
+    const int wave = threadIdx.x >> 6;
     for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
         const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
-        // A1: the loads of all passes (up to 3 x 21, within the wave's 63
-        // outstanding vector loads) are issued before any is used and parked
-        // in the lane's own record slots: one memory round trip per chunk
-        // instead of one per pass
-        {
-            // unconditional (one basic block): lanes past the chunk reload
-            // its last particle and park it in their own unused slot
-            RawP raw[A_PASSES];
+        // A: all position loads of the chunk first (lanes past its end reload
+        // its last particle), then bin keys
+        float xs[A_PASSES][3];
 #pragma unroll
-            for (int it = 0; it < A_PASSES; ++it) {
-                const int slot = min(it * TILE_THREADS + (int)threadIdx.x, nc - 1);
-                load_raw<MAT>(pv, (int)(d.x + c0) + slot, raw[it]);
-            }
-            __builtin_amdgcn_sched_group_barrier(0x0020, A_PASSES * raw_loads(MAT), 0);
+        for (int it = 0; it < A_PASSES; ++it) {
+            const uint32_t slot = (uint32_t)min(it * TILE_THREADS + (int)threadIdx.x, nc - 1);
 #pragma unroll
-            for (int it = 0; it < A_PASSES; ++it) park_raw<MAT>(L, it * TILE_THREADS + (int)threadIdx.x, raw[it]);
+            for (int a = 0; a < 3; ++a) xs[it][a] = ld_plane(pv, PL_X + a, d.x + c0 + slot);
         }
-        // A2: each lane reads back its own slots (no barrier needed), builds
-        // the stencil and record, and bins the particle
 #pragma unroll
         for (int it = 0; it < A_PASSES; ++it) {
             if (it * TILE_THREADS + wave0 >= nc) break;              // wave-uniform
             const int slot = it * TILE_THREADS + (int)threadIdx.x;
             const bool active = slot < nc;
-            const int i = (int)(d.x + c0) + slot;
             Stencil s;
             s.b[0] = s.b[1] = s.b[2] = 0;
-            Scatter sc;
-            if (active) {
-                RawP r;
-                unpark_raw<MAT>(L, slot, r);
-                scatter_from_raw<MAT>(r, mat, dt, sc);
-                bits |= make_stencil(g, r.x, s);
-            }
-            bin_particle<MAT>(g, L, t, active, slot, i, s, sc, staged, spill, counters);
+            if (active) bits |= make_stencil(g, xs[it], s);
+            bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, s, staged, spill, counters);
         }
-        scatter_chunk<MAT>(L, nc, g.inv_h, mat.mass);
+        // B and C
+        __syncthreads();
+        sort_chunk(L, nc);
+        __syncthreads();
+        const int nb = (int)((L.nitems + 63u) >> 6);
+        for (int b = wave; b < nb; b += TILE_WAVES)
+            scatter_items_reload<MAT>(L, L.arena[wave], b, g, mat, pv, (int)(d.x + c0), dt);
+        __syncthreads();
     }
     tile_end(g, t, L, staging, d.w, staged, flags);
     return bits;
@@ -774,7 +837,7 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
                                                             float4 *__restrict__ staging, uint32_t *spill,
                                                             uint32_t *flags, uint32_t *counters)
 {
-    __shared__ TileLds L;
+    __shared__ TileKeys L;
     const uint32_t nt = min(*tile_count, desc_cap);
     uint32_t bits = 0;
     const uint32_t nt8 = round_xcd(nt);
@@ -794,14 +857,14 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
 // the tile time at this register cap -- and per chunk of P2G_CHUNK scenes).
 // Work index = the tiles of the first such scene, then of the next, ...;
 // workgroups walk it with the XCD-aware stride of the single-scene kernel.
-// The scene map (end of each scene's tile range) sits in LDS: 256 B more
-// than the tile's 52,976 B still fits three workgroups per CU (LDS is
-// allocated in granules; a 1 KB map leaves two, measured 40% slower).
+// The scene map (end of each scene's tile range) sits in LDS beside the
+// tile's (256 B; LDS is allocated in granules, so any addition is checked
+// against the workgroups per CU it leaves).
 constexpr int P2G_CHUNK = 64;
 template <int MAT>
 __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
-    __shared__ TileLds L;
+    __shared__ TileKeys L;
     __shared__ uint32_t s_end[P2G_CHUNK];
     __shared__ uint32_t s_total;
     if (threadIdx.x < 64) {

@@ -151,8 +151,7 @@ __device__ __forceinline__ void load_scatter(const PartView &pv, int i, const Ma
 }
 
 // The FAST-form P2G inputs of one particle as loaded (x, v, C and the stored
-// stress; water: J in s[0]), so that a block kernel can issue the loads of
-// several particles before it uses any.
+// stress; water: J in s[0]).
 struct RawP {
     float x[3], v[3], C[9], s[6];
 };
@@ -179,7 +178,6 @@ __device__ __forceinline__ void load_raw(const PartView &pv, int i, RawP &r)
         for (int k = 0; k < 6; ++k) r.s[k] = ld_plane(pv, PL_TAU + k, u);
     }
 }
-constexpr int raw_loads(int mat) { return mat == MPM_WATER ? 16 : 21; }
 
 // load_scatter's arithmetic on loaded inputs (stress stored).
 template <int MAT>

@@ -59,6 +59,13 @@ __global__ void k_probe_svd_residual(const float *F, const float *q, int n, int 
     off[i] = r;
 }
 
+// Plain streaming copy (16 B per lane and iteration, grid-stride).
+__global__ void __launch_bounds__(256) k_probe_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, int64_t n4)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256)
+        gstore4(dst, i, gload4(src, i));
+}
+
 // RAII set of device temporaries for the probes.
 struct DevBufs {
     std::vector<void *> p;
@@ -86,6 +93,42 @@ int mpm_sample_object(const mpm_config *cfg, const mpm_object *obj, float *xyz, 
     int64_t cnt = (int64_t)pts.size() / 3;
     if (xyz) memcpy(xyz, pts.data(), (size_t)std::min(*n, cnt) * 3 * sizeof(float));
     *n = cnt;
+    return MPM_OK;
+}
+
+int mpm_probe_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs)
+{
+    if (!gbs) return set_error(MPM_ERR_INVALID, "null argument");
+    if (bytes < (1 << 20) || bytes > (8ll << 30) || bytes % 16) return set_error(MPM_ERR_INVALID, "bytes must be a multiple of 16 in [1 MiB, 8 GiB]");
+    if (reps < 1 || reps > 100) return set_error(MPM_ERR_INVALID, "reps out of range [1,100]");
+    DevBufs b;
+    float4 *src = nullptr, *dst = nullptr;
+    MPM_HIP(b.alloc(&src, (size_t)bytes));
+    MPM_HIP(b.alloc(&dst, (size_t)bytes));
+    MPM_HIP(hipMemset(src, 0, (size_t)bytes));
+    const int64_t n4 = bytes / 16;
+    const unsigned wg = (unsigned)std::min<int64_t>((n4 + 255) / 256, 16384);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MPM_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    std::vector<float> ms;
+    for (int r = 0; r <= reps && err == hipSuccess; ++r) {            // the first launch is a warm-up
+        hipLaunchKernelGGL(k_probe_copy, dim3(wg), dim3(256), 0, 0, src, dst, n4);
+        if ((err = hipGetLastError()) != hipSuccess) break;
+        if ((err = hipEventRecord(e0, 0)) != hipSuccess) break;
+        hipLaunchKernelGGL(k_probe_copy, dim3(wg), dim3(256), 0, 0, src, dst, n4);
+        if ((err = hipEventRecord(e1, 0)) != hipSuccess) break;
+        if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+        float t = 0.f;
+        if ((err = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) break;
+        if (r > 0) ms.push_back(t);
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    MPM_HIP(err);
+    std::sort(ms.begin(), ms.end());
+    const double med = ms[ms.size() / 2];
+    *gbs = 2.0 * (double)bytes / (med * 1e-3) / 1e9;
     return MPM_OK;
 }
 

@@ -282,3 +282,15 @@ def test_resort_forced_by_spills(gpu):
     finally:
         a.close()
         b.close()
+
+
+def test_copy_bandwidth_probe(gpu):
+    """The streaming-copy probe the benchmark reports beside the HBM peak:
+    a plausible figure (below the 8 TB/s peak) and its argument checks."""
+    g = capi.probe_copy_bandwidth(256 << 20, 5)
+    print(f"copy {g:.1f} GB/s")
+    assert 500.0 < g < 8000.0
+    with pytest.raises(capi.MpmError):
+        capi.probe_copy_bandwidth(1000, 5)
+    with pytest.raises(capi.MpmError):
+        capi.probe_copy_bandwidth(1 << 20, 0)
