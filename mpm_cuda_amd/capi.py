@@ -333,7 +333,7 @@ class Sim:
 
 
 BATCH_STAGES = ("p2g", "grid", "g2p")
-BATCH_CANDIDATES = {"p2g": (768, 1536, 3072, 8192), "grid": (1024, 2048, 4096, 8192), "g2p": (64, 128, 192, 256)}
+BATCH_CANDIDATES = {"p2g": (1024, 2048, 4096, 8192), "grid": (1024, 2048, 4096, 8192), "g2p": (64, 128, 192, 256)}
 
 
 def batch_shape(**kw) -> MpmBatchShape:

@@ -28,7 +28,7 @@ namespace {
 
 constexpr int SPILL_WGS = 128;
 constexpr int NCAND = 4;
-const int P2G_CAND[NCAND] = {768, 1536, 3072, 8192};
+const int P2G_CAND[NCAND] = {1024, 2048, 4096, 8192};   // one-wave workgroups
 const int GRID_CAND[NCAND] = {1024, 2048, 4096, 8192};
 const int G2P_CAND[NCAND] = {64, 128, 192, 256};
 
@@ -408,7 +408,7 @@ void mpm_batch_shape_default(mpm_batch_shape *s)
 {
     if (!s) return;
     memset(s, 0, sizeof *s);
-    s->p2g_workgroups = 3072;
+    s->p2g_workgroups = 4096;
     s->grid_workgroups = 4096;
     s->g2p_threads = 256;
     s->sort_every = 24;

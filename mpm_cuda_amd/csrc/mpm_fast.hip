@@ -324,12 +324,14 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
 // (On gfx950 an LDS float atomic runs ~20x slower than a plain LDS
 // read-modify-write; summing in registers first also divides the LDS traffic
 // by the particles per item.)
-constexpr int TILE_WAVES = 2;
+constexpr int TILE_WAVES = 2;                    // FUSED form: waves per tile
 constexpr int TILE_THREADS = 64 * TILE_WAVES;
+constexpr int P2G_THREADS = 64;                  // FAST form: one wave per tile
 static_assert(ARENA_NODES % TILE_THREADS == 0, "arena nodes per thread must be whole");
 constexpr int CHUNK = 384;          // particles per pass over a tile (a full 4^3-cell block holds ~320 at 5 per cell)
 constexpr int A_PASSES = CHUNK / TILE_THREADS;   // phase-A passes of a wave over a chunk
 static_assert(CHUNK % TILE_THREADS == 0, "whole phase-A passes");
+constexpr int P2G_PASSES = CHUNK / P2G_THREADS;  // FAST phase-A passes over a chunk
 constexpr int ITEM_CAP = 4;         // particles per lane item
 constexpr int NBIN = 216;           // 6^3 base positions inside an arena
 constexpr uint32_t NO_BIN = 255u;
@@ -377,15 +379,15 @@ struct TileLds {
     uint32_t nitems;
 };
 
-// LDS of a FAST-form tile: bin keys, the sort and the arenas (20.7 KB; the
-// particle records stay in HBM / cache).
+// LDS of a FAST-form tile (one wave): bin keys, the sort and the arena
+// (12.5 KB; the particle records stay in HBM / cache).
 struct TileKeys {
-    float4 arena[TILE_WAVES][A_SLOTS];
+    float4 arena[A_SLOTS];
     uint32_t items[CHUNK];
     uint16_t perm[CHUNK];
     uint16_t rnk[CHUNK];
     uint16_t cnt[NBIN];
-    uint16_t cnt1[NBIN];
+    uint16_t start[NBIN];       // each bin's first sorted position
     uint8_t key[CHUNK];
     uint32_t nitems;
 };
@@ -464,14 +466,61 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
     return spill_me;
 }
 
-// Phase B (both waves): counting sort of the chunk's nc particles by bin and
-// the item list.  Wave w counts the 64-particle batches w, w + 2, ... into
-// its own counts; wave 0 then builds the items from the summed counts; both
-// waves place their particles.  Within a bin, wave 0's particles precede
-// wave 1's, each in slot order, so the order (and every sum) is the same in
-// every run.  Items pack bin | first sorted position << 8 | count << 20.
-template <class TL>
-__device__ __forceinline__ void sort_chunk(TL &L, int nc)
+// Phase B items of a chunk from its per-bin counts (one wave; lane l owns
+// bins 4l .. 4l+3, counts c[4]).  A bin of c particles gives c / ITEM_CAP
+// full items and one item of the remainder.  Full items come first, then
+// remainders by decreasing size, so a batch of 64 items mostly holds items
+// of equal length (fewer idle lanes per round) and the items of one bin
+// mostly fall into different batches (fewer turns in the flush).  Each
+// bin's first sorted position goes to start[].  Items pack
+// bin | first sorted position << 8 | count << 20.
+__device__ __forceinline__ void build_items(const uint32_t c[4], uint32_t *items, uint16_t *start, uint32_t *nitems)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t nfull = 0, csum = 0, nrem[ITEM_CAP];
+#pragma unroll
+    for (int r = 0; r < ITEM_CAP; ++r) nrem[r] = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        csum += c[j];
+        nfull += c[j] / ITEM_CAP;
+#pragma unroll
+        for (int r = 1; r < ITEM_CAP; ++r) nrem[r] += (c[j] % ITEM_CAP) == (uint32_t)r ? 1u : 0u;
+    }
+    uint32_t crun = wave_incl_scan(csum) - csum;
+    const uint32_t fincl = wave_incl_scan(nfull);
+    uint32_t frun = fincl - nfull;
+    uint32_t total = __shfl(fincl, 63);
+    uint32_t rrun[ITEM_CAP];
+#pragma unroll
+    for (int r = ITEM_CAP - 1; r >= 1; --r) {
+        const uint32_t inc = wave_incl_scan(nrem[r]);
+        rrun[r] = total + inc - nrem[r];
+        total += __shfl(inc, 63);
+    }
+    if (lane == 0) *nitems = total;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int b = 4 * lane + j;
+        if (b < NBIN) start[b] = (uint16_t)crun;
+        const uint32_t nf = c[j] / ITEM_CAP, rm = c[j] % ITEM_CAP;
+        for (uint32_t q = 0; q < nf; ++q)
+            items[frun + q] = (uint32_t)b | ((crun + q * ITEM_CAP) << 8) | ((uint32_t)ITEM_CAP << 20);
+        frun += nf;
+#pragma unroll
+        for (int r = 1; r < ITEM_CAP; ++r)
+            if (rm == (uint32_t)r) items[rrun[r]++] = (uint32_t)b | ((crun + nf * ITEM_CAP) << 8) | (rm << 20);
+        crun += c[j];
+    }
+}
+
+// Phase B, FUSED form (both waves): counting sort of the chunk's nc
+// particles by bin and the item list.  Wave w counts the 64-particle
+// batches w, w + 2, ... into its own counts; wave 0 then builds the items
+// from the summed counts; both waves place their particles.  Within a bin,
+// wave 0's particles precede wave 1's, each in slot order, so the order
+// (and every sum) is the same in every run.
+__device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
 {
     static_assert(TILE_WAVES == 2, "two counting waves");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -493,49 +542,14 @@ __device__ __forceinline__ void sort_chunk(TL &L, int nc)
     }
     __syncthreads();
     if (wave == 0) {
-        // Lane l owns bins 4l .. 4l+3.  A bin of c particles gives c / ITEM_CAP
-        // full items and one item of the remainder.  Full items come first,
-        // then remainders by decreasing size, so a batch of 64 items mostly
-        // holds items of equal length (fewer idle lanes per round) and the
-        // items of one bin mostly fall into different batches (fewer turns in
-        // the flush).
-        uint32_t c[4], nfull = 0, csum = 0, nrem[ITEM_CAP];
-#pragma unroll
-        for (int r = 0; r < ITEM_CAP; ++r) nrem[r] = 0;
+        uint32_t c[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int b = 4 * lane + j;
             c[j] = b < NBIN ? (uint32_t)L.cnt[b] + (uint32_t)L.cnt1[b] : 0u;
-            csum += c[j];
-            nfull += c[j] / ITEM_CAP;
-#pragma unroll
-            for (int r = 1; r < ITEM_CAP; ++r) nrem[r] += (c[j] % ITEM_CAP) == (uint32_t)r ? 1u : 0u;
         }
-        uint32_t crun = wave_incl_scan(csum) - csum;
-        const uint32_t fincl = wave_incl_scan(nfull);
-        uint32_t frun = fincl - nfull;
-        uint32_t total = __shfl(fincl, 63);
-        uint32_t rrun[ITEM_CAP];
-#pragma unroll
-        for (int r = ITEM_CAP - 1; r >= 1; --r) {
-            const uint32_t inc = wave_incl_scan(nrem[r]);
-            rrun[r] = total + inc - nrem[r];
-            total += __shfl(inc, 63);
-        }
-        if (lane == 0) L.nitems = total;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int b = 4 * lane + j;
-            if (b < NBIN) L.cnt1[b] = (uint16_t)crun;      // now the bin's first sorted position
-            const uint32_t nf = c[j] / ITEM_CAP, rm = c[j] % ITEM_CAP;
-            for (uint32_t q = 0; q < nf; ++q)
-                L.items[frun + q] = (uint32_t)b | ((crun + q * ITEM_CAP) << 8) | ((uint32_t)ITEM_CAP << 20);
-            frun += nf;
-#pragma unroll
-            for (int r = 1; r < ITEM_CAP; ++r)
-                if (rm == (uint32_t)r) L.items[rrun[r]++] = (uint32_t)b | ((crun + nf * ITEM_CAP) << 8) | (rm << 20);
-            crun += c[j];
-        }
+        __asm__ volatile("" ::: "memory");
+        build_items(c, L.items, L.cnt1, &L.nitems);      // cnt1 becomes each bin's first position
     }
     __syncthreads();
     for (int s0 = wave * 64; s0 < nc; s0 += TILE_THREADS) {
@@ -546,6 +560,44 @@ __device__ __forceinline__ void sort_chunk(TL &L, int nc)
             L.perm[(uint32_t)L.cnt1[key] + off + L.rnk[sl]] = (uint16_t)sl;
         }
     }
+}
+
+// Phase B, FAST form (the tile's one wave): the same sort, in slot order
+// within each bin.  One wave's LDS operations execute in order, so the
+// steps need no barrier; the compiler barriers keep their order.
+__device__ __forceinline__ void sort_chunk_wave(TileKeys &L, int nc)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (int b = lane; b < NBIN; b += 64) L.cnt[b] = 0;
+    __asm__ volatile("" ::: "memory");
+    for (int s0 = 0; s0 < nc; s0 += 64) {
+        const int sl = s0 + lane;
+        const uint32_t key = sl < nc ? (uint32_t)L.key[sl] : NO_BIN;
+        const bool v = key != NO_BIN;
+        const uint64_t m = match_key8(key) & __ballot(v);
+        const uint32_t below = (uint32_t)__popcll(m & lt);
+        uint32_t prev = 0;
+        if (v) prev = L.cnt[key];
+        if (v && below == 0) L.cnt[key] = (uint16_t)(prev + (uint32_t)__popcll(m));
+        if (v) L.rnk[sl] = (uint16_t)(prev + below);
+        __asm__ volatile("" ::: "memory");
+    }
+    uint32_t c[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int b = 4 * lane + j;
+        c[j] = b < NBIN ? (uint32_t)L.cnt[b] : 0u;
+    }
+    __asm__ volatile("" ::: "memory");
+    build_items(c, L.items, L.start, &L.nitems);
+    __asm__ volatile("" ::: "memory");
+    for (int s0 = 0; s0 < nc; s0 += 64) {
+        const int sl = s0 + lane;
+        const uint32_t key = sl < nc ? (uint32_t)L.key[sl] : NO_BIN;
+        if (key != NO_BIN) L.perm[(uint32_t)L.start[key] + L.rnk[sl]] = (uint16_t)sl;
+    }
+    __asm__ volatile("" ::: "memory");
 }
 
 // Phase C, one particle given its record in registers (fx, q0, hB and G as
@@ -743,8 +795,7 @@ __device__ __forceinline__ void scatter_chunk(TileLds &L, int nc, float inv_h, f
     __syncthreads();
 }
 
-template <class TL>
-__device__ __forceinline__ void tile_clear(TL &L)
+__device__ __forceinline__ void tile_clear(TileLds &L)
 {
     for (int k = threadIdx.x; k < TILE_WAVES * A_SLOTS; k += TILE_THREADS)
         L.arena[k / A_SLOTS][k % A_SLOTS] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -752,8 +803,7 @@ __device__ __forceinline__ void tile_clear(TL &L)
 
 // Sum the waves' arenas into staging slot `slot` (logical node order) and
 // mark the node blocks that received mass.
-template <class TL>
-__device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const TL &L, float4 *staging,
+__device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const TileLds &L, float4 *staging,
                                          uint32_t slot, bool staged, uint32_t *flags)
 {
     __syncthreads();
@@ -778,12 +828,12 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 }
 
 // One tile (descriptor d = first particle, count, block, staging slot) of
-// the FAST-form P2G; returns health bits.  All threads of the workgroup.
-// Phase A loads only positions (bin keys); phase C reloads x, v, C and the
-// stored stress of each item's particles.  Without LDS records a workgroup
-// needs 20.7 KB instead of 53 KB, so the register file, not LDS, bounds the
-// workgroups per CU (four instead of three: 90 vs 106 us per 1M-particle
-// snow step, measured).
+// the FAST-form P2G, by one wave; returns health bits.  Phase A loads only
+// positions (bin keys); phase C reloads x, v, C and the stored stress of
+// each item's particles, so the tile needs no LDS records (12.5 KB instead
+// of 53 KB) and no barrier, and the register file alone bounds the tiles
+// in flight per CU (8 one-wave tiles instead of 3 two-wave ones with LDS
+// records: 77 vs 106 us per 1M-particle snow step, measured).
 template <int MAT>
 __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, const PartView &pv, const uint4 d,
                                              uint32_t staging_cap, float dt, float4 *__restrict__ staging,
@@ -792,45 +842,55 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     uint32_t bits = 0;
     const TileGeo t = tile_geo(g, d.z);
     const bool staged = d.w < staging_cap;
-    tile_clear(L);
-    const int wave0 = (int)(threadIdx.x & ~63u);
-    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    for (int k = lane; k < A_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
         const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
         // A: all position loads of the chunk first (lanes past its end reload
         // its last particle), then bin keys
-        float xs[A_PASSES][3];
+        float xs[P2G_PASSES][3];
 #pragma unroll
-        for (int it = 0; it < A_PASSES; ++it) {
-            const uint32_t slot = (uint32_t)min(it * TILE_THREADS + (int)threadIdx.x, nc - 1);
+        for (int it = 0; it < P2G_PASSES; ++it) {
+            const uint32_t slot = (uint32_t)min(it * 64 + lane, nc - 1);
 #pragma unroll
             for (int a = 0; a < 3; ++a) xs[it][a] = ld_plane(pv, PL_X + a, d.x + c0 + slot);
         }
 #pragma unroll
-        for (int it = 0; it < A_PASSES; ++it) {
-            if (it * TILE_THREADS + wave0 >= nc) break;              // wave-uniform
-            const int slot = it * TILE_THREADS + (int)threadIdx.x;
+        for (int it = 0; it < P2G_PASSES; ++it) {
+            if (it * 64 >= nc) break;
+            const int slot = it * 64 + lane;
             const bool active = slot < nc;
             Stencil s;
             s.b[0] = s.b[1] = s.b[2] = 0;
             if (active) bits |= make_stencil(g, xs[it], s);
             bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, s, staged, spill, counters);
         }
+        __asm__ volatile("" ::: "memory");
         // B and C
-        __syncthreads();
-        sort_chunk(L, nc);
-        __syncthreads();
+        sort_chunk_wave(L, nc);
         const int nb = (int)((L.nitems + 63u) >> 6);
-        for (int b = wave; b < nb; b += TILE_WAVES)
-            scatter_items_reload<MAT>(L, L.arena[wave], b, g, mat, pv, (int)(d.x + c0), dt);
-        __syncthreads();
+        for (int b = 0; b < nb; ++b) scatter_items_reload<MAT>(L, L.arena, b, g, mat, pv, (int)(d.x + c0), dt);
+        __asm__ volatile("" ::: "memory");
     }
-    tile_end(g, t, L, staging, d.w, staged, flags);
+    // the arena into the tile's staging slot (logical node order); mark the
+    // node blocks that received mass
+    if (staged) {
+        const int64_t dst = (int64_t)d.w * ARENA_NODES;
+        for (int k = lane; k < ARENA_NODES; k += 64) {
+            const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
+            const float4 v = L.arena[a_slot(kx, ky, kz)];
+            gstore4(staging, dst + k, v);
+            const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
+            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+                gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
+        }
+    }
+    __asm__ volatile("" ::: "memory");
     return bits;
 }
 
 template <int MAT>
-__global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_block(GridP g, MatP mat, PartView pv,
+__global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_block(GridP g, MatP mat, PartView pv,
                                                             const uint4 *__restrict__ desc,
                                                             const uint32_t *__restrict__ tile_count, uint32_t desc_cap,
                                                             uint32_t staging_cap, float dt,
@@ -862,7 +922,7 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
 // against the workgroups per CU it leaves).
 constexpr int P2G_CHUNK = 64;
 template <int MAT>
-__global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
+__global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
     __shared__ TileKeys L;
     __shared__ uint32_t s_end[P2G_CHUNK];
@@ -1192,7 +1252,7 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     }
     uint32_t *hw = nullptr;
     if ((rc = spill_host_word(s, group, &hw))) return rc;
-    dim3 gr(wg), bl(TILE_THREADS);
+    dim3 gr(wg), bl(P2G_THREADS);
     {
     PhaseTimer tk(s, TK_P2G);
     switch (m.type) {
@@ -1305,7 +1365,7 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
 // Material `mat` of the scenes [0, nslices) of tab, in chunks of P2G_CHUNK.
 int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs)
 {
-    dim3 gr(wgs), bl(TILE_THREADS);
+    dim3 gr(wgs), bl(P2G_THREADS);
     for (int c0 = 0; c0 < nslices; c0 += P2G_CHUNK) {
         const int n = std::min(P2G_CHUNK, nslices - c0);
         bool any = false;
