@@ -190,37 +190,54 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters, &pv, i);
         out.S = Jn;
     } else if (MAT == MPM_SNOW) {                                        // :620-709
-        float Ft[9], U[9], sg[3], V[9];
+        float Ft[9];
         matmul3<float>(A, F, Ft);
-        if (mat.warm_sweeps > 0) svd_warm(pv, i, Ft, U, sg, V, mat.warm_sweeps, sweeps, counters);
-        else svd_checked(Ft, U, sg, V, sweeps, counters);
-        float ratio = 1.0f;
-        float sc[3];
+        if (sv_in_band<float>(Ft, 1.0f - mat.theta_c, 1.0f + mat.theta_s)) {
+            // no singular value reaches the clamp: F_e = F_tr, J_p unchanged
+            // (its plane is left as it is); the stress needs only the polar
+            // rotation, three Newton steps instead of the SVD
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            sc[k] = fminf(fmaxf(sg[k], 1.0f - mat.theta_c), 1.0f + mat.theta_s);   // :658-660
-            ratio *= sg[k] / sc[k];
+            for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Ft[k]);
+            if (!(Sp > 0.f)) bits |= MPM_H_DET_NONPOS;
+            if (want_tau) {
+                float R[9];
+                polar_newton<float>(Ft, R, 3);
+                const float hc = expf(mat.xi * (1.0f - Sp));
+                fcr_tau_polar<float>(Ft, R, mat.mu * hc, mat.lam * hc, out.tau);
+            }
+            out.S = Sp;
+        } else {
+            float U[9], sg[3], V[9];
+            if (mat.warm_sweeps > 0) svd_warm(pv, i, Ft, U, sg, V, mat.warm_sweeps, sweeps, counters);
+            else svd_checked(Ft, U, sg, V, sweeps, counters);
+            float ratio = 1.0f;
+            float sc[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sc[k] = fminf(fmaxf(sg[k], 1.0f - mat.theta_c), 1.0f + mat.theta_s);   // :658-660
+                ratio *= sg[k] / sc[k];
+            }
+            // Fe = U S* V^T, written as Ftr + U (S* - S) V^T: identical in exact
+            // arithmetic, and exactly Ftr when no singular value is clamped, so
+            // the FP32 SVD's rounding does not leak into F every step.
+            float ds[3] = {sc[0] - sg[0], sc[1] - sg[1], sc[2] - sg[2]};
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    st(pv, PL_F + 3 * r + c, i,
+                       Ft[3 * r + c] + (U[3 * r] * ds[0] * V[3 * c] + U[3 * r + 1] * ds[1] * V[3 * c + 1] +
+                                        U[3 * r + 2] * ds[2] * V[3 * c + 2]));
+            // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
+            float Jp = Sp * ratio;
+            st(pv, PL_S, i, Jp);
+            if (!(Jp > 0.f)) bits |= MPM_H_DET_NONPOS;
+            if (want_tau) {
+                float hc = expf(mat.xi * (1.0f - Jp));
+                fcr_tau_from_svd<float>(U, sc, mat.mu * hc, mat.lam * hc, out.tau);
+            }
+            out.S = Jp;
         }
-        // Fe = U S* V^T, written as Ftr + U (S* - S) V^T: identical in exact
-        // arithmetic, and exactly Ftr when no singular value is clamped, so
-        // the FP32 SVD's rounding does not leak into F every step.
-        float ds[3] = {sc[0] - sg[0], sc[1] - sg[1], sc[2] - sg[2]};
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                st(pv, PL_F + 3 * r + c, i,
-                   Ft[3 * r + c] + (U[3 * r] * ds[0] * V[3 * c] + U[3 * r + 1] * ds[1] * V[3 * c + 1] +
-                                    U[3 * r + 2] * ds[2] * V[3 * c + 2]));
-        // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
-        float Jp = Sp * ratio;
-        st(pv, PL_S, i, Jp);
-        if (!(Jp > 0.f)) bits |= MPM_H_DET_NONPOS;
-        if (want_tau) {
-            float hc = expf(mat.xi * (1.0f - Jp));
-            fcr_tau_from_svd<float>(U, sc, mat.mu * hc, mat.lam * hc, out.tau);
-        }
-        out.S = Jp;
     } else {                                                             // :710-713
         float J = (1.0f + dt * (L[0] + L[4] + L[8])) * Sp;
         st(pv, PL_S, i, J);

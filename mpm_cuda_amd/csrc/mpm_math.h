@@ -261,6 +261,80 @@ MPM_HD T svd3(const T F[9], T U[9], T s[3], T V[9], int sweeps, const T *V0 = nu
     return off;
 }
 
+// ---------------------------------------------------------------------------
+// Elastic snow update without the SVD.  A trial F whose singular values all
+// lie strictly inside the plastic band [lo, hi] is not clamped
+// (MPMCUDA.cu:658-678 leaves F_e = F_tr and F_p unchanged), and its stress
+// needs only the polar rotation R (tau = 2 mu (F - R) F^T + lam J (J - 1) I,
+// MPMCUDA.cu:229-264).
+
+// All singular values of F strictly inside (lo, hi) and det F > 0: the
+// eigenvalues of S = F^T F lie in (lo^2, hi^2) iff S - lo^2 I and
+// hi^2 I - S are positive definite (leading principal minors > 0).
+template <class T>
+MPM_HD bool sv_in_band(const T F[9], T lo, T hi)
+{
+    T S[6];   // xx, yy, zz, xy, xz, yz of F^T F
+    S[0] = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+    S[1] = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+    S[2] = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+    S[3] = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+    S[4] = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+    S[5] = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+    bool ok = det3(F) > T(0);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        // side 0: S - lo^2 I; side 1: hi^2 I - S
+        const T sg = side ? T(-1) : T(1), sh = side ? hi * hi : -(lo * lo);
+        const T a = sg * S[0] + sh, b = sg * S[1] + sh, c = sg * S[2] + sh;
+        const T d = sg * S[3], e = sg * S[4], f = sg * S[5];
+        const T m2 = a * b - d * d;
+        const T m3 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+        ok = ok && a > T(0) && m2 > T(0) && m3 > T(0);
+    }
+    return ok;
+}
+
+// Polar rotation of F (det F > 0) by Newton's iteration R <- (R + R^-T) / 2
+// from R = F (Higham 1986): quadratic convergence, the distance of the
+// singular values from 1 squares (and halves) every step, so three steps
+// reach FP32 precision for singular values within a few percent of 1.
+template <class T>
+MPM_HD void polar_newton(const T F[9], T R[9], int iters)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = F[k];
+    for (int it = 0; it < iters; ++it) {
+        // R^-T = cof(R) / det R
+        T c[9];
+        c[0] = R[4] * R[8] - R[5] * R[7]; c[1] = R[5] * R[6] - R[3] * R[8]; c[2] = R[3] * R[7] - R[4] * R[6];
+        c[3] = R[2] * R[7] - R[1] * R[8]; c[4] = R[0] * R[8] - R[2] * R[6]; c[5] = R[1] * R[6] - R[0] * R[7];
+        c[6] = R[1] * R[5] - R[2] * R[4]; c[7] = R[2] * R[3] - R[0] * R[5]; c[8] = R[0] * R[4] - R[1] * R[3];
+        const T det = R[0] * c[0] + R[1] * c[1] + R[2] * c[2];
+        const T hi = T(0.5) * r_rcp(det);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R[k] = T(0.5) * R[k] + hi * c[k];
+    }
+}
+
+// tau = 2 mu (F - R) F^T + lam J (J - 1) I as (xx, yy, zz, xy, xz, yz).
+template <class T>
+MPM_HD void fcr_tau_polar(const T F[9], const T R[9], T mu, T lam, T tau6[6])
+{
+    const T J = det3(F);
+    const T l = lam * J * (J - T(1));
+    T D[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) D[k] = T(2) * mu * (F[k] - R[k]);
+    // (D F^T)_rc = sum_k D_rk F_ck; the product is symmetric in exact arithmetic
+    tau6[0] = D[0] * F[0] + D[1] * F[1] + D[2] * F[2] + l;
+    tau6[1] = D[3] * F[3] + D[4] * F[4] + D[5] * F[5] + l;
+    tau6[2] = D[6] * F[6] + D[7] * F[7] + D[8] * F[8] + l;
+    tau6[3] = T(0.5) * (D[0] * F[3] + D[1] * F[4] + D[2] * F[5] + D[3] * F[0] + D[4] * F[1] + D[5] * F[2]);
+    tau6[4] = T(0.5) * (D[0] * F[6] + D[1] * F[7] + D[2] * F[8] + D[6] * F[0] + D[7] * F[1] + D[8] * F[2]);
+    tau6[5] = T(0.5) * (D[3] * F[6] + D[4] * F[7] + D[5] * F[8] + D[6] * F[3] + D[7] * F[4] + D[8] * F[5]);
+}
+
 // Kirchhoff stress of the fixed-corotated model from an SVD of F:
 // tau = 2 mu (F - R) F^T + lam J (J - 1) I with R = U V^T (MPMCUDA.cu:229-264)
 // equals U diag(2 mu (s_i - 1) s_i + lam J (J - 1)) U^T.  Output is the
