@@ -767,10 +767,15 @@ __device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena,
 #pragma unroll
     for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t rounds = wave_max_u32(cnt);
-    for (uint32_t r = 0; r < rounds; ++r)
+    // the next round's particle is loaded while this one is accumulated;
+    // lanes without a next particle reload a valid one (one basic block)
+    auto slot_of = [&](uint32_t r) -> int { return cnt ? (int)L.perm[first + min(r, cnt - 1u)] : 0; };
+    RawP nxt;
+    load_raw<MAT>(pv, i0 + slot_of(0), nxt);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const RawP raw = nxt;
+        if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), nxt);
         if (r < cnt) {
-            RawP raw;
-            load_raw<MAT>(pv, i0 + (int)L.perm[first + r], raw);
             Scatter sc;
             scatter_from_raw<MAT>(raw, mat, dt, sc);
             Stencil st;
@@ -779,6 +784,7 @@ __device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena,
             make_record<MAT>(g, st, sc, q0, hB, Gs);
             accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
         }
+    }
     flush_item(arena, has, bin, acc);
 }
 

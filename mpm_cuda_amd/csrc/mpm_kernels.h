@@ -91,14 +91,23 @@ __device__ __forceinline__ void stress_from_F(const MatP &m, const float F[9], f
                                               float tau6[6], uint32_t *counters, const PartView *warm_pv = nullptr,
                                               int warm_i = 0)
 {
-    float U[9], s[3], V[9];
-    if (warm_pv && m.warm_sweeps > 0) svd_warm(*warm_pv, warm_i, F, U, s, V, m.warm_sweeps, sweeps, counters);
-    else svd_checked(F, U, s, V, sweeps, counters);
     float mu = m.mu, lam = m.lam;
     if (MAT == MPM_SNOW) {
         float hc = expf(m.xi * (1.0f - Jp));           // MPMCUDA.cu:345
         mu *= hc; lam *= hc;
     }
+    // F within a factor 2 of a rotation (singular values in (0.5, 2)): the
+    // stress needs only the polar rotation; five Newton steps reach FP32
+    // there (a singular value of 2 or 1/2 goes 1.25, 1.025, 1 + 3e-4, 1 + 5e-8)
+    if (sv_in_band<float>(F, 0.5f, 2.0f)) {
+        float R[9];
+        polar_newton<float>(F, R, 5);
+        fcr_tau_polar<float>(F, R, mu, lam, tau6);
+        return;
+    }
+    float U[9], s[3], V[9];
+    if (warm_pv && m.warm_sweeps > 0) svd_warm(*warm_pv, warm_i, F, U, s, V, m.warm_sweeps, sweeps, counters);
+    else svd_checked(F, U, s, V, sweeps, counters);
     fcr_tau_from_svd<float>(U, s, mu, lam, tau6);
 }
 
