@@ -308,6 +308,15 @@ int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, i
  * the FP64 fallback of the step kernels. U, V row-major, s[3] per matrix. */
 int mpm_probe_svd(const float *F, int64_t n, int32_t sweeps, float *U, float *s, float *V);
 
+/* Kirchhoff stress tau (xx, yy, zz, xy, xz, yz per matrix) of the fixed-
+ * corotated model as the step kernels evaluate it for n row-major FP32
+ * matrices F (MPMCUDA.cu:229-264): the polar rotation by Newton steps for F
+ * within a factor 2 of a rotation, the Jacobi SVD (`sweeps`, FP64 fallback)
+ * otherwise.  type MPM_JELLO or MPM_SNOW; snow scales mu and lam by
+ * exp(xi (1 - Jp[i])) (MPMCUDA.cu:342-349), Jp may be NULL for jello. */
+int mpm_probe_stress(const float *F, const float *Jp, int64_t n, int32_t type, double mu, double lam, double xi,
+                     int32_t sweeps, float *tau);
+
 /* Largest column orthogonality residual |b_p . b_q| / (|b_p| |b_q|) of
  * B = F V after `sweeps` Jacobi sweeps started from V0 = R(q / |q|) (q[n][4];
  * q == NULL: V0 = I), for n row-major FP32 matrices: the acceptance test of

@@ -115,6 +115,8 @@ _SIGS = {
     "mpm_probe_svd": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_sort": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "mpm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "mpm_probe_stress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                   C.c_int32, C.c_void_p]),
     "mpm_batch_shape_default": (None, [C.POINTER(MpmBatchShape)]),
     "mpm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(MpmBatchShape),
                                    C.POINTER(C.c_void_p)]),
@@ -517,6 +519,20 @@ def probe_svd(F, sweeps: int = 4):
     check(lib().mpm_probe_svd(F.ctypes.data_as(C.c_void_p), n, int(sweeps), U.ctypes.data_as(C.c_void_p),
                               s.ctypes.data_as(C.c_void_p), V.ctypes.data_as(C.c_void_p)))
     return U.reshape(n, 3, 3), s, V.reshape(n, 3, 3)
+
+
+def probe_stress(F, type: int, mu: float, lam: float, xi: float = 10.0, Jp=None, sweeps: int = 4):
+    """Kirchhoff stress of the step kernels, (n, 6) as xx, yy, zz, xy, xz, yz."""
+    F = _f32(F).reshape(-1, 9)
+    n = F.shape[0]
+    tau = np.empty((n, 6), np.float32)
+    jp = None
+    if Jp is not None:
+        Jp = _f32(Jp).reshape(n)
+        jp = Jp.ctypes.data_as(C.c_void_p)
+    check(lib().mpm_probe_stress(F.ctypes.data_as(C.c_void_p), jp, n, int(type), float(mu), float(lam), float(xi),
+                                 int(sweeps), tau.ctypes.data_as(C.c_void_p)))
+    return tau
 
 
 def probe_svd_residual(F, q=None, sweeps: int = 1):

@@ -31,6 +31,22 @@ __global__ void k_probe_svd(const float *F, int n, int sweeps, float *U, float *
     s[3 * i] = sg[0]; s[3 * i + 1] = sg[1]; s[3 * i + 2] = sg[2];
 }
 
+// Kirchhoff stress of the step kernels (stress_from_F: polar rotation for F
+// within a factor 2 of a rotation, the SVD otherwise) for n matrices.
+template <int MAT>
+__global__ void k_probe_stress(const float *F, const float *Jp, int n, MatP m, int sweeps, float *tau,
+                               uint32_t *counters)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float f[9], t[6];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k] = F[9 * i + k];
+    stress_from_F<MAT>(m, f, MAT == MPM_SNOW ? Jp[i] : 1.0f, sweeps, t, counters);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau[6 * i + k] = t[k];
+}
+
 // Column orthogonality residual of F V after `sweeps` Jacobi sweeps from
 // V0 = R(q / |q|) (q == nullptr: V0 = I).
 __global__ void k_probe_svd_residual(const float *F, const float *q, int n, int sweeps, float *off)
@@ -172,6 +188,39 @@ int mpm_probe_svd(const float *F, int64_t n, int32_t sweeps, float *U, float *s,
     MPM_HIP(hipMemcpy(U, dU, n * 36, hipMemcpyDeviceToHost));
     MPM_HIP(hipMemcpy(V, dV, n * 36, hipMemcpyDeviceToHost));
     MPM_HIP(hipMemcpy(s, ds, n * 12, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_probe_stress(const float *F, const float *Jp, int64_t n, int32_t type, double mu, double lam, double xi,
+                     int32_t sweeps, float *tau)
+{
+    if (n > 0 && (!F || !tau || (type == MPM_SNOW && !Jp))) return set_error(MPM_ERR_INVALID, "null argument");
+    if (type != MPM_JELLO && type != MPM_SNOW) return set_error(MPM_ERR_INVALID, "type must be JELLO or SNOW");
+    if (n <= 0) return MPM_OK;
+    if (sweeps < 1 || sweeps > 16) return set_error(MPM_ERR_INVALID, "sweeps out of range [1,16]");
+    if (n > (1ll << 26)) return set_error(MPM_ERR_INVALID, "too many matrices");
+    MatP m;
+    memset(&m, 0, sizeof m);
+    m.type = type;
+    m.mu = (float)mu;
+    m.lam = (float)lam;
+    m.xi = (float)xi;
+    DevBufs b;
+    float *dF = nullptr, *dJ = nullptr, *dt = nullptr;
+    uint32_t *cnt = nullptr;
+    MPM_HIP(b.alloc(&dF, n * 36));
+    MPM_HIP(b.alloc(&dJ, n * 4));
+    MPM_HIP(b.alloc(&dt, n * 24));
+    MPM_HIP(b.alloc(&cnt, CT_COUNT * 4));
+    MPM_HIP(hipMemset(cnt, 0, CT_COUNT * 4));
+    MPM_HIP(hipMemcpy(dF, F, n * 36, hipMemcpyHostToDevice));
+    if (type == MPM_SNOW) MPM_HIP(hipMemcpy(dJ, Jp, n * 4, hipMemcpyHostToDevice));
+    else MPM_HIP(hipMemset(dJ, 0, n * 4));
+    const dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    if (type == MPM_SNOW) hipLaunchKernelGGL(k_probe_stress<MPM_SNOW>, gr, bl, 0, 0, dF, dJ, (int)n, m, sweeps, dt, cnt);
+    else hipLaunchKernelGGL(k_probe_stress<MPM_JELLO>, gr, bl, 0, 0, dF, dJ, (int)n, m, sweeps, dt, cnt);
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipMemcpy(tau, dt, n * 24, hipMemcpyDeviceToHost));
     return MPM_OK;
 }
 

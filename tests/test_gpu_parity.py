@@ -294,3 +294,38 @@ def test_copy_bandwidth_probe(gpu):
         capi.probe_copy_bandwidth(1000, 5)
     with pytest.raises(capi.MpmError):
         capi.probe_copy_bandwidth(1 << 20, 0)
+
+
+@pytest.mark.parametrize("type_", [capi.JELLO, capi.SNOW], ids=["jello", "snow"])
+def test_stress_paths_against_oracle(gpu, type_):
+    """The device stress on both of its paths -- the polar rotation by Newton
+    steps for F within a factor 2 of a rotation, the Jacobi SVD outside --
+    against the oracle's FP64 fixed-corotated stress (MPMCUDA.cu:229-264)
+    with the snow hardening exp(xi (1 - J_p)).  Singular values log-uniform in
+    [0.3, 3] cover both paths; bar 2e-5 of max|tau| per matrix."""
+    rng = np.random.default_rng(17)
+    n = 4000
+    mu, lam, xi = O.lame(1.4e5, 0.2)[0], O.lame(1.4e5, 0.2)[1], 10.0
+    F = np.empty((n, 3, 3))
+    for k in range(n):
+        U, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        V, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        U *= np.sign(np.linalg.det(U))
+        V *= np.sign(np.linalg.det(V))
+        s = np.exp(rng.uniform(np.log(0.3), np.log(3.0), 3))
+        F[k] = U @ np.diag(s) @ V.T
+    F = F.astype(np.float32)
+    Jp = rng.uniform(0.9, 1.05, n).astype(np.float32)
+    got = capi.probe_stress(F, type_, mu, lam, xi, Jp if type_ == capi.SNOW else None)
+    inside = np.all((np.linalg.svd(F.astype(np.float64), compute_uv=False) > 0.5)
+                    & (np.linalg.svd(F.astype(np.float64), compute_uv=False) < 2.0), axis=1)
+    assert 0.1 * n < inside.sum() < 0.9 * n          # both paths exercised
+    worst = {True: 0.0, False: 0.0}
+    for k in range(n):
+        h = np.exp(xi * (1.0 - float(Jp[k]))) if type_ == capi.SNOW else 1.0
+        t = O.fcr_tau(F[k].astype(np.float64), mu * h, lam * h)
+        want = np.array([t[0, 0], t[1, 1], t[2, 2], t[0, 1], t[0, 2], t[1, 2]])
+        e = float(np.abs(got[k] - want).max() / np.abs(want).max())
+        worst[bool(inside[k])] = max(worst[bool(inside[k])], e)
+    print({"polar": f"{worst[True]:.2e}", "svd": f"{worst[False]:.2e}"})
+    assert worst[True] < 2e-5 and worst[False] < 2e-5, worst
