@@ -124,33 +124,41 @@ static int scan_excl(hipStream_t st, const uint32_t *in, uint32_t *out, int n, u
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits.  A tile of RS_TILE keys is handled by
-// 4 waves, each owning a contiguous run of 1024 keys processed 64 at a time;
-// equal digits inside a wave are matched with 8 ballots, so ranks follow
-// input order and the sort is stable.
+// Stable LSD radix sort with DB-bit digits (8, or 9 so that 17-18-bit block
+// keys take two passes instead of three).  A tile of RS_TILE keys is handled
+// by 4 waves, each owning a contiguous run of 1024 keys processed 64 at a
+// time; equal digits inside a wave are matched with DB ballots, so ranks
+// follow input order and the sort is stable.
 constexpr int RS_T = 256, RS_WAVE_KEYS = 1024, RS_ITERS = RS_WAVE_KEYS / 64, RS_TILE = 4 * RS_WAVE_KEYS;
 
-__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *hist,
-                                                  int ntiles)
+template <int DB>
+__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t dmask,
+                                                  uint32_t *hist, int ntiles)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr int NB = 1 << DB;
+    const uint32_t MASK = dmask & (NB - 1);       // the last digit may be narrower
+    __shared__ uint32_t h[NB];
+    for (int d = threadIdx.x; d < NB; d += RS_T) h[d] = 0;
     __syncthreads();
     int base = blockIdx.x * RS_TILE;
     for (int k = 0; k < RS_TILE / RS_T; ++k) {
         int i = base + k * RS_T + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & MASK], 1u);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
+template <int DB>
 __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                                                     uint32_t *vout, int n, int shift, const uint32_t *hist,
-                                                     int ntiles)
+                                                     uint32_t *vout, int n, int shift, uint32_t dmask,
+                                                     const uint32_t *hist, int ntiles)
 {
-    __shared__ uint32_t cnt[4][256];
-    for (int k = 0; k < 4; ++k) cnt[k][threadIdx.x] = 0;
+    constexpr int NB = 1 << DB;
+    const uint32_t MASK = dmask & (NB - 1);
+    __shared__ uint32_t cnt[4][NB];
+    for (int k = 0; k < 4; ++k)
+        for (int d = threadIdx.x; d < NB; d += RS_T) cnt[k][d] = 0;
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -160,10 +168,10 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
         int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         bool valid = i < n;
         uint32_t k = valid ? kin[i] : 0u;
-        uint32_t d = (k >> shift) & 255u;
+        uint32_t d = (k >> shift) & MASK;
         uint64_t m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < DB; ++b) {
             bool bit = (d >> b) & 1u;
             uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
@@ -177,11 +185,11 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
         rank[it] = prev + below;
     }
     __syncthreads();
-    {
+    for (int d = threadIdx.x; d < NB; d += RS_T) {
         uint32_t run = 0;
         for (int k = 0; k < 4; ++k) {
-            uint32_t t = cnt[k][threadIdx.x];
-            cnt[k][threadIdx.x] = run;
+            uint32_t t = cnt[k][d];
+            cnt[k][d] = run;
             run += t;
         }
     }
@@ -190,7 +198,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
     for (int it = 0; it < RS_ITERS; ++it) {
         int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         if (i < n) {
-            uint32_t d = (key[it] >> shift) & 255u;
+            uint32_t d = (key[it] >> shift) & MASK;
             uint32_t dst = hist[(int64_t)d * ntiles + blockIdx.x] + cnt[w][d] + rank[it];
             if (dst < (uint32_t)n) {      // always true for a consistent histogram
                 kout[dst] = key[it];
@@ -201,7 +209,8 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
 }
 
 // Sorts (keys, vals) of length n by the low `bits` key bits.  Returns through
-// `in_alt` whether the result ended in the *_alt buffers.
+// `in_alt` whether the result ended in the *_alt buffers.  `hist` holds
+// 512 x ntiles entries.
 static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                       int n, int bits, uint32_t *hist, uint32_t *partial, bool &in_alt)
 {
@@ -209,11 +218,20 @@ static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *
     if (n <= 1) return 0;
     int ntiles = (n + RS_TILE - 1) / RS_TILE;
     uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
-    for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, hist, ntiles);
-        int rc = scan_excl(st, hist, hist, 256 * ntiles, partial, nullptr);
+    const int db = (bits > 16 && bits <= 18) ? 9 : 8;
+    for (int shift = 0; shift < bits; shift += db) {
+        const int w = std::min(db, bits - shift);
+        const uint32_t dmask = (w >= 32) ? ~0u : ((1u << w) - 1u);
+        if (db == 9) hipLaunchKernelGGL(k_rs_hist<9>, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, dmask, hist, ntiles);
+        else hipLaunchKernelGGL(k_rs_hist<8>, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, dmask, hist, ntiles);
+        int rc = scan_excl(st, hist, hist, (1 << db) * ntiles, partial, nullptr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_T), 0, st, ka, va, kb, vb, n, shift, hist, ntiles);
+        if (db == 9)
+            hipLaunchKernelGGL(k_rs_scatter<9>, dim3(ntiles), dim3(RS_T), 0, st, ka, va, kb, vb, n, shift, dmask, hist,
+                               ntiles);
+        else
+            hipLaunchKernelGGL(k_rs_scatter<8>, dim3(ntiles), dim3(RS_T), 0, st, ka, va, kb, vb, n, shift, dmask, hist,
+                               ntiles);
         std::swap(ka, kb);
         std::swap(va, vb);
         in_alt = !in_alt;
@@ -1053,7 +1071,7 @@ static int ensure_fast_buffers(mpm_sim *s)
         void *ptrs[] = {s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan};
         for (void *p : ptrs) if (p) (void)hipFree(p);
         int64_t ntiles = (nmax + RS_TILE - 1) / RS_TILE;
-        int64_t hist_len = 256 * ntiles;
+        int64_t hist_len = 512 * ntiles;     // digits of up to 9 bits
         int64_t scan_words = std::max<int64_t>((hist_len + SCAN_TILE - 1) / SCAN_TILE,
                                                (s->nblocks + SCAN_TILE - 1) / SCAN_TILE) + 16;
         MPM_HIP(hipMalloc(&s->keys, nmax * 4));
@@ -1315,7 +1333,7 @@ int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, 
     if (n > (1ll << 30)) return set_error(MPM_ERR_INVALID, "too many keys");
     if (bits < 1 || bits > 32) return set_error(MPM_ERR_INVALID, "bits out of range [1,32]");
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-    int64_t hist_len = 256 * ntiles;
+    int64_t hist_len = 512 * ntiles;         // digits of up to 9 bits
     int64_t scan_words = (hist_len + SCAN_TILE - 1) / SCAN_TILE + 16;
     uint32_t *k = nullptr, *ka = nullptr, *v = nullptr, *va = nullptr, *hist = nullptr, *scan = nullptr;
     hipStream_t st = nullptr;

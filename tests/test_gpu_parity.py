@@ -57,7 +57,7 @@ def test_base_index_bitexact(gpu):
 
 def test_radix_sort_stable(gpu):
     rng = np.random.default_rng(3)
-    for n, bits in [(1, 8), (1000, 12), (123457, 17), (300000, 24), (70000, 32)]:
+    for n, bits in [(1, 8), (1000, 12), (123457, 17), (200003, 18), (300000, 24), (70000, 32)]:
         keys = rng.integers(0, 2 ** min(bits, 31), size=n, dtype=np.uint64).astype(np.uint32)
         if bits == 32:
             keys = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
@@ -66,6 +66,11 @@ def test_radix_sort_stable(gpu):
         ref = np.argsort(keys, kind="stable")
         assert np.array_equal(perm, ref.astype(np.uint32)), (n, bits)
         assert np.array_equal(srt, keys[ref])
+    # only the low `bits` bits order the keys (9-bit digits: the last one is narrower)
+    keys = rng.integers(0, 2 ** 20, size=50000, dtype=np.uint64).astype(np.uint32)
+    srt, perm = capi.probe_sort(keys, 17)
+    ref = np.argsort(keys & ((1 << 17) - 1), kind="stable")
+    assert np.array_equal(perm, ref.astype(np.uint32))
 
 
 def test_svd_vs_lapack_fixtures(gpu):
