@@ -328,8 +328,9 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
 //      to LDS; both find the base node.  The base, relative to the block's
 //      8^3-node arena (the block's 4 nodes + 1 below + 3 above per axis),
 //      is one of 6^3 bins.
-//   B. both waves sort the chunk by bin (stable counting sort in LDS) and
-//      cut every bin into items of at most ITEM_CAP particles.
+//   B. the tile's waves (FAST: its one wave; FUSED: both) sort the chunk by
+//      bin (stable counting sort in LDS) and cut every bin into items of at
+//      most ITEM_CAP particles.
 //   C. one lane per item: the item's particles share their base node, so the
 //      lane sums their 27 node contributions in registers (FAST reloads each
 //      particle and builds its record in registers; FUSED reads the LDS
@@ -347,7 +348,6 @@ constexpr int TILE_THREADS = 64 * TILE_WAVES;
 constexpr int P2G_THREADS = 64;                  // FAST form: one wave per tile
 static_assert(ARENA_NODES % TILE_THREADS == 0, "arena nodes per thread must be whole");
 constexpr int CHUNK = 384;          // particles per pass over a tile (a full 4^3-cell block holds ~320 at 5 per cell)
-constexpr int A_PASSES = CHUNK / TILE_THREADS;   // phase-A passes of a wave over a chunk
 static_assert(CHUNK % TILE_THREADS == 0, "whole phase-A passes");
 constexpr int P2G_PASSES = CHUNK / P2G_THREADS;  // FAST phase-A passes over a chunk
 constexpr int ITEM_CAP = 4;         // particles per lane item
