@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="oracle steps for the CPU baseline and the FP32-vs-FP64 error")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
+    ap.add_argument("--no-dt-control", action="store_true",
+                    help="skip the long snow run with the fixed dt and with the CFL step-size controller")
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D snow workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -286,6 +288,36 @@ def cpu_baseline(workload, steps):
             "single_thread_value": round(single, 1)}, orc
 
 
+def run_dt_control(steps=600, every=20, max_dt=1e-4, cfl=0.5):
+    """The 1M-particle snow collision run past the point where the fixed dt
+    of its configuration turns unstable (the hardened snow's wave speed
+    outgrows h / dt): fixed dt until the health check fails, then the CFL
+    controller (dt_mode CFL) for `steps` steps.  Reports the first failing
+    check of the fixed run, and for the controlled run the simulated time,
+    health, the dt range and the throughput."""
+    from mpm_cuda_amd import capi, workloads
+    out = {}
+    for name, mode in (("fixed", capi.DT_FIXED), ("cfl", capi.DT_CFL)):
+        sim, info = workloads.snow_spheres(form=capi.FORM_FAST, dt_mode=mode, cfl=cfl)
+        n = sim.total
+        dts, bad, k = [], None, 0
+        t0 = time.perf_counter()
+        while k < steps and bad is None:
+            sim.run(every, max_dt)
+            k += every
+            if sim.health() & ~capi.H_ARENA_SPILL:
+                bad = k
+            dts.append(sim.last_dt)
+        t1 = time.perf_counter()
+        out[name] = {"steps": k, "failed_by_step": bad, "sim_time": round(sim.current_time, 6),
+                     "dt_min": min(dts), "dt_max": max(dts)}
+        if bad is None:
+            out[name]["value"] = round(n * k / (t1 - t0), 1)
+            out[name]["unit"] = "particle-steps/s (health checks every %d steps included)" % every
+        sim.close()
+    return out
+
+
 def run_2d(oracle_steps=400):
     """The reference's 2D snow step (MPMSnow.py) on its 9,000-particle
     workload for its 2000 steps, timed on the device from the initial state
@@ -447,6 +479,8 @@ def main():
             out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
     if not a.no_2d:
         out["snow2d"] = run_2d()
+    if not a.no_dt_control and a.workload == "snow_spheres":
+        out["dt_control"] = run_dt_control()
     if not a.no_batch:
         out["batched"] = run_batch(a.steps, a.warmup)
         out["batched_warm"] = run_batch(a.steps, a.warmup, warm=True)

@@ -147,14 +147,19 @@ __device__ __forceinline__ void gatomic_or(uint32_t *p, uint32_t v)
 __device__ __forceinline__ float step_dt(const GridP &g, float dt) { return g.dtp ? *gp(g.dtp) : dt; }
 
 // Largest v over the wave (all 64 lanes take part; NaN loses), merged into
-// *dst with one atomic max.  Non-negative floats order like their bit
-// patterns as unsigned integers.
+// *dst with an atomic max.  Non-negative floats order like their bit
+// patterns as unsigned integers.  The atomic is skipped when *dst already
+// holds at least v (a plain read first: once a large value is posted, most
+// waves of a launch leave the single contended word alone).
 __device__ __forceinline__ void wave_max_to(uint32_t *dst, float v)
 {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
-    if ((threadIdx.x & 63) == 0 && v > 0.f)
-        (void)__hip_atomic_fetch_max(gp(dst), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((threadIdx.x & 63) == 0 && v > 0.f) {
+        const uint32_t bits = __float_as_uint(v);
+        if (bits > __hip_atomic_load(gp(dst), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            (void)__hip_atomic_fetch_max(gp(dst), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // Workgroups are dispatched to the 8 XCDs of the device round-robin
