@@ -690,7 +690,6 @@ void mpm_destroy(mpm_sim *s)
                     s->staging, s->blk_tile, s->spill, s->dtc};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (s->spill_host) (void)hipHostFree(s->spill_host);
-    for (hipEvent_t e : s->spill_ev) if (e) (void)hipEventDestroy(e);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -17,7 +17,9 @@
 #include "mpm_g2p.h"
 #include "mpm_launch.h"
 
+#include <chrono>
 #include <cstring>
+#include <thread>
 
 namespace mpm {
 
@@ -1182,7 +1184,16 @@ static bool sort_due(mpm_sim *s)
     if (want < s->sorted_at_launch || !s->spill_host) return false;    // counts of the current order only
     const int slot = (int)(want % mpm_sim::SPILL_SLOTS);
     if (s->spill_launch[slot] != want) return false;
-    if (hipEventSynchronize(s->spill_ev[slot]) != hipSuccess) return false;
+    // wait until every group's spill kernel of that launch has published its
+    // count (launch number written after the count with release order)
+    const uint32_t *seq = s->spill_host + mpm_sim::SPILL_SLOTS * 256 + slot * 256;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (size_t k = 0; k < s->groups.size(); ++k) {
+        while (__atomic_load_n(&seq[k], __ATOMIC_ACQUIRE) != (uint32_t)want) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) return false;
+            std::this_thread::yield();
+        }
+    }
     s->spill_launch[slot] = -1;
     uint64_t tot = 0;
     for (size_t k = 0; k < s->groups.size(); ++k) tot += s->spill_host[slot * 256 + k];
@@ -1196,11 +1207,10 @@ static bool sort_due(mpm_sim *s)
 static int spill_host_word(mpm_sim *s, int group, uint32_t **word)
 {
     if (!s->spill_host) {
-        MPM_HIP(hipHostMalloc((void **)&s->spill_host, mpm_sim::SPILL_SLOTS * 256 * sizeof(uint32_t),
-                              hipHostMallocDefault));
-        std::memset(s->spill_host, 0, mpm_sim::SPILL_SLOTS * 256 * sizeof(uint32_t));
-        for (int k = 0; k < mpm_sim::SPILL_SLOTS; ++k)
-            MPM_HIP(hipEventCreateWithFlags(&s->spill_ev[k], hipEventDisableTiming));
+        constexpr size_t words = 2 * mpm_sim::SPILL_SLOTS * 256;
+        MPM_HIP(hipHostMalloc((void **)&s->spill_host, words * sizeof(uint32_t), hipHostMallocCoherent));
+        std::memset(s->spill_host, 0, words / 2 * sizeof(uint32_t));
+        std::memset(s->spill_host + words / 2, 0xff, words / 2 * sizeof(uint32_t));   // no launch yet
     }
     const int slot = (int)(s->tile_launches % mpm_sim::SPILL_SLOTS);
     *word = s->spill_host + slot * 256 + group;
@@ -1208,12 +1218,11 @@ static int spill_host_word(mpm_sim *s, int group, uint32_t **word)
 }
 
 // After group `group`'s tile launch: the launch counter advances after the
-// last group, with an event that marks its spill counts as written.
+// last group.
 static int note_spills(mpm_sim *s, int group)
 {
     const int slot = (int)(s->tile_launches % mpm_sim::SPILL_SLOTS);
     if (group + 1 == (int)s->groups.size()) {
-        MPM_HIP(hipEventRecord(s->spill_ev[slot], s->stream));
         s->spill_launch[slot] = s->tile_launches;
         s->tile_launches += 1;
     }
@@ -1288,7 +1297,7 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags,
-                          hw);
+                          hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }
@@ -1321,7 +1330,7 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "g2p2g_block");
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt_next, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp,
-                          s->flags, hw);
+                          s->flags, hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }

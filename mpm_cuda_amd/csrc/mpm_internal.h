@@ -287,13 +287,14 @@ struct mpm_sim {
     float primed_dt = 0.f;                  // FUSED: dt the primed scatter used for the force term
     bool sorted_valid = false;
     int64_t steps_since_sort = 0;
-    // spill counts of the tile P2G launches, copied to pinned host memory;
-    // a re-sort is brought forward when they grow.  Two slots: the count of
-    // the launch two steps back is read (its copy has finished by then, and
-    // waiting for it keeps at most two steps queued ahead of the device).
+    // spill counts of the tile P2G launches, written to pinned host memory
+    // by the spill kernel with the launch number after them; a re-sort is
+    // brought forward when they grow.  Two slots: the count of the launch two
+    // steps back is read (usually written by then; waiting for its number
+    // keeps at most two steps queued ahead of the device, and no event
+    // packet sits in the stream).
     static constexpr int SPILL_SLOTS = 2;
-    uint32_t *spill_host = nullptr;         // [slot][256 groups]
-    hipEvent_t spill_ev[SPILL_SLOTS] = {};
+    uint32_t *spill_host = nullptr;         // [slot][256 groups] counts, then [slot][256 groups] launch numbers
     int64_t spill_launch[SPILL_SLOTS] = {-1, -1};   // tile launch number held by the slot
     int64_t tile_launches = 0;
     int64_t sorted_at_launch = 0;           // tile_launches when the last sort ran

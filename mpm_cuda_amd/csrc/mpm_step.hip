@@ -282,15 +282,20 @@ __device__ __forceinline__ void spill_particle(const GridP &g, const MatP &mat, 
     mark_blocks(g, s, flags);
 }
 
-// host_count (pinned host memory, or null) receives the spill count.
+// host_count (pinned host memory, or null) receives the spill count, then
+// host_seq the tile launch number `seq` (a release store: a host that reads
+// seq == its launch also reads that launch's count).
 template <int MAT>
 __global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView pv, float dt,
                                                    const uint32_t *__restrict__ list, uint32_t *counters,
                                                    float4 *grid, uint32_t *flags, uint32_t list_cap,
-                                                   uint32_t *host_count)
+                                                   uint32_t *host_count, uint32_t *host_seq, uint32_t seq)
 {
     const uint32_t listed = counters[CT_SPILL];
-    if (host_count && blockIdx.x == 0 && threadIdx.x == 0) *host_count = listed;
+    if (host_count && blockIdx.x == 0 && threadIdx.x == 0) {
+        *host_count = listed;
+        if (host_seq) __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     const uint32_t n = min(listed, list_cap);
     dt = step_dt(g, dt);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -464,14 +469,14 @@ int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv
 
 int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                      const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
-                     uint32_t *host_count)
+                     uint32_t *host_count, uint32_t *host_seq, uint32_t seq)
 {
     if (pv.n <= 0) return 0;
     dim3 gr(64), bl(256);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_spill<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_spill<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count); break;
-    default: hipLaunchKernelGGL(k_p2g_spill<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_spill<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_spill<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq); break;
+    default: hipLaunchKernelGGL(k_p2g_spill<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq); break;
     }
     return (int)hipGetLastError();
 }
