@@ -572,7 +572,7 @@ int mpm_stats(mpm_sim *s, double st[8])
     st[4] = (double)s->sorts;
     st[5] = mpm_current_time(s);
     st[6] = c[CT_TILES];
-    st[7] = c[CT_SPILL];
+    st[7] = s->cfg.form == MPM_FORM_FUSED ? c[CT_SPILL] : c[CT_SPILLED];
     return MPM_OK;
 }
 
@@ -740,8 +740,14 @@ static int run_phase(mpm_sim *s, int phase, float dt)
     int rc = MPM_OK;
     s->last_phase = phase;
     if (phase == MPM_PHASE_P2G) {
-        rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 3), "zero");
-        if (rc) return rc;
+        // FAST form: the last block compaction cleared the spill counter and
+        // this P2G's spill kernels clear the active-block and node counters
+        const bool clean = s->cfg.form == MPM_FORM_FAST && s->counters_clean && s->total > 0;
+        s->counters_clean = false;
+        if (!clean) {
+            rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 3), "zero");
+            if (rc) return rc;
+        }
         if (fast) {
             {
                 PhaseTimer t(s, 0);
@@ -764,6 +770,7 @@ static int run_phase(mpm_sim *s, int phase, float dt)
             PhaseTimer t(s, 2);
             rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
             if (rc) return rc;
+            s->counters_clean = true;
             const bool staged = fast && s->blk_tile;
             PhaseTimer tk(s, TK_GRID);
             rc = check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,

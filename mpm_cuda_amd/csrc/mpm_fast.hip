@@ -1297,7 +1297,7 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags,
-                          hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches);
+                          hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches, 1);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }

@@ -75,7 +75,7 @@ enum Counter : int {
     CT_FALLBACK = 4,    // FP64 SVD fallbacks
     CT_REJECT = 5,      // warm-start rejections
     CT_TILES = 6,       // P2G tiles (FAST / FUSED forms)
-    CT_SPILLED = 7,     // reserved
+    CT_SPILLED = 7,     // spill count of the last block compaction (CT_SPILL is cleared there)
     CT_SPEED = 8,       // step-size controller: max_p |v_p| + c_p (float bits), since the last control
     CT_DEFORM = 9,      // step-size controller: max_p max(J, 1/J) (float bits)
     CT_COUNT = 16
@@ -297,6 +297,7 @@ struct mpm_sim {
     uint32_t *spill_host = nullptr;         // [slot][256 groups] counts, then [slot][256 groups] launch numbers
     int64_t spill_launch[SPILL_SLOTS] = {-1, -1};   // tile launch number held by the slot
     int64_t tile_launches = 0;
+    bool counters_clean = false;            // CT_SPILL cleared by the last block compaction (FAST form)
     int64_t sorted_at_launch = 0;           // tile_launches when the last sort ran
     int64_t sort_interval = 0;              // learned interval: spills ended the last one after this many steps
     int last_phase = -1;

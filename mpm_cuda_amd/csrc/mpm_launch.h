@@ -19,7 +19,8 @@ int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv
 int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);
 int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                      const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
-                     uint32_t *host_count, uint32_t *host_seq = nullptr, uint32_t seq = 0);
+                     uint32_t *host_count, uint32_t *host_seq = nullptr, uint32_t seq = 0,
+                     int clear_active = 0);
 int launch_dt_scan(hipStream_t st, const MatP &m, const PartView &pv, uint32_t *counters);
 int launch_dt_control(hipStream_t st, DtCtl *c, uint32_t *counters, float max_dt, float cfl_h, int mode);
 

@@ -64,11 +64,17 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(GridP g, MatP mat, PartView 
 // ---------------------------------------------------------------------------
 // Active-block list: compacts the block flags written by P2G (and clears them).
 // One global atomic per workgroup (256 flags): wave counts are combined in LDS.
-// Thread t of the workgroup handles flag b0 + t.
+// Thread t of the workgroup handles flag b0 + t.  The workgroup of flag 0
+// also moves the step's spill count (consumed by the spill kernel, which ran
+// before) to CT_SPILLED and clears CT_SPILL for the next P2G.
 __device__ __forceinline__ void compact_flags(uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters,
                                               int b0)
 {
     __shared__ uint32_t wave_cnt[4], wg_base;
+    if (b0 == 0 && threadIdx.x == 0) {
+        gp(counters)[CT_SPILLED] = gp(counters)[CT_SPILL];
+        gp(counters)[CT_SPILL] = 0u;
+    }
     int b = b0 + (int)threadIdx.x;
     bool f = b < nblocks && gp(flags)[b] != 0u;
     if (f) gp(flags)[b] = 0u;
@@ -284,17 +290,27 @@ __device__ __forceinline__ void spill_particle(const GridP &g, const MatP &mat, 
 
 // host_count (pinned host memory, or null) receives the spill count, then
 // host_seq the tile launch number `seq` (a release store: a host that reads
-// seq == its launch also reads that launch's count).
+// seq == its launch also reads that launch's count).  With clear_active the
+// kernel also clears the active-block and node counters for the block
+// compaction that follows (no kernel between P2G and it touches them), which
+// saves the FAST form its step-start counter reset.
 template <int MAT>
 __global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView pv, float dt,
                                                    const uint32_t *__restrict__ list, uint32_t *counters,
                                                    float4 *grid, uint32_t *flags, uint32_t list_cap,
-                                                   uint32_t *host_count, uint32_t *host_seq, uint32_t seq)
+                                                   uint32_t *host_count, uint32_t *host_seq, uint32_t seq,
+                                                   int clear_active)
 {
     const uint32_t listed = counters[CT_SPILL];
-    if (host_count && blockIdx.x == 0 && threadIdx.x == 0) {
-        *host_count = listed;
-        if (host_seq) __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (host_count) {
+            *host_count = listed;
+            if (host_seq) __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        if (clear_active) {
+            gp(counters)[CT_ACTIVE] = 0u;
+            gp(counters)[CT_NODES] = 0u;
+        }
     }
     const uint32_t n = min(listed, list_cap);
     dt = step_dt(g, dt);
@@ -469,14 +485,14 @@ int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv
 
 int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                      const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
-                     uint32_t *host_count, uint32_t *host_seq, uint32_t seq)
+                     uint32_t *host_count, uint32_t *host_seq, uint32_t seq, int clear_active)
 {
     if (pv.n <= 0) return 0;
     dim3 gr(64), bl(256);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_spill<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_spill<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq); break;
-    default: hipLaunchKernelGGL(k_p2g_spill<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_spill<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq, clear_active); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_spill<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq, clear_active); break;
+    default: hipLaunchKernelGGL(k_p2g_spill<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, list, counters, grid, flags, list_cap, host_count, host_seq, seq, clear_active); break;
     }
     return (int)hipGetLastError();
 }
