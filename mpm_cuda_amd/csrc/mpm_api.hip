@@ -744,6 +744,7 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         // this P2G's spill kernels clear the active-block and node counters
         const bool clean = s->cfg.form == MPM_FORM_FAST && s->counters_clean && s->total > 0;
         s->counters_clean = false;
+        if ((rc = flush_spill(s))) return rc;
         if (!clean) {
             rc = check_launch(launch_zero_u32(s->stream, s->counters + CT_ACTIVE, 3), "zero");
             if (rc) return rc;
@@ -768,7 +769,10 @@ static int run_phase(mpm_sim *s, int phase, float dt)
     if (phase == MPM_PHASE_GRID) {
         {
             PhaseTimer t(s, 2);
-            rc = check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters), "compact");
+            const bool fused = s->spill_pending;
+            rc = fused ? spill_compact(s)
+                       : check_launch(launch_compact(s->stream, s->flags, (int)s->nblocks, s->list, s->counters),
+                                      "compact");
             if (rc) return rc;
             s->counters_clean = true;
             const bool staged = fast && s->blk_tile;
@@ -776,7 +780,7 @@ static int run_phase(mpm_sim *s, int phase, float dt)
             rc = check_launch(launch_grid_update(s->stream, s->g, dt, s->list, s->counters, s->grid_mp, s->grid_v,
                                                  (int)s->nblocks, staged ? s->staging : nullptr,
                                                  staged ? s->blk_tile : nullptr, (int)s->groups.size(),
-                                                 (uint32_t)s->staging_cap),
+                                                 (uint32_t)s->staging_cap, fused ? 1 : 0),
                               "grid_update");
             if (rc) return rc;
         }

@@ -298,6 +298,7 @@ int prepare(mpm_batch *b)
     for (size_t k = 0; k < n; ++k) {
         mpm_sim *s = b->sims[k];
         s->counters_clean = false;     // batch steps reset the counters themselves
+        if (int rc = flush_spill(s)) return rc;
         if (parity_of(b, k) < 0) return set_error(MPM_ERR_INVALID, "a scene's particle storage changed since mpm_batch_create");
         if (s->groups.size() != 1) return set_error(MPM_ERR_INVALID, "a scene of the batch gained a particle group");
         int rc = fast_ready(s);

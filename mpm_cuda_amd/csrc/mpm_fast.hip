@@ -425,10 +425,12 @@ __device__ __forceinline__ TileGeo tile_geo(const GridP &g, uint32_t blk)
 }
 
 // Phase A tail: the particle's bin key, or NO_BIN and an entry in the list
-// of the atomic fallback.  Returns true when the particle was listed.
+// of the atomic fallback (whose node blocks are flagged here when `flags` is
+// given).  Returns true when the particle was listed.
 template <class TL>
 __device__ __forceinline__ bool bin_key(const GridP &g, TL &L, const TileGeo &t, bool active, int slot, int i,
-                                        const Stencil &s, bool staged, uint32_t *spill, uint32_t *counters)
+                                        const Stencil &s, bool staged, uint32_t *spill, uint32_t *counters,
+                                        uint32_t *flags)
 {
     const int lane = threadIdx.x & 63;
     const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
@@ -442,7 +444,10 @@ __device__ __forceinline__ bool bin_key(const GridP &g, TL &L, const TileGeo &t,
         if (lane == leader) base = gatomic_add(counters + CT_SPILL, (uint32_t)__popcll(sm));
         base = __shfl(base, leader);
         const uint64_t below = lane ? (sm & ((~0ull) >> (64 - lane))) : 0ull;
-        if (spill_me) gp(spill)[base + __popcll(below)] = (uint32_t)i;
+        if (spill_me) {
+            gp(spill)[base + __popcll(below)] = (uint32_t)i;
+            if (flags) mark_blocks(g, s, flags);
+        }
     }
     if (active) L.key[slot] = inside ? (uint8_t)(lx + 6 * (ly + 6 * lz)) : (uint8_t)NO_BIN;
     return spill_me;
@@ -472,7 +477,7 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
                                              int i, const Stencil &s, const Scatter &sc, bool staged,
                                              uint32_t *spill, uint32_t *counters)
 {
-    const bool spill_me = bin_key(g, L, t, active, slot, i, s, staged, spill, counters);
+    const bool spill_me = bin_key(g, L, t, active, slot, i, s, staged, spill, counters, nullptr);
     if (active && !spill_me) {
         float q0[3], hB[9], Gs[6];
         make_record<MAT>(g, s, sc, q0, hB, Gs);
@@ -889,7 +894,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
             Stencil s;
             s.b[0] = s.b[1] = s.b[2] = 0;
             if (active) bits |= make_stencil(g, xs[it], s);
-            bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, s, staged, spill, counters);
+            bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, s, staged, spill, counters, flags);
         }
         __asm__ volatile("" ::: "memory");
         // B and C
@@ -921,10 +926,16 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
                                                             const uint32_t *__restrict__ tile_count, uint32_t desc_cap,
                                                             uint32_t staging_cap, float dt,
                                                             float4 *__restrict__ staging, uint32_t *spill,
-                                                            uint32_t *flags, uint32_t *counters)
+                                                            uint32_t *flags, uint32_t *counters, int clear_active)
 {
     __shared__ TileKeys L;
     const uint32_t nt = min(*tile_count, desc_cap);
+    // the active-block and node counters are not used during P2G: clear
+    // them here for the block compaction that follows
+    if (clear_active && blockIdx.x == 0 && threadIdx.x == 0) {
+        gp(counters)[CT_ACTIVE] = 0u;
+        gp(counters)[CT_NODES] = 0u;
+    }
     uint32_t bits = 0;
     const uint32_t nt8 = round_xcd(nt);
     dt = step_dt(g, dt);
@@ -1289,17 +1300,49 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     {
     PhaseTimer tk(s, TK_P2G);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
-    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters, 1); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters, 1); break;
+    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters, 1); break;
     }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
+    if (s->groups.size() == 1) {
+        // one group: the listed particles are scattered by the block
+        // compaction of the grid phase (the tiles flagged their blocks)
+        s->spill_pending = true;
+        s->spill_dt = dt;
+        s->spill_hw = hw;
+        s->spill_seq = (uint32_t)s->tile_launches;
+        return note_spills(s, group);
+    }
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags,
-                          hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches, 1);
+                          hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
+}
+
+int spill_compact(mpm_sim *s)
+{
+    if (!s->spill_pending) return hip_fail(hipErrorInvalidValue, "spill_compact");
+    s->spill_pending = false;
+    const PartView pv = view_of(s, 0);
+    const int rc = launch_spill_compact(s->stream, s->g, s->groups[0].mat, pv, s->spill_dt, s->spill,
+                                        (uint32_t)pv.n, s->counters, s->grid_mp, s->flags, (int)s->nblocks,
+                                        s->list, s->spill_hw, s->spill_hw + mpm_sim::SPILL_SLOTS * 256,
+                                        s->spill_seq);
+    return rc ? hip_fail((hipError_t)rc, "spill_compact") : MPM_OK;
+}
+
+int flush_spill(mpm_sim *s)
+{
+    if (!s->spill_pending) return MPM_OK;
+    s->spill_pending = false;
+    const PartView pv = view_of(s, 0);
+    const int rc = launch_p2g_spill(s->stream, s->g, s->groups[0].mat, pv, s->spill_dt, s->spill, (uint32_t)pv.n,
+                                    s->counters, s->grid_mp, s->flags, s->spill_hw,
+                                    s->spill_hw + mpm_sim::SPILL_SLOTS * 256, s->spill_seq);
+    return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;
 }
 
 int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)

@@ -297,7 +297,12 @@ struct mpm_sim {
     uint32_t *spill_host = nullptr;         // [slot][256 groups] counts, then [slot][256 groups] launch numbers
     int64_t spill_launch[SPILL_SLOTS] = {-1, -1};   // tile launch number held by the slot
     int64_t tile_launches = 0;
-    bool counters_clean = false;            // CT_SPILL cleared by the last block compaction (FAST form)
+    bool counters_clean = false;            // CT_SPILL cleared by the last grid phase (FAST form)
+    // one-group tile P2G: its spill scatter waits for the grid phase
+    bool spill_pending = false;
+    float spill_dt = 0.f;
+    uint32_t *spill_hw = nullptr;
+    uint32_t spill_seq = 0;
     int64_t sorted_at_launch = 0;           // tile_launches when the last sort ran
     int64_t sort_interval = 0;              // learned interval: spills ended the last one after this many steps
     int last_phase = -1;

@@ -25,6 +25,18 @@ struct Stencil {
     float w[3][3], dw[3][3];
 };
 
+// Flag the (up to 2x2x2) 4^3-node blocks a particle's 3x3x3 stencil touches.
+__device__ __forceinline__ void mark_blocks(const GridP &g, const Stencil &s, uint32_t *flags)
+{
+    int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2;
+    int by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
+    int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
+    for (int bz = bz0; bz <= bz1; ++bz)
+        for (int by = by0; by <= by1; ++by)
+            for (int bx = bx0; bx <= bx1; ++bx)
+                gp(flags)[((int64_t)bz * g.nby + by) * g.nbx + bx] = 1u;
+}
+
 __device__ __forceinline__ uint32_t make_stencil(const GridP &g, const float x[3], Stencil &s)
 {
     uint32_t bits = 0;

@@ -13,7 +13,7 @@ int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list,
 int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters, const float4 *gv);
 int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
                        uint32_t *counters, float4 *gmp, float4 *gv, int nblocks, const float4 *staging,
-                       const int32_t *blk_tile, int ngroups, uint32_t staging_cap);
+                       const int32_t *blk_tile, int ngroups, uint32_t staging_cap, int clear_spill = 0);
 int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads);
 int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);
@@ -21,12 +21,18 @@ int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartVi
                      const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
                      uint32_t *host_count, uint32_t *host_seq = nullptr, uint32_t seq = 0,
                      int clear_active = 0);
+int launch_spill_compact(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+                         const uint32_t *spill, uint32_t spill_cap, uint32_t *counters, float4 *grid,
+                         uint32_t *flags, int nblocks, uint32_t *list, uint32_t *host_count, uint32_t *host_seq,
+                         uint32_t seq);
 int launch_dt_scan(hipStream_t st, const MatP &m, const PartView &pv, uint32_t *counters);
 int launch_dt_control(hipStream_t st, DtCtl *c, uint32_t *counters, float max_dt, float cfl_h, int mode);
 
 // FAST form (mpm_fast.hip)
 int fast_prepare(mpm_sim *s);                 // (re)sort particles and build P2G tiles when due
 int launch_p2g_tiles(mpm_sim *s, int group, float dt);
+int spill_compact(mpm_sim *s);                // grid phase after a one-group tile P2G: spill scatter + compaction
+int flush_spill(mpm_sim *s);                  // spill scatter of a tile P2G whose grid phase did not run
 int launch_g2p_fast(mpm_sim *s, int group, float dt);
 int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next);   // FUSED form
 int sort_all(mpm_sim *s);                     // sort every group now and rebuild the tiles

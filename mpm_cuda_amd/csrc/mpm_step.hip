@@ -17,17 +17,6 @@ __global__ void k_zero_u32(uint32_t *p, int n)
     if (i < n) p[i] = 0u;
 }
 
-__device__ __forceinline__ void mark_blocks(const GridP &g, const Stencil &s, uint32_t *flags)
-{
-    int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2;
-    int by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
-    int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
-    for (int bz = bz0; bz <= bz1; ++bz)
-        for (int by = by0; by <= by1; ++by)
-            for (int bx = bx0; bx <= bx1; ++bx)
-                gp(flags)[((int64_t)bz * g.nby + by) * g.nbx + bx] = 1u;
-}
-
 // ---------------------------------------------------------------------------
 // PLAIN form P2G: one thread per particle, 27 nodes x 4 FP32 global atomics,
 // the stress SVD evaluated here (MPMCUDA.cu:310-433).
@@ -66,12 +55,13 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(GridP g, MatP mat, PartView 
 // One global atomic per workgroup (256 flags): wave counts are combined in LDS.
 // Thread t of the workgroup handles flag b0 + t.  The workgroup of flag 0
 // also moves the step's spill count (consumed by the spill kernel, which ran
-// before) to CT_SPILLED and clears CT_SPILL for the next P2G.
+// before) to CT_SPILLED and clears CT_SPILL for the next P2G, when
+// clear_spill is set (else the grid update does it).
 __device__ __forceinline__ void compact_flags(uint32_t *flags, int nblocks, uint32_t *list, uint32_t *counters,
-                                              int b0)
+                                              int b0, bool clear_spill)
 {
     __shared__ uint32_t wave_cnt[4], wg_base;
-    if (b0 == 0 && threadIdx.x == 0) {
+    if (clear_spill && b0 == 0 && threadIdx.x == 0) {
         gp(counters)[CT_SPILLED] = gp(counters)[CT_SPILL];
         gp(counters)[CT_SPILL] = 0u;
     }
@@ -96,7 +86,7 @@ __device__ __forceinline__ void compact_flags(uint32_t *flags, int nblocks, uint
 __global__ void __launch_bounds__(256) k_compact_blocks(uint32_t *flags, int nblocks, uint32_t *list,
                                                         uint32_t *counters)
 {
-    compact_flags(flags, nblocks, list, counters, (int)(blockIdx.x * blockDim.x));
+    compact_flags(flags, nblocks, list, counters, (int)(blockIdx.x * blockDim.x), true);
 }
 
 // ---------------------------------------------------------------------------
@@ -174,10 +164,14 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
 __global__ void __launch_bounds__(64) k_grid_update(GridP g, float dt, const uint32_t *list, uint32_t *counters,
                                                     float4 *gmp, float4 *gv, const float4 *__restrict__ staging,
                                                     const int32_t *__restrict__ blk_tile, int ngroups,
-                                                    uint32_t staging_cap, int64_t nblocks)
+                                                    uint32_t staging_cap, int64_t nblocks, int clear_spill)
 {
     const uint32_t nact = counters[CT_ACTIVE];
     const uint32_t nact8 = round_xcd(nact);
+    if (clear_spill && blockIdx.x == 0 && threadIdx.x == 0) {     // see compact_flags
+        gp(counters)[CT_SPILLED] = gp(counters)[CT_SPILL];
+        gp(counters)[CT_SPILL] = 0u;
+    }
     dt = step_dt(g, dt);
     for (uint32_t k8 = blockIdx.x; k8 < nact8; k8 += gridDim.x) {
         const uint32_t k = xcd_spread(k8, nact8);
@@ -285,7 +279,7 @@ __device__ __forceinline__ void spill_particle(const GridP &g, const MatP &mat, 
                 gatomic_add(dst + 2, c.z);
                 gatomic_add(dst + 3, c.w);
             }
-    mark_blocks(g, s, flags);
+    if (flags) mark_blocks(g, s, flags);
 }
 
 // host_count (pinned host memory, or null) receives the spill count, then
@@ -319,6 +313,36 @@ __global__ void __launch_bounds__(256) k_p2g_spill(GridP g, MatP mat, PartView p
         if (i < 0 || i >= pv.n) continue;
         spill_particle<MAT>(g, mat, pv, i, dt, counters, grid, flags);
     }
+}
+
+// Spill scatter and block compaction in one launch (FAST form, one particle
+// group): the tile P2G already flagged the blocks of the particles it listed,
+// so the workgroups scatter their share of the list with atomics (into
+// grid_mp, read only by the grid update after this kernel) and compact their
+// 256 block flags without waiting on each other.  CT_SPILL is read by every
+// workgroup here; the grid update clears it.
+template <int MAT>
+__global__ void __launch_bounds__(256) k_spill_compact(GridP g, MatP mat, PartView pv, float dt,
+                                                       const uint32_t *__restrict__ spill, uint32_t spill_cap,
+                                                       uint32_t *counters, float4 *grid, uint32_t *flags,
+                                                       int nblocks, uint32_t *list, uint32_t *host_count,
+                                                       uint32_t *host_seq, uint32_t seq)
+{
+    const uint32_t listed = counters[CT_SPILL];
+    if (host_count && blockIdx.x == 0 && threadIdx.x == 0) {
+        *host_count = listed;
+        __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    const uint32_t n = min(listed, spill_cap);
+    if (n) {
+        dt = step_dt(g, dt);
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+            const int i = (int)spill[k];
+            if (i < 0 || i >= pv.n) continue;
+            spill_particle<MAT>(g, mat, pv, i, dt, counters, grid, nullptr);
+        }
+    }
+    compact_flags(flags, nblocks, list, counters, (int)(blockIdx.x * blockDim.x), false);
 }
 
 // Kirchhoff stress of the current state, for the first P2G of the FAST form
@@ -388,7 +412,7 @@ __global__ void __launch_bounds__(256) k_compact_batch(const BatchSlice *__restr
     if (blockIdx.x >= sc.total) return;
     const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(blockIdx.x));
     const BatchSlice &b = tab[sid];
-    compact_flags(b.flags, (int)b.nblocks, b.list, b.counters, (int)((blockIdx.x - sc.start(sid)) * 256u));
+    compact_flags(b.flags, (int)b.nblocks, b.list, b.counters, (int)((blockIdx.x - sc.start(sid)) * 256u), true);
 }
 
 // Grid update over the active blocks of every scene (one wave per block).
@@ -461,12 +485,12 @@ int launch_compact(hipStream_t st, uint32_t *flags, int nblocks, uint32_t *list,
 
 int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
                        uint32_t *counters, float4 *gmp, float4 *gv, int nblocks, const float4 *staging,
-                       const int32_t *blk_tile, int ngroups, uint32_t staging_cap)
+                       const int32_t *blk_tile, int ngroups, uint32_t staging_cap, int clear_spill)
 {
     int wg = nblocks < 8192 ? nblocks : 8192;
     if (wg < 1) wg = 1;
     hipLaunchKernelGGL(k_grid_update, dim3(wg), dim3(64), 0, st, g, dt, list, counters, gmp, gv, staging, blk_tile,
-                       ngroups, staging_cap, (int64_t)nblocks);
+                       ngroups, staging_cap, (int64_t)nblocks, clear_spill);
     return (int)hipGetLastError();
 }
 
@@ -588,6 +612,21 @@ __global__ void __launch_bounds__(64) k_dt_control(DtCtl *c, uint32_t *counters,
     gp(c)->time = gp(c)->time + (double)dt;
     gp(counters)[CT_SPEED] = 0u;
     gp(counters)[CT_DEFORM] = 0u;
+}
+
+int launch_spill_compact(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
+                         const uint32_t *spill, uint32_t spill_cap, uint32_t *counters, float4 *grid,
+                         uint32_t *flags, int nblocks, uint32_t *list, uint32_t *host_count, uint32_t *host_seq,
+                         uint32_t seq)
+{
+    if (!host_count || !host_seq) return (int)hipErrorInvalidValue;
+    dim3 gr(nblk(nblocks, 256)), bl(256);
+    switch (m.type) {
+    case MPM_JELLO: hipLaunchKernelGGL(k_spill_compact<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, spill, spill_cap, counters, grid, flags, nblocks, list, host_count, host_seq, seq); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_spill_compact<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, spill, spill_cap, counters, grid, flags, nblocks, list, host_count, host_seq, seq); break;
+    default: hipLaunchKernelGGL(k_spill_compact<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, spill, spill_cap, counters, grid, flags, nblocks, list, host_count, host_seq, seq); break;
+    }
+    return (int)hipGetLastError();
 }
 
 int launch_dt_scan(hipStream_t st, const MatP &m, const PartView &pv, uint32_t *counters)
