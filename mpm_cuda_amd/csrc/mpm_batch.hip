@@ -26,7 +26,7 @@ using namespace mpm;
 
 namespace {
 
-constexpr int SPILL_WGS = 128;
+constexpr int SPILL_WGS = 512;
 constexpr int NCAND = 4;
 const int P2G_CAND[NCAND] = {1024, 2048, 4096, 8192};   // one-wave workgroups
 const int GRID_CAND[NCAND] = {1024, 2048, 4096, 8192};

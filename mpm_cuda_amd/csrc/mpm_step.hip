@@ -388,11 +388,16 @@ __device__ __forceinline__ void spill_scene(const BatchSlice &b, uint32_t n, uin
     }
 }
 
-// Spilled particles of every scene, scene after scene.
+// Spilled particles of every scene: the workgroups form groups of `per`,
+// group q takes scenes q, q + groups, ... (the scenes' counter loads then
+// overlap instead of following one another).
 __global__ void __launch_bounds__(256) k_spill_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    for (int sid = 0; sid < nslices; ++sid) {
+    const uint32_t per = max(1u, gridDim.x / (uint32_t)nslices);
+    const uint32_t groups = gridDim.x / per;
+    if (blockIdx.x / per >= groups) return;
+    const uint32_t tid = (blockIdx.x % per) * blockDim.x + threadIdx.x, stride = per * blockDim.x;
+    for (uint32_t sid = blockIdx.x / per; sid < (uint32_t)nslices; sid += groups) {
         const BatchSlice &b = tab[sid];
         const uint32_t n = min(gp(b.counters)[CT_SPILL], (uint32_t)b.pv.n);
         if (n == 0u) continue;
