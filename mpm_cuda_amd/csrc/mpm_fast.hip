@@ -277,7 +277,11 @@ __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, i
                           const uint32_t *__restrict__ perm, const uint32_t *__restrict__ pid,
                           uint32_t *__restrict__ pida, uint64_t plane_mask)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware: a re-sort's permutation is close to the identity, so an
+    // XCD that writes a contiguous eighth of the positions reads its sources
+    // from about the same eighth, and each source line is fetched into one L2
+    const uint32_t nb8 = gridDim.x;
+    const int j = (int)(xcd_spread(blockIdx.x, nb8) * blockDim.x + threadIdx.x);
     if (j >= n) return;
     uint32_t src = perm[j];
     if (src >= (uint32_t)n) src = (uint32_t)j;    // never taken for a valid permutation
@@ -1148,7 +1152,7 @@ int sort_all(mpm_sim *s)
             return hip_fail((hipError_t)rc, "radix_sort");
         const uint32_t *perm = in_alt ? s->vals_alt : s->vals;
         const uint32_t *skeys = in_alt ? s->keys_alt : s->keys;
-        hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, s->P, s->P_alt, s->cap, pv.off, n, perm,
+        hipLaunchKernelGGL(k_permute, dim3(round_xcd((uint32_t)nblk(n, 256))), dim3(256), 0, st, s->P, s->P_alt, s->cap, pv.off, n, perm,
                            s->pid, s->pid_alt, plane_mask);
         // tiles: one per non-empty block, numbered in block order
         hipLaunchKernelGGL(k_block_runs, dim3(nblk(n, 256)), dim3(256), 0, st, skeys, n, s->blk_start, blk_end);
