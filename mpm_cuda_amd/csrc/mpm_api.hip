@@ -250,6 +250,7 @@ int mpm_add_particles(mpm_sim *s, const mpm_material *mat, int64_t n, const floa
     int rc = validate_material(mat);
     if (rc) return rc;
     if (n <= 0 || !xyz) return set_error(MPM_ERR_INVALID, "no particles");
+    if ((rc = flush_spill(s))) return rc;     // a P2G phase's spill scatter reads the state it saw
     if (s->total + n > (1ll << 31) - 1) return set_error(MPM_ERR_INVALID, "too many particles");
     // every particle's stencil must lie inside the grid (MPMSimulator.cpp:210-222)
     const GridP &g = s->g;
@@ -437,6 +438,7 @@ int mpm_write_particles(mpm_sim *s, int32_t group, int32_t field, const float *i
     if (field == MPM_P_TAU) return set_error(MPM_ERR_INVALID, "TAU is derived state and read-only");
     int w = width_of(field), p0 = plane_of(field);
     if (w < 0) return set_error(MPM_ERR_INVALID, "unknown particle field");
+    if (int rc = flush_spill(s)) return rc;   // a P2G phase's spill scatter reads the state it saw
     const GroupInfo &gi = s->groups[group];
     int64_t n = gi.n;
     if (field == MPM_P_POS) {
@@ -486,6 +488,7 @@ int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
 {
     if (!s || !out) return set_error(MPM_ERR_INVALID, "null argument");
     if (field < MPM_G_MASS || field > MPM_G_VELOCITY) return set_error(MPM_ERR_INVALID, "unknown grid field");
+    if (int rc = flush_spill(s)) return rc;   // after a P2G phase: its spilled particles' share
     const GridP &g = s->g;
     size_t nslots = (size_t)s->nblocks * BLK_NODES;
     std::vector<float4> mp(nslots), gv(nslots);
