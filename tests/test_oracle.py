@@ -203,6 +203,81 @@ def test_p2g_force_is_minus_volume_stress_gradient():
     assert np.allclose(M, -dt * h ** 3 * tau, rtol=1e-9, atol=1e-18)
 
 
+def _first_moment(sim, xp):
+    """sum_i p_i (x_i - x_p)^T over the nodes with mass, and sum_i p_i."""
+    h = sim.h
+    p = sim.grid_vec()
+    z, y, xx = np.nonzero(sim.grid_mass())
+    M = np.zeros((3, 3))
+    for a, b, c in zip(z, y, xx):
+        M += np.outer(p[a, b, c], np.array([c, b, a]) * h - xp)
+    return M, p.reshape(-1, 3).sum(0)
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+@pytest.mark.parametrize("Jp", [0.8, 1.0, 1.1])
+def test_p2g_snow_force_carries_the_hardening(Jp):
+    """Snow P2G (MPMCUDA.cu:342-349): with v = C = 0 the node momenta are
+    dt f_i, f_i = -V h tau grad w_i, h = exp(xi (1 - det F_p)).  For
+    F_e = s R (R a rotation) the fixed-corotated stress is isotropic,
+    tau = [2 mu (s - 1) s + lam s^3 (s^3 - 1)] I, and
+    sum_i grad w_i (x_i - x_p)^T = I, so the first moment is
+    -dt V exp(xi (1 - J_p)) [2 mu (s-1) s + lam s^3 (s^3-1)] I.
+    Lame constants from (E, nu) by the textbook formulas; F_p is a
+    rotated non-diagonal factor with det J_p.  A flipped or inverted
+    hardening exponent changes the result by exp(+-2 xi (1 - J_p))."""
+    rng = np.random.default_rng(21)
+    n_grid, h = 32, 1.0 / 32
+    E, nu, xi, s = 1.4e5, 0.2, 10.0, 0.97
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    x = np.array([[0.503, 0.491, 0.517]])
+    R = _rotation(rng)
+    Q = _rotation(rng)
+    Fp = Q @ np.diag([Jp, 1.0, 1.0]) @ Q.T @ _rotation(rng)   # det = Jp, not symmetric
+    assert abs(np.linalg.det(Fp) - Jp) < 1e-12
+    g = O.Group(x, np.zeros(3), type=O.SNOW, mass=1.0, vol=h ** 3, E=E, nu=nu, xi=xi,
+                Fe=(s * R)[None], Fp=Fp[None])
+    sim = O.Sim(n_grid, n_grid, n_grid, h, [g])
+    sim.clear_grid()
+    dt = 1e-4
+    assert sim.p2g(dt) == 0
+    M, P = _first_moment(sim, x[0])
+    tau_iso = 2.0 * mu * (s - 1.0) * s + lam * s ** 3 * (s ** 3 - 1.0)
+    want = -dt * h ** 3 * np.exp(xi * (1.0 - Jp)) * tau_iso
+    assert np.abs(P).max() < 1e-12 * abs(want) / h
+    assert np.allclose(M, want * np.eye(3), rtol=1e-9, atol=1e-12 * abs(want))
+    # compressed snow (s < 1) pushes its nodes outward: positive first moment
+    assert want > 0
+
+
+@pytest.mark.parametrize("J,pk", [(0.5, 127.0), (1.25, 0.2097152 - 1.0), (1.0, 0.0)])
+def test_p2g_water_pressure_force(J, pk):
+    """Water P2G (MPMCUDA.cu:351-354, 412-414): p = k (J^-gamma - 1),
+    f_i = +V p J grad w_i, so with v = C = 0 the first moment of the node
+    momenta is +dt V p J I.  gamma = 7: J = 1/2 gives J^-7 - 1 = 127 and
+    J = 5/4 gives 0.8^7 - 1 = 0.2097152 - 1 (both exact).  Compressed water
+    (J < 1) pushes outward (positive moment), stretched water pulls in."""
+    n_grid, h = 32, 1.0 / 32
+    k, dt = 1e4, 1e-4
+    x = np.array([[0.488, 0.509, 0.521]])
+    g = O.Group(x, np.zeros(3), type=O.WATER, mass=1.0, vol=h ** 3, eos_k=k, eos_gamma=7.0, J=np.array([J]))
+    sim = O.Sim(n_grid, n_grid, n_grid, h, [g])
+    sim.clear_grid()
+    assert sim.p2g(dt) == 0
+    M, P = _first_moment(sim, x[0])
+    want = dt * h ** 3 * k * pk * J
+    scale = dt * h ** 3 * k
+    assert np.abs(P).max() < 1e-12 * scale / h
+    assert np.allclose(M, want * np.eye(3), rtol=1e-9, atol=1e-12 * scale)
+    assert np.sign(M[0, 0]) == np.sign(1.0 - J)
+
+
 def test_grid_update_gravity_and_walls():
     n, h = 24, 1.0 / 24
     sim = O.Sim(n, n, n, h, [], bc_pad=3, bc_mode=O.BC_AXIS)

@@ -193,6 +193,39 @@ def test_plasticity_invariants():
     assert np.abs(s2.Fp - np.eye(2)).max() < 1e-12
 
 
+@pytest.mark.parametrize("Jp", [0.85, 1.0, 1.08])
+def test_p2g_force_carries_the_hardening(Jp):
+    """P2G of MPMSnow.py:86-103 with v = C = 0 and no gravity: node momenta
+    m_i v_i are dt f_i, f_i = -V h tau grad w_i with h = exp(xi (1 - det F_p))
+    (:87).  For F_e = s R the 2D fixed-corotated stress is isotropic,
+    tau = [2 mu (s - 1) s + lam (s^2 - 1) s^2] I, and
+    sum_i grad w_i (x_i - x_p)^T = I, so the first moment of the momenta is
+    -dt V exp(xi (1 - J_p)) [2 mu (s-1) s + lam (s^2-1) s^2] I.  Lame
+    constants from (E, nu) by the textbook formulas."""
+    E, nu, xi, s = 1e3, 0.2, 10.0, 0.96
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    Fp = np.array([[Jp, 0.3], [0.0, 1.0]])                     # det = Jp
+    x = np.array([[0.5031, 0.4917]])
+    sim = O.Sim2D(x, 0.0, gravity=(0.0, 0.0), E=E, nu=nu, xi=xi, Fe=(s * R)[None], Fp=Fp[None],
+                  theta_c=0.5, theta_s=0.5)
+    dt = 1e-5
+    assert sim.step(dt) == 0
+    n = sim.n_grid
+    m = sim.grid_m
+    mom = m[:, None] * sim.grid_v.reshape(-1, 2)
+    idx = np.nonzero(m > 0)[0]
+    xi_ = np.stack([idx % n, idx // n], 1) / n
+    M = (mom[idx][:, :, None] * (xi_ - x[0])[:, None, :]).sum(0)
+    tau_iso = 2.0 * mu * (s - 1.0) * s + lam * (s * s - 1.0) * s * s
+    want = -dt * sim.params.p_vol * np.exp(xi * (1.0 - Jp)) * tau_iso
+    assert np.abs(mom[idx].sum(0)).max() < 1e-12 * abs(want) * n
+    assert np.allclose(M, want * np.eye(2), rtol=1e-9, atol=1e-12 * abs(want))
+    assert want > 0                                   # compressed: pushes outward
+
+
 def test_walls_and_gravity():
     x = np.array([[0.5, 0.5], [3.2 / 128, 0.5]])
     s = O.Sim2D(x, 0.0, gravity=(0.0, -50.0))
