@@ -7,7 +7,11 @@
 //   M^-1 p lies inside the shape (:249-251).
 // * Point-in-mesh: parity of +z ray crossings (the method of mesh_query's
 //   point_inside_mesh, thirdparties/mesh_query0.1/mesh_query.cpp:303), with
-//   triangles bucketed on a 2D grid over x-y.
+//   triangles bucketed on a 2D grid over x-y.  Orientations are exact (sign
+//   of a floating-point expansion when the plain double value is too close to
+//   zero to trust), and a ray through an edge or a vertex of the projected
+//   mesh is decided by the half-open rules of mesh_query's tri_zcast
+//   (mesh_query.cpp:18-111), so every crossing is counted once.
 // * OBJ reader (v / f records only) standing in for tinyobj.
 // * BGEO writer standing in for Partio::write (FilesIO.cpp:6-23).
 #include <math.h>
@@ -16,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -96,6 +101,119 @@ static bool invert(const Affine &M, Affine &I)
 }
 
 // ---------------------------------------------------------------------------
+// Exact orientation signs.  A sum of products of doubles is accumulated
+// without rounding as a floating-point expansion (Shewchuk 1997, "Adaptive
+// precision floating-point arithmetic and fast robust geometric predicates":
+// Knuth's two-sum, the exact product a*b = p + fma(a, b, -p), and
+// grow-expansion with zero elimination); the components stay nonoverlapping
+// and ordered by magnitude, so the sign of the sum is the sign of the last.
+struct Expansion {
+    double c[128];
+    int n = 0;
+    void add(double b)
+    {
+        double q = b;
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            const double x = q + c[i];
+            const double bv = x - q, av = x - bv;
+            const double err = (q - av) + (c[i] - bv);
+            q = x;
+            if (err != 0.0) c[m++] = err;
+        }
+        if (q != 0.0) c[m++] = q;
+        n = m;
+    }
+    void add_product(double a, double b)              // + a b
+    {
+        const double p = a * b;
+        add(std::fma(a, b, -p));
+        add(p);
+    }
+    void add_product3(double a, double b, double d)   // + a b d
+    {
+        const double p = a * b, pe = std::fma(a, b, -p);
+        const double q = p * d, qe = std::fma(p, d, -q);
+        const double r = pe * d, re = std::fma(pe, d, -r);
+        add(qe); add(re); add(r); add(q);
+    }
+    int sign() const { return n == 0 ? 0 : (c[n - 1] > 0.0 ? 1 : -1); }
+};
+
+// sign of (b - a) x (c - a) in the x-y plane: +1 when a, b, c turn
+// counter-clockwise
+static int orient2d(const double *a, const double *b, const double *c)
+{
+    const double l = (b[0] - a[0]) * (c[1] - a[1]), r = (b[1] - a[1]) * (c[0] - a[0]);
+    const double det = l - r;
+    // far above the rounding error of the double evaluation (< 4e-16 of the
+    // permanent): its sign is exact
+    if (fabs(det) > 1e-12 * (fabs(l) + fabs(r))) return det > 0.0 ? 1 : -1;
+    Expansion e;
+    e.add_product(b[0], c[1]); e.add_product(-b[0], a[1]); e.add_product(-a[0], c[1]);
+    e.add_product(-b[1], c[0]); e.add_product(b[1], a[0]); e.add_product(a[1], c[0]);
+    return e.sign();
+}
+
+static double det3(const double *u, const double *v, const double *w)
+{
+    return u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0]);
+}
+
+static void add_det3(Expansion &e, double sg, const double *u, const double *v, const double *w)
+{
+    e.add_product3(sg * u[0], v[1], w[2]); e.add_product3(-sg * u[0], v[2], w[1]);
+    e.add_product3(-sg * u[1], v[0], w[2]); e.add_product3(sg * u[1], v[2], w[0]);
+    e.add_product3(sg * u[2], v[0], w[1]); e.add_product3(-sg * u[2], v[1], w[0]);
+}
+
+// sign of det[a - d; b - d; c - d] (rows), i.e. of (a - d) . ((b - d) x (c - d))
+static int orient3d(const double *a, const double *b, const double *c, const double *d)
+{
+    double A[3], B[3], C[3];
+    for (int k = 0; k < 3; ++k) { A[k] = a[k] - d[k]; B[k] = b[k] - d[k]; C[k] = c[k] - d[k]; }
+    const double det = det3(A, B, C);
+    const double perm = fabs(A[0]) * (fabs(B[1] * C[2]) + fabs(B[2] * C[1])) +
+                        fabs(A[1]) * (fabs(B[0] * C[2]) + fabs(B[2] * C[0])) +
+                        fabs(A[2]) * (fabs(B[0] * C[1]) + fabs(B[1] * C[0]));
+    if (fabs(det) > 1e-12 * perm) return det > 0.0 ? 1 : -1;
+    // det of [[a, 1], [b, 1], [c, 1], [d, 1]] = |abc| - |abd| + |acd| - |bcd|
+    Expansion e;
+    add_det3(e, 1.0, a, b, c);
+    add_det3(e, -1.0, a, b, d);
+    add_det3(e, 1.0, a, c, d);
+    add_det3(e, -1.0, b, c, d);
+    return e.sign();
+}
+
+// Does the ray p + t (0, 0, 1), t > 0, cross triangle (q, r, s)?  A triangle
+// whose x-y projection is degenerate is never crossed; one the ray touches
+// only at or below p is not crossed.  A ray through the projection's edge or
+// vertex is counted by half-open rules on the projected, counter-clockwise
+// triangle (those of mesh_query's tri_zcast, mesh_query.cpp:18-111): on edge
+// u -> v it crosses when u lies above v (or level with v and to its left);
+// at vertex w (incoming from u, outgoing to v) when w.y >= v.y and
+// w.y < u.y.  Two triangles sharing an edge see it in opposite directions
+// when they lie on both sides of it in projection, so exactly one counts it.
+static bool ray_crosses(const double *p, const double *q, const double *r, const double *s)
+{
+    const int o = orient2d(q, r, s);
+    if (o == 0) return false;
+    if (o < 0) std::swap(r, s);
+    if (orient3d(p, q, r, s) >= 0) return false;       // p on or above the triangle's plane
+    const int a = orient2d(p, q, r), b = orient2d(p, r, s), c = orient2d(p, s, q);
+    if (a < 0 || b < 0 || c < 0) return false;
+    const int zeros = (a == 0) + (b == 0) + (c == 0);
+    if (zeros == 0) return true;
+    auto edge = [](const double *u, const double *v) { return u[1] > v[1] || (u[1] == v[1] && u[0] < v[0]); };
+    if (zeros == 1) return a == 0 ? edge(q, r) : (b == 0 ? edge(r, s) : edge(s, q));
+    // on a vertex: the one shared by the two zero edges
+    if (a == 0 && c == 0) return q[1] >= r[1] && q[1] < s[1];
+    if (a == 0 && b == 0) return r[1] >= s[1] && r[1] < q[1];
+    return s[1] >= q[1] && s[1] < r[1];
+}
+
+// ---------------------------------------------------------------------------
 // Triangle mesh with a 2D (x-y) bucket grid for +z ray parity queries.
 struct Mesh {
     std::vector<double> v;      // 3 per vertex
@@ -120,6 +238,8 @@ struct Mesh {
                 x0 = std::min(x0, p[0]); x1 = std::max(x1, p[0]);
                 y0 = std::min(y0, p[1]); y1 = std::max(y1, p[1]);
             }
+            // cell_x / cell_y are monotone, so a triangle whose closed
+            // projection holds p is listed in p's cell
             int cx0 = cell_x(x0), cx1 = cell_x(x1), cy0 = cell_y(y0), cy1 = cell_y(y1);
             for (int cy = cy0; cy <= cy1; ++cy)
                 for (int cx = cx0; cx <= cx1; ++cx) cells[(size_t)cy * gx + cx].push_back(t);
@@ -144,17 +264,8 @@ struct Mesh {
             return false;
         const std::vector<int> &c = cells[(size_t)cell_y(p[1]) * gx + cell_x(p[0])];
         int crossings = 0;
-        for (int t : c) {
-            const double *a = &v[3 * f[3 * t]], *b = &v[3 * f[3 * t + 1]], *d = &v[3 * f[3 * t + 2]];
-            double o_ab = (b[0] - a[0]) * (p[1] - a[1]) - (b[1] - a[1]) * (p[0] - a[0]);
-            double o_bd = (d[0] - b[0]) * (p[1] - b[1]) - (d[1] - b[1]) * (p[0] - b[0]);
-            double o_da = (a[0] - d[0]) * (p[1] - d[1]) - (a[1] - d[1]) * (p[0] - d[0]);
-            bool pos = o_ab > 0 && o_bd > 0 && o_da > 0, neg = o_ab < 0 && o_bd < 0 && o_da < 0;
-            if (!pos && !neg) continue;
-            double sum = o_ab + o_bd + o_da;     // barycentric weights: a <- o_bd, b <- o_da, d <- o_ab
-            double z = (o_bd * a[2] + o_da * b[2] + o_ab * d[2]) / sum;
-            if (z > p[2]) ++crossings;
-        }
+        for (int t : c)
+            crossings += ray_crosses(p, &v[3 * f[3 * t]], &v[3 * f[3 * t + 1]], &v[3 * f[3 * t + 2]]) ? 1 : 0;
         return (crossings & 1) != 0;
     }
 };

@@ -6,8 +6,10 @@ oracle/Makefile compiles that code from the reference's sources into
 oracle/_ref (it is not committed and the reference is not present on every
 machine, hence the skip).  Meshes here are generated (the reference ships no
 .obj files): a subdivided icosphere, a torus, and a non-convex extruded L.
-Random points are in general position, so the exact and the floating-point
-ray tests must agree on every point."""
+Random points are in general position; degenerate ones (rays through the
+projected mesh's vertices and edges, points on faces) exercise the exact
+orientation signs and the half-open tie-break rules, which must give the
+reference's decision on every point."""
 import ctypes as C
 import os
 
@@ -112,3 +114,33 @@ def test_point_in_mesh_matches_reference(tmp_path, shape):
     assert 0.05 < ref.mean() < 0.95, ref.mean()       # a real mix of inside and outside points
     mism = np.nonzero(ours != ref)[0]
     assert mism.size == 0, (shape, mism.size, pts[mism[:5]])
+
+
+def degenerate_points(v, f, zs):
+    """x-y positions on the projected mesh's vertices, on its edges (a
+    quarter, half and three quarters along, rounded to doubles: on the edge
+    or within an ulp of it) and at triangle centroids, each at heights zs."""
+    xy = [v[:, :2]]
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        pa, pb = v[f[:, a], :2], v[f[:, b], :2]
+        for t in (0.25, 0.5, 0.75):
+            xy.append(pa + t * (pb - pa))
+    xy.append(v[f].mean(1)[:, :2])
+    xy = np.unique(np.concatenate(xy), axis=0)
+    return np.array([(x, y, z) for x, y in xy for z in zs])
+
+
+@pytest.mark.parametrize("shape", ["cube", "boat", "L", "icosphere"])
+def test_point_in_mesh_degenerate_rays_match_reference(tmp_path, shape):
+    from tests.meshes import boat_hull, unit_cube
+    v, f = {"cube": unit_cube, "boat": boat_hull, "L": extruded_L, "icosphere": lambda: icosphere(1)}[shape]()
+    path = os.path.join(tmp_path, shape + ".obj")
+    write_obj(path, v, f)
+    lo, hi = v.min(0)[2], v.max(0)[2]
+    zs = sorted(set([lo - 0.5, lo, 0.5 * (lo + hi), hi, hi + 0.5] + list(np.unique(v[:, 2]))))
+    pts = degenerate_points(v, f, zs)
+    ours = capi.probe_point_in_mesh(path, pts)
+    ref = reference_inside(v, f, pts)
+    assert 0.02 < ref.mean() < 0.98, ref.mean()
+    mism = np.nonzero(ours != ref)[0]
+    assert mism.size == 0, (shape, mism.size, len(pts), pts[mism[:5]], ref[mism[:5]])
