@@ -5,10 +5,14 @@ Default workload: 3D snow, two jittered spheres (r = 0.15, +-3 m/s), exactly
 1,000,000 particles on a 192^3 grid (dx = 1/192), dt = 1e-4, FP32 -- the
 1M-particle snow configuration the metric is quoted on.  A "step" is one
 full explicit step (P2G -> grid update -> G2P, plus the re-sorts of the FAST
-form).  Particle state is resident in HBM before timing starts.  The default
-window (20 warm-up + 180 timed steps) covers the approach and the first 55
-steps of the collision; past step ~210 this explicit scheme is unstable at
-dt = 1e-4 (CFL of the hardened snow), in the FP64 oracle as on the device.
+form).  Particle state is resident in HBM before timing starts.  The timed
+steps end at step 200, inside the collision (the spheres' stencils meet at
+~150): an untimed pre-roll of up to 175 steps precedes the warm-up, so a
+short window (e.g. 5 warm-up + 20 timed steps) times plastic flow; the
+default 20 + 180 steps cover the approach and the collision.  Past step ~210
+this explicit scheme is unstable at dt = 1e-4 (CFL of the hardened snow), in
+the FP64 oracle as on the device; the step-size controller's leg times the
+collision beyond it.
 
 Prints one JSON line: particle-steps/s, ms per step, the roofline of the
 dominant kernel (algorithmic bytes / average launch time from HIP events on
@@ -30,25 +34,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes of each kernel: every persistent FP32 particle field
-# counted once per read pass and once per write pass, and per grid node with
-# mass 16 B for the P2G write-back and 12 B for the G2P velocity gather.
-#   PLAIN: P2G reads {x, v, C, F, J_p}; G2P reads {x, F, J_p}, writes {x, v, C, F, J_p}.
-#   FAST:  P2G reads {x, v, C, tau(6)}; G2P reads {x, F, J_p}, writes {x, v, C, F, J_p, tau}.
-#   FUSED: one kernel reads {x, F, J_p}, writes {x, v, C, F, J_p}, gathers and scatters the grid.
-# (jello has no J_p; water has J instead of F.)
-KERNEL_BYTES = {
-    # (material, form): {timed phase: (bytes per particle, bytes per node with mass)}
-    ("snow", "plain"): {"p2g": (100, 16), "g2p": (152, 12)},
-    ("snow", "fast"): {"p2g": (84, 16), "g2p": (176, 12)},
-    ("snow", "fused"): {"g2p": (152, 28)},
-    ("jello", "plain"): {"p2g": (96, 16), "g2p": (144, 12)},
-    ("jello", "fast"): {"p2g": (84, 16), "g2p": (168, 12)},
-    ("jello", "fused"): {"g2p": (144, 28)},
-    ("water", "plain"): {"p2g": (64, 16), "g2p": (80, 12)},
-    ("water", "fast"): {"p2g": (64, 16), "g2p": (80, 12)},
-    ("water", "fused"): {"g2p": (80, 28)},
+# Algorithmic bytes of each kernel (per particle, per grid node with mass):
+# every persistent FP32 particle field of the material counted once per read
+# pass and once per write pass, whatever the form keeps in between, and per
+# grid node with mass 16 B for the P2G write-back and 12 B for the G2P
+# velocity gather:
+#   P2G reads {x, v, C, F_e, J_p} (snow) / {x, v, C, F} (jello) / {x, v, C, J} (water);
+#   G2P reads {x, F_e, J_p} / {x, F} / {x, J} and writes the P2G set back.
+# The FAST form's P2G reads the stress its G2P stored instead of F (and J_p);
+# the figures stay those of the state the step advances.  FUSED: one kernel
+# does both.
+ALG_BYTES = {
+    "snow": {"p2g": (100, 16), "g2p": (152, 12)},
+    "jello": {"p2g": (96, 16), "g2p": (144, 12)},
+    "water": {"p2g": (64, 16), "g2p": (80, 12)},
 }
+KERNEL_BYTES = {}
+for _mat, _b in ALG_BYTES.items():
+    for _form in ("plain", "fast"):
+        KERNEL_BYTES[(_mat, _form)] = dict(_b)
+    KERNEL_BYTES[(_mat, "fused")] = {"g2p": (_b["p2g"][0] + _b["g2p"][0], _b["p2g"][1] + _b["g2p"][1])}
 KERNEL_NAME = {("plain", "p2g"): "k_p2g_atomic", ("plain", "g2p"): "k_g2p", ("fast", "p2g"): "k_p2g_block",
                ("fast", "g2p"): "k_g2p", ("fused", "g2p"): "k_g2p2g_block", ("plain", "grid"): "k_grid_update",
                ("fast", "grid"): "k_grid_update", ("fused", "grid"): "k_grid_update"}
@@ -63,6 +68,9 @@ def parse():
     # step as the device)
     ap.add_argument("--steps", type=int, default=180)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--preroll", type=int, default=-1,
+                    help="untimed steps before the warm-up (default: the snow collision window -- the timed "
+                         "steps end at step 200, at most 175 steps of pre-roll; 0 for the other workloads)")
     ap.add_argument("--workload", default="snow_spheres", choices=["snow_spheres", "jello_cube", "water_dam"])
     ap.add_argument("--form", default="fast", choices=["fused", "fast", "plain"])
     ap.add_argument("--sort-every", type=int, default=0)
@@ -113,15 +121,48 @@ def _make_sim(workload, form, sort_every=0, tile=0, warm_sweeps=0):
     return sim, info
 
 
-def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0):
+COLLISION_END = 200      # the snow spheres meet at ~150; fixed dt = 1e-4 holds until ~210
+
+
+def preroll_steps(workload, steps, warmup, preroll=-1):
+    """Untimed steps before the warm-up: by default the 1M snow's timed steps
+    end at step 200, inside the collision (at most 175 steps of pre-roll)."""
+    if preroll >= 0:
+        return preroll
+    if workload != "snow_spheres":
+        return 0
+    return max(0, min(175, COLLISION_END - warmup - steps))
+
+
+def svd_path_fraction(sim, steps, dt):
+    """Fraction of particle-steps that took the G2P's SVD path over the next
+    `steps` steps (stepping one at a time).  A snow particle takes it exactly
+    when its trial F leaves the clamp band, and then its J_p changes (the
+    clamp scales it by prod(sigma / sigma*)); on the elastic path J_p stays
+    bit-identical.  Also the fraction of particles plastically compressed
+    (J_p < 1 - 1e-3) at the end."""
+    from mpm_cuda_amd import capi
+    jp = sim.read(0, capi.P_JP)[:, 0].copy()
+    changed = 0
+    for _ in range(steps):
+        sim.step(dt)
+        j = sim.read(0, capi.P_JP)[:, 0]
+        changed += int(np.count_nonzero(j != jp))
+        jp = j.copy()
+    return changed / (sim.total * max(steps, 1)), float(np.mean(jp < 1 - 1e-3))
+
+
+def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0, preroll=0, svd_stats=False):
     """Throughput run (no per-kernel events in the timed region), then the
-    same warm-up + timed steps again on a fresh simulator with HIP events
-    around each phase and each phase's main kernel (the events cost host
-    time, so they stay out of the throughput run)."""
+    same pre-roll + warm-up + timed steps again on a fresh simulator with HIP
+    events around each phase and each phase's main kernel (the events cost
+    host time, so they stay out of the throughput run).  With svd_stats, a
+    third simulator steps the same window one step at a time to count the
+    particle-steps that took the SVD path."""
     sim, info = _make_sim(workload, form, sort_every, tile, warm_sweeps)
     n = sim.total
     dt = info["cfg"].dt
-    sim.run(warmup, dt)
+    sim.run(preroll + warmup, dt)
     sim.sync()
     sim.health()
     st0 = sim.stats()
@@ -136,7 +177,7 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0):
     sim.close()
     # timing pass: identical steps, events on during the timed region
     sim, _ = _make_sim(workload, form, sort_every, tile, warm_sweeps)
-    sim.run(warmup, dt)
+    sim.run(preroll + warmup, dt)
     sim.sync()
     sim.phase_times()
     sim.kernel_times()
@@ -146,7 +187,13 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0):
     ph = sim.phase_times()
     kt = sim.kernel_times()
     sim.close()
-    return {"n": n, "seconds": t1 - t0, "phases": ph, "kernels": kt, "health": health, "stats": stats, "info": info}
+    out = {"n": n, "seconds": t1 - t0, "phases": ph, "kernels": kt, "health": health, "stats": stats, "info": info}
+    if svd_stats and info["mat"].type == 1:
+        sim, _ = _make_sim(workload, form, sort_every, tile, warm_sweeps)
+        sim.run(preroll + warmup, dt)
+        out["svd_fraction"], out["plastic_fraction"] = svd_path_fraction(sim, steps, dt)
+        sim.close()
+    return out
 
 
 def run_batch(steps, warmup, warm=False, autotune_reps=2, seq_steps=5):
@@ -315,6 +362,23 @@ def run_dt_control(steps=600, every=20, max_dt=1e-4, cfl=0.5):
             out[name]["value"] = round(n * k / (t1 - t0), 1)
             out[name]["unit"] = "particle-steps/s (health checks every %d steps included)" % every
         sim.close()
+    # the collision under the controller, timed without health checks:
+    # 175 untimed steps, 20 warm-up, then `window` timed steps (past the
+    # step where the fixed dt fails), then the health check
+    pre, wu, window = 175, 20, 300
+    sim, info = workloads.snow_spheres(form=capi.FORM_FAST, dt_mode=capi.DT_CFL, cfl=cfl)
+    sim.run(pre + wu, max_dt)
+    dt0 = sim.last_dt
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(window, max_dt)
+    sim.sync()
+    t1 = time.perf_counter()
+    out["cfl_collision"] = {"timed_steps": [pre + wu, pre + wu + window], "value": round(sim.total * window / (t1 - t0), 1),
+                            "unit": "particle-steps/s", "ms_per_step": round((t1 - t0) / window * 1e3, 4),
+                            "dt_first": dt0, "dt_last": sim.last_dt, "sim_time": round(sim.current_time, 6),
+                            "health_bits": sim.health() & ~capi.H_ARENA_SPILL}
+    sim.close()
     return out
 
 
@@ -429,7 +493,8 @@ def main():
                           "batched_warm": run_batch(a.steps, a.warmup, warm=True)}), flush=True)
         return
     material = {"snow_spheres": "snow", "jello_cube": "jello", "water_dam": "water"}[a.workload]
-    res = run_gpu(a.workload, a.form, a.steps, a.warmup, a.sort_every, a.tile)
+    pre = preroll_steps(a.workload, a.steps, a.warmup, a.preroll)
+    res = run_gpu(a.workload, a.form, a.steps, a.warmup, a.sort_every, a.tile, preroll=pre, svd_stats=True)
     n = res["n"]
     value = n * a.steps / res["seconds"]
     out = {
@@ -446,12 +511,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": res["info"]["desc"], "particles": n, "form": a.form, "warm_start": "none",
-                   "sort_every": int(res["info"]["cfg"].sort_every), "tile": int(res["info"]["cfg"].tile)},
+                   "sort_every": int(res["info"]["cfg"].sort_every), "tile": int(res["info"]["cfg"].tile),
+                   "timed_steps": [pre + a.warmup, pre + a.warmup + a.steps]},
         "roofline": roofline(res, material, a.form, a.steps, a.workload),
         "health_bits": res["health"],
         "fp64_svd_fallbacks": res["stats"]["fp64_fallbacks"],
         "sorts_in_timed_steps": int(res["stats"]["sorts_timed"]),
     }
+    if "svd_fraction" in res:
+        # particle-steps of the timed window whose G2P took the Jacobi SVD
+        # (trial F outside the clamp band: plastic flow) instead of the
+        # elastic path, and the particles compressed plastically by its end
+        out["svd_path_fraction"] = round(res["svd_fraction"], 5)
+        out["plastic_fraction"] = round(res["plastic_fraction"], 5)
     # the achievable bandwidth beside the published peak
     from mpm_cuda_amd import capi
     cp = round(capi.probe_copy_bandwidth(1 << 30, 10), 1)
@@ -460,20 +532,26 @@ def main():
     if not a.no_variants:
         ksteps = max(10, a.steps // 4)
         out["variants"] = {}
+        kwu = min(a.warmup, 5)
+        kpre = preroll_steps(a.workload, ksteps, kwu, a.preroll)
         for form in ("fast", "fused", "plain"):
             if form == a.form:
                 continue
-            r = run_gpu(a.workload, form, ksteps, min(a.warmup, 5))
+            r = run_gpu(a.workload, form, ksteps, kwu, preroll=kpre)
             v = r["n"] * ksteps / r["seconds"]
             out["variants"][form] = {"value": round(v, 1), "ms_per_step": round(r["seconds"] / ksteps * 1e3, 4),
                                      "roofline": roofline(r, material, form, ksteps, a.workload)}
         # learned warm start of the G2P SVD: network quaternion + 2 Jacobi
         # sweeps, cold 4-sweep SVD for every particle above the 1e-6 bound
-        r = run_gpu(a.workload, a.form, ksteps, min(a.warmup, 5), warm_sweeps=2)
+        r = run_gpu(a.workload, a.form, ksteps, kwu, warm_sweeps=2, preroll=kpre, svd_stats=True)
         v = r["n"] * ksteps / r["seconds"]
+        svd_calls = r.get("svd_fraction", 0.0) * r["n"] * ksteps
         out["variants"]["warm_start"] = {
             "value": round(v, 1), "ms_per_step": round(r["seconds"] / ksteps * 1e3, 4), "form": a.form,
-            "warm_sweeps": 2, "rejected_fraction": round(r["stats"]["warm_rejects_timed"] / (r["n"] * ksteps), 4),
+            "warm_sweeps": 2, "timed_steps": [kpre + kwu, kpre + kwu + ksteps],
+            "svd_path_fraction": round(r.get("svd_fraction", 0.0), 5),
+            "warm_svd_calls": int(round(svd_calls)), "warm_rejects": int(r["stats"]["warm_rejects_timed"]),
+            "rejected_fraction": round(r["stats"]["warm_rejects_timed"] / svd_calls, 4) if svd_calls else None,
             "kernel_avg_ms": roofline(r, material, a.form, ksteps, a.workload)["kernel_avg_ms"]}
         if "plain" in out["variants"]:
             out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)

@@ -93,7 +93,7 @@ typedef struct mpm_config {
     uint64_t seed;            /* jitter seed for mpm_add_object */
     int32_t form;             /* MPM_FORM_* */
     int32_t sort_every;       /* FAST/FUSED: most steps between re-sorts (24); a re-sort
-                                 comes earlier once >0.1% of particles leave their arena */
+                                 comes earlier once >0.01% of particles leave their arena */
     int32_t svd_sweeps;       /* FP32 Jacobi sweeps (plain solve) */
     int32_t warm_start;       /* MPM_WARM_* */
     int32_t warm_sweeps;      /* Jacobi sweeps after a warm start */
@@ -190,7 +190,9 @@ int mpm_add_particles(mpm_sim *sim, const mpm_material *mat, int64_t n, const fl
 /* MPMSimulator::step (MPMCUDA.cu:793-925); dt <= 0 uses cfg.dt. Asynchronous. */
 int mpm_step(mpm_sim *sim, double dt);
 
-/* nsteps calls of mpm_step, launched back to back (graph-replayed in FAST form). */
+/* nsteps calls of mpm_step, queued back to back on the simulator's stream (one
+ * host call per step, no graph; in the sorted forms the host waits at most
+ * until the device is two steps behind, to read the spill counts). */
 int mpm_run(mpm_sim *sim, int32_t nsteps, double dt);
 
 /* One phase of the step, for phase-by-phase parity checks. */

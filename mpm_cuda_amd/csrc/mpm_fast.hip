@@ -1,17 +1,20 @@
-// mpm_fast.hip -- FAST execution form: cell-block sort and LDS-blocked P2G.
+// mpm_fast.hip -- FAST execution form: block sort and LDS-blocked P2G.
 //
 // No counterpart in the reference (JerryYan97/MPM_CUDA scatters unsorted
 // particles with global FP64 atomics, MPMCUDA.cu:388-421).  Pieces:
 //   * exclusive scan (three-kernel, wave-shuffle based);
-//   * stable LSD radix sort of 32-bit keys, 8-bit digits, ballot-matched
-//     in-wave ranking;
-//   * per-group sort of particles by (4x4x4 block, cell) of their base
+//   * stable LSD radix sort of 32-bit keys, 8- or 9-bit digits,
+//     ballot-matched in-wave ranking;
+//   * per-group sort of particles by the 4x4x4 node block of their base
 //     node, payload permutation of every SoA plane, and one tile (block,
 //     first particle, count, staging slot) per non-empty block;
-//   * P2G over tiles without atomics: one wave per block scatters into an
-//     8^3-node LDS arena by plain read-modify-write, lane l owning cell l,
-//     and writes the arena to a staging slot that the grid update gathers.
-// (On gfx950 an LDS float atomic runs ~20x slower than a plain LDS
+//   * P2G over tiles without atomics: one wave per block bins its particles
+//     by base node in LDS, sums each bin's particles in registers and adds
+//     the sums to an 8^3-node LDS arena by plain read-modify-write, then
+//     writes the arena's nodes with mass, and a mask of them, to a staging
+//     slot that the grid update gathers.
+// (On gfx950 an LDS float atomic add costs ~200 cycles per wave instruction
+// even when the 64 lanes hit different addresses, ~15x a float4 plain LDS
 // read-modify-write, and global float atomics at ~77 G/s would dominate a
 // per-node flush.)
 #include "mpm_g2p.h"
@@ -403,10 +406,19 @@ struct TileLds {
     uint32_t nitems;
 };
 
+// Arena layout of the FAST tile: rows of 12 slots (8 nodes + 4 of padding),
+// layers of 8 rows, so the 27 node offsets of a stencil are constants added
+// to the base slot (immediate offsets of the LDS instructions) and the 8
+// lanes one LDS cycle serves -- consecutive bins: four cells of a row and
+// four of the next -- fall on 8 different 16-byte bank groups (row stride
+// 12 = 4 mod 8).
+constexpr int FA_ROW = 12, FA_LAYER = FA_ROW * ARENA, FA_SLOTS = FA_LAYER * ARENA;
+__device__ __forceinline__ int fa_slot(int x, int y, int z) { return x + FA_ROW * y + FA_LAYER * z; }
+
 // LDS of a FAST-form tile (one wave): bin keys, the sort and the arena
-// (12.5 KB; the particle records stay in HBM / cache).
+// (16.5 KB; the particle records stay in HBM / cache).
 struct TileKeys {
-    float4 arena[A_SLOTS];
+    float4 arena[FA_SLOTS];
     uint32_t items[CHUNK];
     uint16_t perm[CHUNK];
     uint16_t rnk[CHUNK];
@@ -433,11 +445,11 @@ __device__ __forceinline__ TileGeo tile_geo(const GridP &g, uint32_t blk)
 // given).  Returns true when the particle was listed.
 template <class TL>
 __device__ __forceinline__ bool bin_key(const GridP &g, TL &L, const TileGeo &t, bool active, int slot, int i,
-                                        const Stencil &s, bool staged, uint32_t *spill, uint32_t *counters,
+                                        const int b[3], bool staged, uint32_t *spill, uint32_t *counters,
                                         uint32_t *flags)
 {
     const int lane = threadIdx.x & 63;
-    const int lx = s.b[0] - t.ax, ly = s.b[1] - t.ay, lz = s.b[2] - t.az;
+    const int lx = b[0] - t.ax, ly = b[1] - t.ay, lz = b[2] - t.az;
     const bool inside = active && staged && lx >= 0 && lx <= ARENA - 3 && ly >= 0 && ly <= ARENA - 3 && lz >= 0 &&
                         lz <= ARENA - 3;
     const bool spill_me = active && !inside;
@@ -450,7 +462,7 @@ __device__ __forceinline__ bool bin_key(const GridP &g, TL &L, const TileGeo &t,
         const uint64_t below = lane ? (sm & ((~0ull) >> (64 - lane))) : 0ull;
         if (spill_me) {
             gp(spill)[base + __popcll(below)] = (uint32_t)i;
-            if (flags) mark_blocks(g, s, flags);
+            if (flags) mark_blocks(g, b, flags);
         }
     }
     if (active) L.key[slot] = inside ? (uint8_t)(lx + 6 * (ly + 6 * lz)) : (uint8_t)NO_BIN;
@@ -481,7 +493,7 @@ __device__ __forceinline__ bool bin_particle(const GridP &g, TileLds &L, const T
                                              int i, const Stencil &s, const Scatter &sc, bool staged,
                                              uint32_t *spill, uint32_t *counters)
 {
-    const bool spill_me = bin_key(g, L, t, active, slot, i, s, staged, spill, counters, nullptr);
+    const bool spill_me = bin_key(g, L, t, active, slot, i, s.b, staged, spill, counters, nullptr);
     if (active && !spill_me) {
         float q0[3], hB[9], Gs[6];
         make_record<MAT>(g, s, sc, q0, hB, Gs);
@@ -731,6 +743,14 @@ __device__ __forceinline__ void accum_particle(const TileLds &L, int slot, float
 // arena in a fixed order (compiler barrier; LDS executes one wave's
 // operations in order).  Lanes of different bins touch different slots in
 // each instruction; lanes of the same bin take turns.
+struct XorArena {
+    static __device__ __forceinline__ int slot(int x, int y, int z) { return a_slot(x, y, z); }
+};
+struct PaddedArena {
+    static __device__ __forceinline__ int slot(int x, int y, int z) { return fa_slot(x, y, z); }
+};
+
+template <class Layout>
 __device__ __forceinline__ void flush_item(float4 *arena, bool has, uint32_t bin, const float4 acc[27])
 {
     const int lane = threadIdx.x & 63;
@@ -747,7 +767,7 @@ __device__ __forceinline__ void flush_item(float4 *arena, bool has, uint32_t bin
                 for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
                     for (int ox = 0; ox < 3; ++ox) {
-                        const int a = a_slot(bx + ox, by + oy, bz + oz);
+                        const int a = Layout::slot(bx + ox, by + oy, bz + oz);
                         const float4 &c = acc[ox + 3 * oy + 9 * oz];
                         float4 v = arena[a];
                         v.x += c.x; v.y += c.y; v.z += c.z; v.w += c.w;
@@ -773,10 +793,12 @@ __device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int bat
     float4 acc[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t rounds = wave_max_u32(cnt);
+    // item counts never increase along the item list (build_items), so the
+    // batch's first lane holds its largest
+    const uint32_t rounds = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
     for (uint32_t r = 0; r < rounds; ++r)
         if (r < cnt) accum_particle<MAT>(L, L.perm[first + r], inv_h, m, acc);
-    flush_item(arena, has, bin, acc);
+    flush_item<XorArena>(arena, has, bin, acc);
 }
 
 // Phase C for one wave, FAST form: the item's particles are loaded again
@@ -795,26 +817,32 @@ __device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena,
     float4 acc[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) acc[o] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t rounds = wave_max_u32(cnt);
-    // the next round's particle is loaded while this one is accumulated;
-    // lanes without a next particle reload a valid one (one basic block)
+    // item counts never increase along the item list (build_items), so the
+    // batch's first lane holds its largest
+    const uint32_t rounds = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
+    // the next round's particle is loaded while this one is accumulated
+    // (two register sets taking turns, no copies); lanes without a next
+    // particle reload a valid one (one basic block)
     auto slot_of = [&](uint32_t r) -> int { return cnt ? (int)L.perm[first + min(r, cnt - 1u)] : 0; };
-    RawP nxt;
-    load_raw<MAT>(pv, i0 + slot_of(0), nxt);
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const RawP raw = nxt;
-        if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), nxt);
-        if (r < cnt) {
-            Scatter sc;
-            scatter_from_raw<MAT>(raw, mat, dt, sc);
-            Stencil st;
-            make_stencil(g, raw.x, st);
-            float q0[3], hB[9], Gs[6];
-            make_record<MAT>(g, st, sc, q0, hB, Gs);
-            accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
-        }
+    auto accumulate = [&](const RawP &raw) {
+        Scatter sc;
+        scatter_from_raw<MAT>(raw, mat, dt, sc);
+        Stencil st;
+        make_stencil(g, raw.x, st);
+        float q0[3], hB[9], Gs[6];
+        make_record<MAT>(g, st, sc, q0, hB, Gs);
+        accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
+    };
+    RawP ra, rb;
+    load_raw<MAT>(pv, i0 + slot_of(0), ra);
+    for (uint32_t r = 0; r < rounds; r += 2) {
+        if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), rb);
+        if (r < cnt) accumulate(ra);
+        if (r + 1 >= rounds) break;
+        if (r + 2 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 2), ra);
+        if (r + 1 < cnt) accumulate(rb);
     }
-    flush_item(arena, has, bin, acc);
+    flush_item<PaddedArena>(arena, has, bin, acc);
 }
 
 // Phases B and C of one chunk (all threads; ends synchronized).
@@ -843,7 +871,9 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 {
     __syncthreads();
     if (staged) {
-        const int64_t dst = (int64_t)slot * ARENA_NODES;
+        const int64_t dst = (int64_t)slot * STAGE_STRIDE;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        uint32_t *mask = (uint32_t *)(staging + dst + ARENA_NODES);
         for (int k = threadIdx.x; k < ARENA_NODES; k += TILE_THREADS) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
             const int a = a_slot(kx, ky, kz);
@@ -853,11 +883,16 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
                 const float4 u = L.arena[w][a];
                 v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
             }
-            gstore4(staging, dst + k, v);
+            const bool nz = v.x != 0.f;      // no mass, no momentum (every weight and gradient factor vanishes)
+            if (nz) gstore4(staging, dst + k, v);
+            // nodes k - lane .. k - lane + 63 are one 64-bit mask word
+            const uint64_t m = __ballot(nz);
+            if (lane < 2) gp(mask)[((k - lane) >> 5) + lane] = (uint32_t)(m >> (32 * lane));
             const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
-            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+            if (nz && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
                 gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
         }
+        (void)wave;
     }
     __syncthreads();
 }
@@ -878,7 +913,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     const TileGeo t = tile_geo(g, d.z);
     const bool staged = d.w < staging_cap;
     const int lane = threadIdx.x & 63;
-    for (int k = lane; k < A_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = lane; k < FA_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
         const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
         // A: all position loads of the chunk first (lanes past its end reload
@@ -895,10 +930,9 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
             if (it * 64 >= nc) break;
             const int slot = it * 64 + lane;
             const bool active = slot < nc;
-            Stencil s;
-            s.b[0] = s.b[1] = s.b[2] = 0;
-            if (active) bits |= make_stencil(g, xs[it], s);
-            bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, s, staged, spill, counters, flags);
+            int b[3] = {0, 0, 0};
+            if (active) bits |= base_only(g, xs[it], b);
+            bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, b, staged, spill, counters, flags);
         }
         __asm__ volatile("" ::: "memory");
         // B and C
@@ -910,13 +944,17 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     // the arena into the tile's staging slot (logical node order); mark the
     // node blocks that received mass
     if (staged) {
-        const int64_t dst = (int64_t)d.w * ARENA_NODES;
+        const int64_t dst = (int64_t)d.w * STAGE_STRIDE;
+        uint32_t *mask = (uint32_t *)(staging + dst + ARENA_NODES);
         for (int k = lane; k < ARENA_NODES; k += 64) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
-            const float4 v = L.arena[a_slot(kx, ky, kz)];
-            gstore4(staging, dst + k, v);
+            const float4 v = L.arena[fa_slot(kx, ky, kz)];
+            const bool nz = v.x != 0.f;      // no mass, no momentum (every weight and gradient factor vanishes)
+            if (nz) gstore4(staging, dst + k, v);
+            const uint64_t m = __ballot(nz);
+            if (lane < 2) gp(mask)[((k - lane) >> 5) + lane] = (uint32_t)(m >> (32 * lane));
             const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
-            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+            if (nz && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
                 gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
         }
     }
@@ -1119,7 +1157,7 @@ static int ensure_fast_buffers(mpm_sim *s)
         MPM_HIP(hipMalloc(&s->tile_desc, need * sizeof(uint4)));
         // every non-empty block may need a staging slot; blocks beyond the cap scatter with atomics
         int64_t scap = std::min<int64_t>(need, (int64_t)1 << 19);
-        MPM_HIP(hipMalloc(&s->staging, scap * ARENA_NODES * sizeof(float4)));
+        MPM_HIP(hipMalloc(&s->staging, scap * STAGE_STRIDE * sizeof(float4)));
         MPM_HIP(hipMalloc(&s->blk_tile, (int64_t)s->groups.size() * s->nblocks * sizeof(int32_t)));
         s->staging_cap = scap;
         s->tile_cap = need;
@@ -1316,6 +1354,7 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
         // compaction of the grid phase (the tiles flagged their blocks)
         s->spill_pending = true;
         s->spill_dt = dt;
+        s->spill_g = s->g;          // a later scatter takes the dt this launch took
         s->spill_hw = hw;
         s->spill_seq = (uint32_t)s->tile_launches;
         return note_spills(s, group);
@@ -1331,7 +1370,7 @@ int spill_compact(mpm_sim *s)
     if (!s->spill_pending) return hip_fail(hipErrorInvalidValue, "spill_compact");
     s->spill_pending = false;
     const PartView pv = view_of(s, 0);
-    const int rc = launch_spill_compact(s->stream, s->g, s->groups[0].mat, pv, s->spill_dt, s->spill,
+    const int rc = launch_spill_compact(s->stream, s->spill_g, s->groups[0].mat, pv, s->spill_dt, s->spill,
                                         (uint32_t)pv.n, s->counters, s->grid_mp, s->flags, (int)s->nblocks,
                                         s->list, s->spill_hw, s->spill_hw + mpm_sim::SPILL_SLOTS * 256,
                                         s->spill_seq);
@@ -1343,7 +1382,7 @@ int flush_spill(mpm_sim *s)
     if (!s->spill_pending) return MPM_OK;
     s->spill_pending = false;
     const PartView pv = view_of(s, 0);
-    const int rc = launch_p2g_spill(s->stream, s->g, s->groups[0].mat, pv, s->spill_dt, s->spill, (uint32_t)pv.n,
+    const int rc = launch_p2g_spill(s->stream, s->spill_g, s->groups[0].mat, pv, s->spill_dt, s->spill, (uint32_t)pv.n,
                                     s->counters, s->grid_mp, s->flags, s->spill_hw,
                                     s->spill_hw + mpm_sim::SPILL_SLOTS * 256, s->spill_seq);
     return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;

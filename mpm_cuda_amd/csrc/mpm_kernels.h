@@ -25,12 +25,13 @@ struct Stencil {
     float w[3][3], dw[3][3];
 };
 
-// Flag the (up to 2x2x2) 4^3-node blocks a particle's 3x3x3 stencil touches.
-__device__ __forceinline__ void mark_blocks(const GridP &g, const Stencil &s, uint32_t *flags)
+// Flag the (up to 2x2x2) 4^3-node blocks the 3x3x3 stencil of base node b
+// touches.
+__device__ __forceinline__ void mark_blocks(const GridP &g, const int b[3], uint32_t *flags)
 {
-    int bx0 = s.b[0] >> 2, bx1 = (s.b[0] + 2) >> 2;
-    int by0 = s.b[1] >> 2, by1 = (s.b[1] + 2) >> 2;
-    int bz0 = s.b[2] >> 2, bz1 = (s.b[2] + 2) >> 2;
+    int bx0 = b[0] >> 2, bx1 = (b[0] + 2) >> 2;
+    int by0 = b[1] >> 2, by1 = (b[1] + 2) >> 2;
+    int bz0 = b[2] >> 2, bz1 = (b[2] + 2) >> 2;
     for (int bz = bz0; bz <= bz1; ++bz)
         for (int by = by0; by <= by1; ++by)
             for (int bx = bx0; bx <= bx1; ++bx)
@@ -50,6 +51,21 @@ __device__ __forceinline__ uint32_t make_stencil(const GridP &g, const float x[3
         // relative coordinate, one rounding: (x - o) * inv_h - b
         s.fx[a] = fmaf(x[a] - o[a], g.inv_h, -(float)b);
         bspline_axis_fast<float>(s.fx[a], s.w[a], s.dw[a]);
+    }
+    return bits;
+}
+
+// The base node alone (as make_stencil computes it, same clamp and flag).
+__device__ __forceinline__ uint32_t base_only(const GridP &g, const float x[3], int b[3])
+{
+    uint32_t bits = 0;
+    const float o[3] = {g.ox, g.oy, g.oz};
+    const int n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int v = base_index_f32(x[a], o[a], g.inv_h);
+        if (v > n[a] - 3) { v = n[a] - 3; bits |= MPM_H_STENCIL_OOB; }
+        b[a] = v;
     }
     return bits;
 }

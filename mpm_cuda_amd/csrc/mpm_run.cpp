@@ -4,10 +4,18 @@
 // <out>/FrameNNNNNN.bgeo whenever int(t / (1/fps)) reaches the next frame
 // number (main.cpp:28-29, 214-221).
 //
-//   mpm_run --scene snow|jello|water|obj [--obj path] [--time 1.0] [--fps 36]
-//           [--out Animations] [--form fast|plain] [--grid N]
-//           [--dt-mode fixed|cfl|cfl_deform] [--cfl 0.5]
-// With a dt controller the scene's dt is the largest step (max_dt).
+//   mpm_run --scene snow|jello|water|obj|boat [--obj path] [--obj2 path]
+//           [--time 1.0] [--fps 36] [--out Animations] [--form fast|plain|fused]
+//           [--grid N] [--dt-mode fixed|cfl|cfl_deform] [--cfl 0.5]
+//           [--steps K] [--final path.bgeo]
+// With a dt controller the scene's dt is the largest step (max_dt).  --steps
+// stops after K steps instead of at --time; --final writes the positions
+// after the last step.
+// The boat scene is the reference application's own (main.cpp:138-161): a
+// water cube (--obj, scaled 2x3x2, at (6, 5, 4)) and a jello boat (--obj2,
+// turned 90 degrees about y, at (6, 5, 10)), gap 0.04, dt 1/5000, a
+// 300x500x500 grid and 10 particles per cell; --gap, --nx, --ny, --nz
+// coarsen it.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -43,11 +51,14 @@ int main(int argc, char **argv)
 
     mpm_config cfg;
     mpm_config_default(&cfg);
-    cfg.form = form == "plain" ? MPM_FORM_PLAIN : MPM_FORM_FAST;
+    cfg.form = form == "plain" ? MPM_FORM_PLAIN : (form == "fused" ? MPM_FORM_FUSED : MPM_FORM_FAST);
     cfg.dt_mode = dt_mode == "cfl" ? MPM_DT_CFL : (dt_mode == "cfl_deform" ? MPM_DT_CFL_DEFORM : MPM_DT_FIXED);
     cfg.cfl = (float)atof(arg(argc, argv, "--cfl", "0.5"));
     double h = 1.0 / n, dt = 1e-4;
     unsigned ppc = 8;
+    unsigned nx = (unsigned)n, ny = (unsigned)n, nz = (unsigned)n;
+    const long max_steps = atol(arg(argc, argv, "--steps", "0"));
+    const std::string final_path = arg(argc, argv, "--final", "");
     std::vector<ObjInitInfo> infos;
     try {
         if (scene == "snow") {
@@ -87,17 +98,44 @@ int main(int argc, char **argv)
             o.initTranslation = vec3(0.5f, 0.5f, 0.5f);
             o.mMaterial = Material(1e4, 0.4, 800.0, JELLO);
             infos.push_back(o);
+        } else if (scene == "boat") {
+            ObjInitInfo w;                                   // main.cpp:139-147
+            w.objPath = arg(argc, argv, "--obj", "");
+            w.initVel = {0.0, 0.0, 0.0};
+            w.initRotationDegree = 0.f;
+            w.initScale = vec3(2.f, 3.f, 2.f);
+            w.initTranslation = vec3(6.f, 5.f, 4.f);
+            w.initRotationAxis = vec3(1.f, 0.f, 0.f);
+            w.mMaterial = Material(5e3, 0.3, 1.1, WATER);
+            infos.push_back(w);
+            ObjInitInfo b;                                   // main.cpp:150-158
+            b.objPath = arg(argc, argv, "--obj2", "");
+            b.initVel = {0.0, 0.0, 0.0};
+            b.initRotationDegree = 90.f;
+            b.initScale = vec3(1.f);
+            b.initTranslation = vec3(6.f, 5.f, 10.f);
+            b.initRotationAxis = vec3(0.f, 1.f, 0.f);
+            b.mMaterial = Material(1e4, 0.4, 0.8, JELLO);
+            infos.push_back(b);
+            h = atof(arg(argc, argv, "--gap", "0.04"));      // main.cpp:161
+            dt = 1.0 / 5000.0;
+            ppc = 10;
+            nx = (unsigned)atoi(arg(argc, argv, "--nx", "300"));
+            ny = (unsigned)atoi(arg(argc, argv, "--ny", "500"));
+            nz = (unsigned)atoi(arg(argc, argv, "--nz", "500"));
         } else {
             fprintf(stderr, "unknown --scene %s\n", scene.c_str());
             return 2;
         }
-        MPMSimulator sim(h, dt, n, n, n, ppc, infos, &cfg);
+        MPMSimulator sim(h, dt, nx, ny, nz, ppc, infos, &cfg);
         printf("particles: %d\n", sim.totalParticlesNum());
+        for (size_t k = 0; k < sim.mParticlesGroupsVec.size(); ++k)
+            printf("group %zu: %u\n", k, sim.mParticlesGroupsVec[k].particleNum);
         const float time_per_frame = 1.f / (float)fps;
         int frame = 0;
         std::vector<float> pos;
         auto t0 = std::chrono::steady_clock::now();
-        while (sim.current_time <= t_end) {
+        while (max_steps > 0 ? sim.current_frame < max_steps : sim.current_time <= t_end) {
             sim.step();
             if ((int)(sim.current_time / time_per_frame) >= frame) {
                 sim.getGLParticlesPos(pos);
@@ -106,6 +144,10 @@ int main(int argc, char **argv)
                 OutputBego(ss.str(), pos);
                 ++frame;
             }
+        }
+        if (!final_path.empty()) {
+            sim.getGLParticlesPos(pos);
+            OutputBego(final_path, pos);
         }
         double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         printf("steps: %d  frames: %d  wall: %.3f s  particle-steps/s: %.3e\n", sim.current_frame, frame, wall,

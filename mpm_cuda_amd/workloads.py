@@ -5,8 +5,9 @@
   5 samples per cell, trimmed to 500,000 particles each (1,000,000 total);
   E = 1.4e5, nu = 0.2, rho = 400, theta_c = 2.5e-2, theta_s = 7.5e-3, xi = 10,
   dt = 1e-4.
-* ``jello_cube``    -- 3D jello, 128^3 grid (dx = 1/128), a 0.25^3 cube of
-  32^3 cells x 8 samples = 262,144 particles, bottom at y = 0.1, v0 = (0,-2,0),
+* ``jello_cube``    -- 3D jello, 128^3 grid (dx = 1/128), a 0.25^3 cube
+  (32^3 cells' worth at 8 samples per cell; its bottom at y = 0.1 is not on
+  a cell face, and 262,146 of the jittered samples fall inside), v0 = (0,-2,0),
   E = 5e4, nu = 0.3, rho = 1e3, sticky walls, dt = 1e-4.
 * ``water_dam``     -- 3D water dam break, 256^3 grid (dx = 1/256), a
   0.5 x 0.4 x 0.5 block in the lower-left corner (inside the 10-cell wall

@@ -105,3 +105,47 @@ def rel(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def device_fields(sim, group):
+    """One group's device state (FP32) in a common layout."""
+    return {"x": sim.read(group, capi.P_POS), "v": sim.read(group, capi.P_VEL),
+            "C": sim.read(group, capi.P_C).reshape(-1, 3, 3), "F": sim.read(group, capi.P_F).reshape(-1, 3, 3),
+            "S": sim.read(group, capi.P_JP)[:, 0]}
+
+
+def oracle_group(mat, vol, f):
+    """An oracle group holding a device group's state `f` (J_p enters the
+    oracle as F_p = J_p^(1/3) I: only det F_p is ever used)."""
+    S = f["S"].astype(np.float64)
+    kw = {}
+    if mat.type == capi.SNOW:
+        kw["Fp"] = np.cbrt(S)[:, None, None] * np.eye(3)
+    if mat.type == capi.WATER:
+        kw["J"] = S
+    return O.Group(f["x"].astype(np.float64), f["v"].astype(np.float64), type=mat.type, mass=mat.rho * vol, vol=vol,
+                   E=mat.E, nu=mat.nu, theta_c=mat.theta_c, theta_s=mat.theta_s, xi=mat.xi, eos_k=mat.eos_k,
+                   eos_gamma=mat.eos_gamma, water_c_mode=mat.water_c_mode, C=f["C"].astype(np.float64),
+                   Fe=f["F"].astype(np.float64), **kw)
+
+
+def oracle_from_device(sim, cfg, mats, vol=None):
+    """The FP64 oracle holding the device's current state, group by group
+    (mats: one material per group, in group order)."""
+    vol = cfg.h ** 3 if vol is None else vol
+    groups = [oracle_group(m, vol, device_fields(sim, k)) for k, m in enumerate(mats)]
+    return O.Sim(cfg.nx, cfg.ny, cfg.nz, cfg.h, groups, gravity=tuple(cfg.gravity), bc_pad=cfg.bc_pad,
+                 bc_mode=cfg.bc_mode)
+
+
+def oracle_fields(o: "O.Group"):
+    S = np.linalg.det(o.Fp) if o.mat.type == capi.SNOW else o.J
+    return {"x": o.x, "v": o.v, "C": o.C, "F": o.Fe, "S": S}
+
+
+def host_threads(cap=16):
+    import os
+    try:
+        return max(1, min(cap, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(cap, os.cpu_count() or 1))

@@ -12,7 +12,7 @@ from tests.scenes import rel
 pytestmark = pytest.mark.gpu
 
 
-def _state(seed, n=3000, moving=True):
+def _state(seed, n=9000, moving=True):
     rng = np.random.default_rng(seed)
     x = np.concatenate([rng.random((n // 3, 2)) * 0.2 + [0.3 + 0.1 * g, 0.05 + 0.32 * g] for g in range(3)])
     v = (2.0 * rng.normal(size=x.shape)) if moving else np.zeros_like(x)

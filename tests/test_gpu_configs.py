@@ -1,0 +1,219 @@
+"""The 3D workloads at their own sizes (GPU only): the 1M-particle snow
+collision (192^3), the 262,146-particle jello cube (128^3, 500 steps), the
+2M-particle water dam break (256^3) and the 64-scene batch, each against the
+FP64 oracle (OpenMP on the host cores) over a window the oracle finishes in
+seconds, and over the full run through properties that do not depend on
+size: health bits clear, mass conserved, particles finite and inside the
+grid, the contact the scene is built for.
+
+Relative L2 errors (||a - b|| / ||b||) per field, device FP32 against the
+oracle in FP64 on the same FP32 initial state.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mpm_cuda_amd import capi, workloads
+from tests.scenes import device_fields, host_threads, oracle_fields, oracle_from_device, oracle_group, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _threads():
+    O.set_threads(host_threads())
+
+
+def _errors(g, o, keys):
+    return {k: rel(g[k], o[k]) for k in keys}
+
+
+def _inside(sim, cfg):
+    for k in range(sim.group_count()):
+        x = sim.read(k, capi.P_POS)
+        assert np.isfinite(x).all()
+        hi = np.array([cfg.nx - 1, cfg.ny - 1, cfg.nz - 1]) * cfg.h
+        assert (x >= 0).all() and (x <= hi).all()
+
+
+def _grid_mass_matches(sim, mass_total):
+    m = sim.grid(capi.G_MASS).astype(np.float64)
+    assert abs(m.sum() / mass_total - 1) < 1e-5, m.sum() / mass_total
+
+
+def test_snow_1m_from_start_against_oracle(gpu):
+    """1,000,000 snow particles on 192^3, 20 steps from the initial state:
+    x within 1e-6, v within 1e-5, F within 1e-6 (relative L2), J_p within
+    1e-6 absolute, and C within 1e-4 of the velocity-gradient scale
+    max|v| / h (the oracle's C is ~0 before the spheres meet, so its relative
+    error carries no meaning)."""
+    sim, info = workloads.snow_spheres(form=capi.FORM_FAST)
+    cfg, mat = info["cfg"], info["mat"]
+    try:
+        _threads()
+        orc = O.Sim(cfg.nx, cfg.ny, cfg.nz, cfg.h,
+                    [oracle_group(mat, cfg.h ** 3, {"x": info["x"], "v": info["v"], "C": np.zeros((len(info["x"]), 3, 3)),
+                                                    "F": np.broadcast_to(np.eye(3), (len(info["x"]), 3, 3)),
+                                                    "S": np.ones(len(info["x"]))})],
+                    gravity=tuple(cfg.gravity), bc_pad=cfg.bc_pad, bc_mode=cfg.bc_mode)
+        for _ in range(20):
+            sim.step(cfg.dt)
+            assert orc.step(cfg.dt) == 0
+        g, o = device_fields(sim, 0), oracle_fields(orc.groups[0])
+        e = _errors(g, o, ("x", "v", "F"))
+        eS = float(np.abs(g["S"] - o["S"]).max())
+        eC = float(np.abs(g["C"] - o["C"]).max()) / (float(np.abs(o["v"]).max()) / cfg.h)
+        print("snow 1M from start", {k: f"{v:.2e}" for k, v in e.items()}, f"J_p {eS:.2e} C {eC:.2e}")
+        assert e["x"] < 1e-6 and e["v"] < 1e-5 and e["F"] < 1e-6 and eS < 1e-6 and eC < 1e-4
+        assert sim.health() & ~capi.H_ARENA_SPILL == 0
+        _grid_mass_matches(sim, mat.rho * cfg.h ** 3 * sim.total)
+    finally:
+        O.set_threads(1)
+        sim.close()
+
+
+def test_snow_1m_in_collision_against_oracle(gpu):
+    """The same scene from the device state at step 170 (the spheres' stencils
+    met at ~150: plastic flow, hardening, SVDs on the clamp path), 20 more
+    steps on both sides: x within 1e-6, v within 1e-5, F within 1e-6, J_p
+    within 1e-6 (relative L2), C within 3e-4; at least 1% of the particles
+    are plastically compressed (J_p < 1 - 1e-3) by the end."""
+    sim, info = workloads.snow_spheres(form=capi.FORM_FAST)
+    cfg, mat = info["cfg"], info["mat"]
+    try:
+        sim.run(170, cfg.dt)
+        _threads()
+        orc = oracle_from_device(sim, cfg, [mat])
+        jp0 = device_fields(sim, 0)["S"]
+        for _ in range(20):
+            sim.step(cfg.dt)
+            assert orc.step(cfg.dt) == 0
+        g, o = device_fields(sim, 0), oracle_fields(orc.groups[0])
+        e = _errors(g, o, ("x", "v", "C", "F", "S"))
+        print("snow 1M in collision", {k: f"{v:.2e}" for k, v in e.items()},
+              f"plastic particles {np.mean(g['S'] < 1 - 1e-3):.3f} (at step 170 {np.mean(jp0 < 1 - 1e-3):.3f})")
+        assert e["x"] < 1e-6 and e["v"] < 1e-5 and e["F"] < 1e-6 and e["S"] < 1e-6 and e["C"] < 3e-4
+        assert np.mean(g["S"] < 1 - 1e-3) > 0.01
+        assert sim.health() & ~capi.H_ARENA_SPILL == 0
+    finally:
+        O.set_threads(1)
+        sim.close()
+
+
+def test_jello_cube_262k(gpu):
+    """262,146 jello particles on 128^3 dropped at 2 m/s onto the sticky
+    floor: 100 steps against the oracle (x within 5e-6, v within 1e-4, F
+    within 1e-5, the single-scene rollout bars), then all 500 steps: health
+    clear, mass conserved, particles inside the grid, the cube on the
+    floor band (lowest particle within 3 cells above the sticky band, none
+    inside it, the bottom layer slower than 0.5 m/s: it arrived at 2 m/s)."""
+    sim, info = workloads.jello_cube(form=capi.FORM_FAST)
+    cfg, mat = info["cfg"], info["mat"]
+    n = sim.total
+    try:
+        assert n == 262146
+        _threads()
+        orc = oracle_from_device(sim, cfg, [mat])
+        for _ in range(100):
+            sim.step(cfg.dt)
+            assert orc.step(cfg.dt) == 0
+        g, o = device_fields(sim, 0), oracle_fields(orc.groups[0])
+        e = _errors(g, o, ("x", "v", "F"))
+        print("jello cube 100 steps", {k: f"{v:.2e}" for k, v in e.items()})
+        assert e["x"] < 5e-6 and e["v"] < 1e-4 and e["F"] < 1e-5
+        del orc
+        sim.run(400, cfg.dt)
+        assert sim.health() & ~capi.H_ARENA_SPILL == 0
+        _inside(sim, cfg)
+        _grid_mass_matches(sim, mat.rho * cfg.h ** 3 * n)
+        x = sim.read(0, capi.P_POS)
+        v = sim.read(0, capi.P_VEL)
+        pad = cfg.bc_pad * cfg.h
+        ymin = float(x[:, 1].min())
+        print(f"jello after 500 steps: lowest y {ymin / cfg.h:.2f} cells (band {cfg.bc_pad}), "
+              f"bottom-layer |v| {np.abs(v[x[:, 1] < ymin + cfg.h]).max():.3e}")
+        assert pad - cfg.h < ymin < pad + 3 * cfg.h
+        assert np.abs(v[x[:, 1] < ymin + cfg.h]).max() < 0.5
+    finally:
+        O.set_threads(1)
+        sim.close()
+
+
+def test_water_dam_2m(gpu):
+    """2,000,000 water particles on 256^3 (dt = 5e-5): 10 steps against the
+    oracle (x within 5e-6, v within 1e-4, J within 1e-5 absolute), then 200
+    steps: health clear, mass conserved, particles inside the grid, J within
+    (0.5, 2), and the column starting to collapse (centre of mass lower by
+    more than 0.1 mm, the front moving out along x; 210 steps are 10.5 ms of
+    the break)."""
+    sim, info = workloads.water_dam(form=capi.FORM_FAST)
+    cfg, mat = info["cfg"], info["mat"]
+    n = sim.total
+    try:
+        assert n == 2_000_000
+        x0 = sim.read(0, capi.P_POS).astype(np.float64)
+        _threads()
+        orc = oracle_from_device(sim, cfg, [mat])
+        for _ in range(10):
+            sim.step(cfg.dt)
+            assert orc.step(cfg.dt) == 0
+        g, o = device_fields(sim, 0), oracle_fields(orc.groups[0])
+        e = _errors(g, o, ("x", "v"))
+        eJ = float(np.abs(g["S"] - o["S"]).max())
+        print("water dam 10 steps", {k: f"{v:.2e}" for k, v in e.items()}, f"J {eJ:.2e}")
+        assert e["x"] < 5e-6 and e["v"] < 1e-4 and eJ < 1e-5
+        del orc
+        sim.run(200, cfg.dt)
+        assert sim.health() & ~capi.H_ARENA_SPILL == 0
+        _inside(sim, cfg)
+        _grid_mass_matches(sim, mat.rho * cfg.h ** 3 * n)
+        x = sim.read(0, capi.P_POS).astype(np.float64)
+        J = sim.read(0, capi.P_JP)[:, 0]
+        print(f"water after 210 steps: J in [{J.min():.3f}, {J.max():.3f}], centre of mass y "
+              f"{x0[:, 1].mean():.4f} -> {x[:, 1].mean():.4f}, front x {x0[:, 0].max():.4f} -> {x[:, 0].max():.4f}")
+        assert 0.5 < J.min() and J.max() < 2.0
+        assert x[:, 1].mean() < x0[:, 1].mean() - 1e-4
+        assert x[:, 0].max() > x0[:, 0].max()
+    finally:
+        O.set_threads(1)
+        sim.close()
+
+
+def test_batch_64_scenes(gpu):
+    """The 64-scene batch (log-uniform 10k-500k particles, jello / snow /
+    water, 64^3-160^3 grids): 10 batched steps; the smallest and the largest
+    scene of each material against the oracle over the same 10 steps (x
+    within 5e-6, v within 1e-4, F within 1e-5, J_p / J within 1e-5 absolute);
+    every scene healthy, finite and inside its grid, grid mass conserved."""
+    scenes = workloads.batch_scenes()
+    sims = [s for s, _ in scenes]
+    try:
+        assert len(sims) == 64
+        picks = []
+        for t in (capi.JELLO, capi.SNOW, capi.WATER):
+            ks = [k for k, (_, inf) in enumerate(scenes) if inf["type"] == t]
+            ks.sort(key=lambda k: scenes[k][1]["n"])
+            picks += [ks[0], ks[-1]]
+        orcs = {k: oracle_from_device(sims[k], scenes[k][1]["cfg"], [scenes[k][1]["mat"]]) for k in picks}
+        with capi.Batch(sims) as b:
+            b.run(10)
+            b.sync()
+        _threads()
+        for k in picks:
+            inf = scenes[k][1]
+            for _ in range(10):
+                assert orcs[k].step(inf["dt"]) == 0
+            g, o = device_fields(sims[k], 0), oracle_fields(orcs[k].groups[0])
+            e = _errors(g, o, ("x", "v") + (("F",) if inf["type"] != capi.WATER else ()))
+            eS = float(np.abs(g["S"] - o["S"]).max())
+            print(f"scene {k} type {inf['type']} n {inf['n']} grid {inf['grid']}^3",
+                  {a: f"{v:.2e}" for a, v in e.items()}, f"S {eS:.2e}")
+            assert e["x"] < 5e-6 and e["v"] < 1e-4 and e.get("F", 0.0) < 1e-5 and eS < 1e-5
+        for s, inf in scenes:
+            assert s.health() & ~capi.H_ARENA_SPILL == 0
+            assert s.current_frame == 10
+            _inside(s, inf["cfg"])
+            _grid_mass_matches(s, inf["mat"].rho * inf["cfg"].h ** 3 * s.total)
+    finally:
+        O.set_threads(1)
+        for s in sims:
+            s.close()

@@ -180,3 +180,45 @@ def test_batch_refuses_a_controlled_scene(gpu):
     finally:
         a.close()
         b.close()
+
+
+def test_deferred_spill_scatter_takes_the_phase_dt(gpu):
+    """A one-group FAST P2G leaves its spilled particles' atomic scatter to
+    the next grid phase, or to a grid read / particle write before it.  With
+    the controller on, that scatter must use the dt the P2G phase was given
+    (phases ignore the controller), not the controller's device dt: the grid
+    read right after each P2G phase equals the plain form's at that dt, on a
+    fresh simulator (controller never run, device dt 0) and after full
+    controlled steps, on steps where particles spilled."""
+    kw = dict(type_=capi.JELLO, seed=17, randomize=False, dt=1e-4, gravity=(0.0, 0.0, 0.0))
+    a = Pair(form=capi.FORM_PLAIN, **kw)
+    b = Pair(form=capi.FORM_FAST, sort_every=1000, extra_cfg=_cfg(capi.DT_CFL), **kw)
+    try:
+        rng = np.random.default_rng(18)
+        v = np.tile(np.array([[20.0, 5.0, -3.0]], np.float32), (a.n, 1))
+        C = (30.0 * rng.normal(size=(a.n, 9))).astype(np.float32)
+        F = (np.eye(3).reshape(1, 9) + 0.05 * rng.normal(size=(a.n, 9))).astype(np.float32)
+        for p in (a, b):
+            p.gpu.write(0, capi.P_VEL, v)
+            p.gpu.write(0, capi.P_C, C)
+            p.gpu.write(0, capi.P_F, F)
+        spilled = 0
+        for k in range(24):
+            if k == 12:              # full controlled steps in between: the device dt is now set
+                b.gpu.step(MAX_DT)
+                a.gpu.step(b.gpu.last_dt)
+            dt = 1e-4 if k % 2 else 5e-5
+            for p in (a, b):
+                p.gpu.phase(capi.PHASE_P2G, dt)
+            for f in (capi.G_MASS, capi.G_MOMENTUM):
+                e = rel(b.gpu.grid(f), a.gpu.grid(f))
+                assert e < 1e-5, (k, f, e)
+            for p in (a, b):
+                p.gpu.phase(capi.PHASE_GRID, dt)
+                p.gpu.phase(capi.PHASE_G2P, dt)
+            spilled += b.gpu.stats()["spilled"] > 0
+        print("steps with spills:", spilled)
+        assert spilled >= 1
+    finally:
+        a.close()
+        b.close()

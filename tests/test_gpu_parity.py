@@ -154,6 +154,39 @@ def test_one_step_particle_parity(gpu, case):
         p.close()
 
 
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_p2g_force_term_parity(gpu, case):
+    """P2G with v = C = 0: the node momenta are the stress / pressure force
+    alone (snow with J_p in [0.75, 1.05], so the hardening factor
+    exp(10 (1 - J_p)) spans 0.6-12; water with J in [0.7, 1.3], pressure of
+    both signs), device against the oracle.  Bar 1e-5 normwise on the
+    momenta, whose FP32 error is that of the stress."""
+    mat, form = case
+    p = Pair(type_=mat, form=form, seed=31)
+    try:
+        n = p.n
+        rng = np.random.default_rng(32)
+        zero3 = np.zeros((n, 3), np.float32)
+        S = rng.uniform(0.75, 1.05, n) if mat == capi.SNOW else rng.uniform(0.7, 1.3, n)
+        S = S.astype(np.float32)
+        p.gpu.write(0, capi.P_VEL, zero3)
+        p.gpu.write(0, capi.P_C, np.zeros((n, 9), np.float32))
+        p.gpu.write(0, capi.P_JP, S.reshape(n, 1))
+        o = p.orc.groups[0]
+        o.v[:] = 0.0
+        o.C[:] = 0.0
+        if mat == capi.SNOW:
+            o.Fp[:] = np.cbrt(S.astype(np.float64))[:, None, None] * np.eye(3)
+        elif mat == capi.WATER:
+            o.J[:] = S.astype(np.float64)
+        em, ep = _p2g_compare(p)
+        print(MAT_ID[mat], FORM_ID[form], f"force-only grid mass {em:.2e} momentum {ep:.2e}")
+        assert em < TOL_GRID and ep < TOL_GRID, (em, ep)
+        assert np.linalg.norm(p.orc.grid_vec()) > 0
+    finally:
+        p.close()
+
+
 ROLLOUT = [(m, f, s) for m in MATS for f in FORMS for s in ("rest", "perturbed")]
 
 

@@ -10,6 +10,10 @@ from tests.scenes import Pair, rel
 
 pytestmark = pytest.mark.gpu
 
+# largest share of warm-started SVDs whose start the acceptance test rejects
+# (measured on the MI355X: jello 0.66, snow 0.41 with the network below)
+WARM_REJECT_BAR = {capi.JELLO: 0.8, capi.SNOW: 0.55}
+
 
 def _deformations(n, seed, spread=0.3):
     rng = np.random.default_rng(seed)
@@ -85,26 +89,80 @@ def test_trained_warm_start_gain(gpu):
     assert np.median(off_w) < np.median(off_c)
 
 
+def _out_of_band(n, seed, frac, lo, hi):
+    """F = U diag(s) V^T with s uniform in [lo, hi] for a fraction `frac` of
+    n particles (the rest identity), FP32 row-major."""
+    rng = np.random.default_rng(seed)
+    F = np.tile(np.eye(3), (n, 1, 1))
+    pick = rng.random(n) < frac
+    m = int(pick.sum())
+    U, _ = np.linalg.qr(rng.normal(size=(m, 3, 3)))
+    V, _ = np.linalg.qr(rng.normal(size=(m, 3, 3)))
+    U *= np.sign(np.linalg.det(U))[:, None, None]
+    V *= np.sign(np.linalg.det(V))[:, None, None]
+    F[pick] = U * rng.uniform(lo, hi, (m, 1, 3)) @ V.transpose(0, 2, 1)
+    return F.reshape(n, 9).astype(np.float32)
+
+
+def _svd_calls(sim, mat, n):
+    """Particles of the step just taken whose stress or plasticity took the
+    (warm-started) SVD: snow particles whose J_p changed (the clamp), jello
+    particles whose new F has a singular value outside (0.5, 2) (outside
+    the Newton polar path)."""
+    if mat == capi.SNOW:
+        return None
+    F = sim.read(0, capi.P_F).reshape(n, 3, 3).astype(np.float64)
+    s = np.linalg.svd(F, compute_uv=False)
+    return int(np.count_nonzero((s.min(1) <= 0.5) | (s.max(1) >= 2.0)))
+
+
 @pytest.mark.parametrize("mat", [capi.JELLO, capi.SNOW], ids=["jello", "snow"])
 def test_simulation_with_warm_start(gpu, mat):
-    """FAST form with the warm start (two sweeps, cold fallback) matches the
-    cold-start run within the FP32 parity bars; rejections are counted."""
+    """FAST form with the warm start (two sweeps, cold fallback) against the
+    cold start, with a third of the particles deformed so that the warm
+    path runs: snow trial F outside the clamp band (singular values in
+    [0.93, 1.03] against [0.975, 1.0075]), jello F outside (0.5, 2) (the
+    stress takes the SVD instead of the Newton polar rotation).  Fields
+    match the cold run within the FP32 parity bars; the warm-started SVD ran
+    for at least a tenth of the particle-steps and the network's start was
+    rejected (cold SVD redone) for at most the stated share of them."""
     F = _deformations(20000, 11, spread=0.05)
     p, _ = capi.mlp_train(F, M.init_params(12), steps=300, batch=2048, lr=2e-3, seed=3)
     a = Pair(type_=mat, form=capi.FORM_FAST, seed=21)
     b = Pair(type_=mat, form=capi.FORM_FAST, seed=21, extra_cfg={"warm_start": capi.WARM_MLP, "warm_sweeps": 2})
     try:
+        n = b.n
+        F0 = _out_of_band(n, 22, 1 / 3, 0.93, 1.03) if mat == capi.SNOW else _out_of_band(n, 22, 1 / 3, 0.3, 0.45)
+        if mat == capi.JELLO:
+            big = np.random.default_rng(23).random(n) < 0.5         # half of them stretched instead
+            F0 = F0.reshape(n, 3, 3)
+            s = np.linalg.svd(F0.astype(np.float64), compute_uv=False)
+            sel = big & (s.max(1) < 0.5)
+            F0[sel] *= 6.0                                          # [0.3, 0.45] -> [1.8, 2.7]
+            F0 = F0.reshape(n, 9).astype(np.float32)
+        for q in (a, b):
+            q.gpu.write(0, capi.P_F, F0)
         b.gpu.set_warm_start_weights(p)
+        calls = 0
+        jp = b.gpu.read(0, capi.P_JP)[:, 0].copy()
         for _ in range(5):
             a.gpu.step(a.dt)
             b.gpu.step(b.dt)
+            if mat == capi.SNOW:
+                j = b.gpu.read(0, capi.P_JP)[:, 0]
+                calls += int(np.count_nonzero(j != jp))
+                jp = j.copy()
+            else:
+                calls += _svd_calls(b.gpu, mat, n)
         ga, gb = a.gpu_particles(), b.gpu_particles()
         for k in ("x", "v"):
             assert rel(gb[k], ga[k]) < 1e-5, k
         assert rel(gb["F"] - np.eye(3), ga["F"] - np.eye(3)) < 1e-4
-        st = b.gpu.stats()
-        print("warm rejections", st["warm_rejects"], "of", 5 * b.n)
-        assert st["warm_rejects"] <= 5 * b.n
+        rejects = b.gpu.stats()["warm_rejects"]
+        frac = rejects / max(calls, 1)
+        print(f"warm-started SVDs {calls} of {5 * n} particle-steps, rejected {rejects} ({frac:.3f})")
+        assert calls >= 0.1 * 5 * n
+        assert frac <= WARM_REJECT_BAR[mat], frac
         assert b.gpu.health() & ~capi.H_ARENA_SPILL == 0
     finally:
         a.close()
