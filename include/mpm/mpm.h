@@ -329,6 +329,19 @@ int mpm_probe_svd_residual(const float *F, const float *q, int64_t n, int32_t sw
  * the sorted keys and the permutation (sorted[i] = keys[perm[i]]). */
 int mpm_probe_sort(const uint32_t *keys, int64_t n, int32_t bits, uint32_t *sorted, uint32_t *perm);
 
+/* The sort of mpm_probe_sort on device memory, allocating nothing: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  keys, sorted and perm are device pointers to n words
+ * (keys may equal sorted); the work is queued on `stream` (a hipStream_t,
+ * NULL for the null stream) and is complete when the stream reaches it.
+ * Its temporaries are one fixed device array per GPU, sized for
+ * MPM_SORT_NOALLOC_MAX_KEYS keys: larger n is refused (MPM_ERR_RANGE), and
+ * calls that overlap on one GPU (other streams, other threads) race on it. */
+#define MPM_SORT_NOALLOC_MAX_KEYS (1 << 20)
+int mpm_sort_keys_noalloc(const uint32_t *keys, int64_t n, int32_t bits, uint32_t *sorted, uint32_t *perm,
+                          void *stream);
+
 /* ---- Batches of independent scenes ------------------------------------
  * No counterpart in the reference, which steps one MPMSimulator per process
  * (main.cpp:202-221).  For callers with many small scenes: every scene of a

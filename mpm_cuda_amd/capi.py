@@ -114,6 +114,7 @@ _SIGS = {
     "mpm_probe_base_index": (C.c_int, [C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
     "mpm_probe_svd": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_sort": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
+    "mpm_sort_keys_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "mpm_probe_stress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
                                    C.c_int32, C.c_void_p]),
@@ -608,6 +609,15 @@ def probe_sort(keys, bits: int):
     check(lib().mpm_probe_sort(keys.ctypes.data_as(C.c_void_p), n, int(bits), srt.ctypes.data_as(C.c_void_p),
                                perm.ctypes.data_as(C.c_void_p)))
     return srt, perm
+
+
+SORT_NOALLOC_MAX_KEYS = 1 << 20
+
+
+def sort_keys_noalloc(keys_ptr: int, n: int, bits: int, sorted_ptr: int, perm_ptr: int, stream: int = 0) -> None:
+    """mpm_sort_keys_noalloc on device pointers (queued on `stream`)."""
+    check(lib().mpm_sort_keys_noalloc(C.c_void_p(keys_ptr), int(n), int(bits), C.c_void_p(sorted_ptr),
+                                      C.c_void_p(perm_ptr), C.c_void_p(stream)))
 
 
 def write_bgeo(path: str, xyz) -> None:

@@ -22,6 +22,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <thread>
 
 namespace mpm {
@@ -1456,6 +1457,81 @@ int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, 
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Sort that allocates nothing (mpm_sort_keys_noalloc): the ping-pong key and
+// value buffers, the digit histograms and the scan partials of the radix
+// sort live in one fixed device array, `scratch`, sized for
+// MPM_SORT_NOALLOC_MAX_KEYS keys (about 8.9 MB, held by every GPU the
+// library runs on until the process ends).  Host code never passes the
+// array's name to a kernel: each device's address of it is looked up once
+// (hipGetSymbolAddress with that device current) and kept per device.
+// Nothing in it needs to start at zero -- the histogram kernel writes every
+// entry it later reads, and so do the scan kernels -- but every call on one
+// GPU shares it, so calls that overlap on one GPU (different streams or
+// threads) race.
+constexpr int64_t NA_KEYS = MPM_SORT_NOALLOC_MAX_KEYS;
+constexpr int64_t NA_TILES = (NA_KEYS + RS_TILE - 1) / RS_TILE;
+constexpr int64_t NA_HIST = 512 * NA_TILES;                              // digits of up to 9 bits
+constexpr int64_t NA_PARTIAL = (NA_HIST + SCAN_TILE - 1) / SCAN_TILE + 16;
+constexpr int64_t NA_WORDS = 2 * NA_KEYS + NA_HIST + NA_PARTIAL;
+static __device__ __align__(256) uint32_t scratch[NA_WORDS];
+
+constexpr int NA_MAX_DEVICES = 64;
+static uint32_t *na_scratch_ptr[NA_MAX_DEVICES];
+static std::once_flag na_once[NA_MAX_DEVICES];
+static hipError_t na_lookup_rc[NA_MAX_DEVICES];
+
+// This device's address of `scratch` (looked up on the first call on it).
+static int noalloc_scratch(uint32_t **ptr)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return set_error(MPM_ERR_INVALID, "device index beyond the scratch table");
+    std::call_once(na_once[dev], [dev] {
+        void *p = nullptr;
+        na_lookup_rc[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        na_scratch_ptr[dev] = na_lookup_rc[dev] == hipSuccess ? (uint32_t *)p : nullptr;
+    });
+    if (na_lookup_rc[dev] != hipSuccess) return hip_fail(na_lookup_rc[dev], "hipGetSymbolAddress(scratch)");
+    *ptr = na_scratch_ptr[dev];
+    return MPM_OK;
+}
+
+__global__ void k_iota(uint32_t *v, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+int sort_keys_noalloc(const uint32_t *keys, int64_t n, int bits, uint32_t *sorted, uint32_t *perm, hipStream_t st)
+{
+    if (n < 0) return set_error(MPM_ERR_INVALID, "negative key count");
+    if (n > NA_KEYS) return set_error(MPM_ERR_RANGE, "more keys than the no-allocation sort holds (MPM_SORT_NOALLOC_MAX_KEYS)");
+    if (bits < 1 || bits > 32) return set_error(MPM_ERR_INVALID, "bits out of range [1,32]");
+    if (n == 0) return MPM_OK;
+    if (!keys || !sorted || !perm) return set_error(MPM_ERR_INVALID, "null argument");
+    uint32_t *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    uint32_t *ka = base, *va = base + NA_KEYS, *hist = base + 2 * NA_KEYS, *partial = hist + NA_HIST;
+    hipError_t e = hipSuccess;
+    if (keys != sorted) e = hipMemcpyAsync(sorted, keys, (size_t)n * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "copy keys");
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, perm, (int)n);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "iota");
+    bool in_alt = false;
+    rc = radix_sort(st, sorted, perm, ka, va, (int)n, bits, hist, partial, in_alt);
+    if (rc) return hip_fail((hipError_t)rc, "radix_sort");
+    if (in_alt) {
+        if ((e = hipMemcpyAsync(sorted, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+            return hip_fail(e, "copy sorted keys");
+        if ((e = hipMemcpyAsync(perm, va, (size_t)n * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+            return hip_fail(e, "copy permutation");
+    }
+    return MPM_OK;
+}
+
 // Batch descriptor of a one-group simulator in its current particle order.
 void fill_batch_slice(mpm_sim *s, BatchSlice &b)
 {
@@ -1509,3 +1585,9 @@ int launch_g2p_fast(mpm_sim *s, int group, float dt)
 }
 
 }  // namespace mpm
+
+extern "C" int mpm_sort_keys_noalloc(const uint32_t *keys, int64_t n, int32_t bits, uint32_t *sorted, uint32_t *perm,
+                                     void *stream)
+{
+    return mpm::sort_keys_noalloc(keys, n, bits, sorted, perm, (hipStream_t)stream);
+}
