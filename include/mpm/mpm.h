@@ -302,6 +302,15 @@ int mpm_probe_point_in_mesh(const char *obj_path, const double *pts, int64_t n, 
  * launches after one warm-up, in GB/s. */
 int mpm_probe_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs);
 
+/* LDS scatter rate on the device: 8,192 one-wave workgroups each add, for
+ * reps x 64 particles, a 27-node x 4-float P2G stencil into an 8 KB LDS
+ * arena; *ms is the launch's duration.  mode 0: float atomics (ds_add_f32),
+ * every lane of an instruction on its own address; 1: float atomics, hashed
+ * cells (collisions); 2: plain float4 read-modify-write, distinct cells; 3:
+ * plain read-modify-write, hashed cells (a rate only: colliding lanes
+ * overwrite each other). */
+int mpm_probe_lds_scatter(int32_t mode, int32_t reps, double *ms);
+
 /* Device FP32 base-node index max(0, floor((x - origin) * inv_h - 0.5)) of
  * n coordinates, as used by P2G / G2P / the sort keys. */
 int mpm_probe_base_index(const float *x, int64_t n, float origin, float inv_h, int32_t *out);

@@ -116,6 +116,7 @@ _SIGS = {
     "mpm_probe_sort": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "mpm_sort_keys_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "mpm_probe_lds_scatter": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "mpm_probe_stress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
                                    C.c_int32, C.c_void_p]),
     "mpm_batch_shape_default": (None, [C.POINTER(MpmBatchShape)]),
@@ -600,6 +601,18 @@ def probe_copy_bandwidth(bytes_: int = 1 << 30, reps: int = 10) -> float:
     g = C.c_double()
     check(lib().mpm_probe_copy_bandwidth(int(bytes_), int(reps), C.byref(g)))
     return g.value
+
+
+LDS_PROBE_MODES = {0: "float atomics, distinct cells", 1: "float atomics, hashed cells",
+                   2: "float4 read-modify-write, distinct cells", 3: "float4 read-modify-write, hashed cells"}
+
+
+def probe_lds_scatter(mode: int, reps: int = 256) -> float:
+    """ms of one mpm_probe_lds_scatter launch (8,192 one-wave workgroups,
+    reps x 64 particles each, 27 nodes x 4 floats per particle)."""
+    ms = C.c_double()
+    check(lib().mpm_probe_lds_scatter(int(mode), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def probe_sort(keys, bits: int):

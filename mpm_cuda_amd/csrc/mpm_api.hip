@@ -516,16 +516,14 @@ int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
             std::vector<uint4> desc(nt);
             MPM_HIP(hipMemcpy(desc.data(), (const uint4 *)s->tile_desc + s->group_tile_off[k], nt * sizeof(uint4),
                               hipMemcpyDeviceToHost));
-            std::vector<float4> arena(STAGE_STRIDE);
-            const uint32_t *mask = (const uint32_t *)(arena.data() + ARENA_NODES);
+            std::vector<float4> arena(ARENA_NODES);
             for (uint32_t t = 0; t < nt; ++t) {
                 if (desc[t].w >= (uint32_t)s->staging_cap) continue;
-                MPM_HIP(hipMemcpy(arena.data(), s->staging + (int64_t)desc[t].w * STAGE_STRIDE,
-                                  STAGE_STRIDE * sizeof(float4), hipMemcpyDeviceToHost));
+                MPM_HIP(hipMemcpy(arena.data(), s->staging + (int64_t)desc[t].w * ARENA_NODES,
+                                  ARENA_NODES * sizeof(float4), hipMemcpyDeviceToHost));
                 const int b = (int)desc[t].z;
                 const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
                 for (int q = 0; q < ARENA_NODES; ++q) {
-                    if (!((mask[q >> 5] >> (q & 31)) & 1u)) continue;     // no mass staged here
                     const int x = bx * BLK - 1 + (q & 7), y = by * BLK - 1 + ((q >> 3) & 7), z = bz * BLK - 1 + (q >> 6);
                     if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
                     float4 &dst = mp[node_slot(g, x, y, z)];

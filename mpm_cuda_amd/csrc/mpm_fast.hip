@@ -407,19 +407,10 @@ struct TileLds {
     uint32_t nitems;
 };
 
-// Arena layout of the FAST tile: rows of 12 slots (8 nodes + 4 of padding),
-// layers of 8 rows, so the 27 node offsets of a stencil are constants added
-// to the base slot (immediate offsets of the LDS instructions) and the 8
-// lanes one LDS cycle serves -- consecutive bins: four cells of a row and
-// four of the next -- fall on 8 different 16-byte bank groups (row stride
-// 12 = 4 mod 8).
-constexpr int FA_ROW = 12, FA_LAYER = FA_ROW * ARENA, FA_SLOTS = FA_LAYER * ARENA;
-__device__ __forceinline__ int fa_slot(int x, int y, int z) { return x + FA_ROW * y + FA_LAYER * z; }
-
 // LDS of a FAST-form tile (one wave): bin keys, the sort and the arena
-// (16.5 KB; the particle records stay in HBM / cache).
+// (12.5 KB; the particle records stay in HBM / cache).
 struct TileKeys {
-    float4 arena[FA_SLOTS];
+    float4 arena[A_SLOTS];
     uint32_t items[CHUNK];
     uint16_t perm[CHUNK];
     uint16_t rnk[CHUNK];
@@ -744,14 +735,6 @@ __device__ __forceinline__ void accum_particle(const TileLds &L, int slot, float
 // arena in a fixed order (compiler barrier; LDS executes one wave's
 // operations in order).  Lanes of different bins touch different slots in
 // each instruction; lanes of the same bin take turns.
-struct XorArena {
-    static __device__ __forceinline__ int slot(int x, int y, int z) { return a_slot(x, y, z); }
-};
-struct PaddedArena {
-    static __device__ __forceinline__ int slot(int x, int y, int z) { return fa_slot(x, y, z); }
-};
-
-template <class Layout>
 __device__ __forceinline__ void flush_item(float4 *arena, bool has, uint32_t bin, const float4 acc[27])
 {
     const int lane = threadIdx.x & 63;
@@ -768,7 +751,7 @@ __device__ __forceinline__ void flush_item(float4 *arena, bool has, uint32_t bin
                 for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
                     for (int ox = 0; ox < 3; ++ox) {
-                        const int a = Layout::slot(bx + ox, by + oy, bz + oz);
+                        const int a = a_slot(bx + ox, by + oy, bz + oz);
                         const float4 &c = acc[ox + 3 * oy + 9 * oz];
                         float4 v = arena[a];
                         v.x += c.x; v.y += c.y; v.z += c.z; v.w += c.w;
@@ -799,7 +782,7 @@ __device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int bat
     const uint32_t rounds = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
     for (uint32_t r = 0; r < rounds; ++r)
         if (r < cnt) accum_particle<MAT>(L, L.perm[first + r], inv_h, m, acc);
-    flush_item<XorArena>(arena, has, bin, acc);
+    flush_item(arena, has, bin, acc);
 }
 
 // Phase C for one wave, FAST form: the item's particles are loaded again
@@ -821,29 +804,25 @@ __device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena,
     // item counts never increase along the item list (build_items), so the
     // batch's first lane holds its largest
     const uint32_t rounds = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
-    // the next round's particle is loaded while this one is accumulated
-    // (two register sets taking turns, no copies); lanes without a next
-    // particle reload a valid one (one basic block)
+    // the next round's particle is loaded while this one is accumulated;
+    // lanes without a next particle reload a valid one (one basic block)
     auto slot_of = [&](uint32_t r) -> int { return cnt ? (int)L.perm[first + min(r, cnt - 1u)] : 0; };
-    auto accumulate = [&](const RawP &raw) {
-        Scatter sc;
-        scatter_from_raw<MAT>(raw, mat, dt, sc);
-        Stencil st;
-        make_stencil(g, raw.x, st);
-        float q0[3], hB[9], Gs[6];
-        make_record<MAT>(g, st, sc, q0, hB, Gs);
-        accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
-    };
-    RawP ra, rb;
-    load_raw<MAT>(pv, i0 + slot_of(0), ra);
-    for (uint32_t r = 0; r < rounds; r += 2) {
-        if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), rb);
-        if (r < cnt) accumulate(ra);
-        if (r + 1 >= rounds) break;
-        if (r + 2 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 2), ra);
-        if (r + 1 < cnt) accumulate(rb);
+    RawP nxt;
+    load_raw<MAT>(pv, i0 + slot_of(0), nxt);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const RawP raw = nxt;
+        if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), nxt);
+        if (r < cnt) {
+            Scatter sc;
+            scatter_from_raw<MAT>(raw, mat, dt, sc);
+            Stencil st;
+            make_stencil(g, raw.x, st);
+            float q0[3], hB[9], Gs[6];
+            make_record<MAT>(g, st, sc, q0, hB, Gs);
+            accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
+        }
     }
-    flush_item<PaddedArena>(arena, has, bin, acc);
+    flush_item(arena, has, bin, acc);
 }
 
 // Phases B and C of one chunk (all threads; ends synchronized).
@@ -872,9 +851,7 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 {
     __syncthreads();
     if (staged) {
-        const int64_t dst = (int64_t)slot * STAGE_STRIDE;
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        uint32_t *mask = (uint32_t *)(staging + dst + ARENA_NODES);
+        const int64_t dst = (int64_t)slot * ARENA_NODES;
         for (int k = threadIdx.x; k < ARENA_NODES; k += TILE_THREADS) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
             const int a = a_slot(kx, ky, kz);
@@ -884,16 +861,11 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
                 const float4 u = L.arena[w][a];
                 v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
             }
-            const bool nz = v.x != 0.f;      // no mass, no momentum (every weight and gradient factor vanishes)
-            if (nz) gstore4(staging, dst + k, v);
-            // nodes k - lane .. k - lane + 63 are one 64-bit mask word
-            const uint64_t m = __ballot(nz);
-            if (lane < 2) gp(mask)[((k - lane) >> 5) + lane] = (uint32_t)(m >> (32 * lane));
+            gstore4(staging, dst + k, v);
             const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
-            if (nz && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
                 gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
         }
-        (void)wave;
     }
     __syncthreads();
 }
@@ -914,7 +886,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     const TileGeo t = tile_geo(g, d.z);
     const bool staged = d.w < staging_cap;
     const int lane = threadIdx.x & 63;
-    for (int k = lane; k < FA_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = lane; k < A_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
         const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
         // A: all position loads of the chunk first (lanes past its end reload
@@ -945,17 +917,13 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     // the arena into the tile's staging slot (logical node order); mark the
     // node blocks that received mass
     if (staged) {
-        const int64_t dst = (int64_t)d.w * STAGE_STRIDE;
-        uint32_t *mask = (uint32_t *)(staging + dst + ARENA_NODES);
+        const int64_t dst = (int64_t)d.w * ARENA_NODES;
         for (int k = lane; k < ARENA_NODES; k += 64) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
-            const float4 v = L.arena[fa_slot(kx, ky, kz)];
-            const bool nz = v.x != 0.f;      // no mass, no momentum (every weight and gradient factor vanishes)
-            if (nz) gstore4(staging, dst + k, v);
-            const uint64_t m = __ballot(nz);
-            if (lane < 2) gp(mask)[((k - lane) >> 5) + lane] = (uint32_t)(m >> (32 * lane));
+            const float4 v = L.arena[a_slot(kx, ky, kz)];
+            gstore4(staging, dst + k, v);
             const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
-            if (nz && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
                 gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
         }
     }
@@ -1158,7 +1126,7 @@ static int ensure_fast_buffers(mpm_sim *s)
         MPM_HIP(hipMalloc(&s->tile_desc, need * sizeof(uint4)));
         // every non-empty block may need a staging slot; blocks beyond the cap scatter with atomics
         int64_t scap = std::min<int64_t>(need, (int64_t)1 << 19);
-        MPM_HIP(hipMalloc(&s->staging, scap * STAGE_STRIDE * sizeof(float4)));
+        MPM_HIP(hipMalloc(&s->staging, scap * ARENA_NODES * sizeof(float4)));
         MPM_HIP(hipMalloc(&s->blk_tile, (int64_t)s->groups.size() * s->nblocks * sizeof(int32_t)));
         s->staging_cap = scap;
         s->tile_cap = need;

@@ -17,7 +17,7 @@ constexpr int A_SLOTS = ARENA_NODES;
 __device__ __forceinline__ int a_slot(int x, int y, int z) { return (x ^ ((y & 1) << 2)) + ARENA * (y + ARENA * z); }
 
 struct GlobalFetch {
-    static constexpr bool kArena = false;
+    static constexpr bool kArena = false, kGlobal = true;
     const float4 *gv;
     __device__ __forceinline__ float4 operator()(const GridP &g, int x, int y, int z) const
     {
@@ -25,9 +25,35 @@ struct GlobalFetch {
     }
 };
 
+// Block-major slots are separable: slot(x, y, z) = X(x) + Y(y) + Z(z) with
+// X(x) = 64 (x >> 2) + (x & 3), Y(y) = 64 nbx (y >> 2) + 4 (y & 3),
+// Z(z) = 64 nbx nby (z >> 2) + 16 (z & 3) (node_slot expanded), so a
+// stencil's 27 slots come from 9 per-axis terms and 27 adds (32-bit: a grid
+// holds fewer than 2^32 node slots).
+struct StencilSlots {
+    uint32_t xy[9];     // X(b0 + ox) + Y(b1 + oy), index ox + 3 oy
+    uint32_t z[3];      // Z(b2 + oz)
+    __device__ __forceinline__ void init(const GridP &g, const int b[3])
+    {
+        uint32_t X[3], Y[3];
+        const uint32_t ry = 64u * (uint32_t)g.nbx, rz = ry * (uint32_t)g.nby;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            const uint32_t x = (uint32_t)(b[0] + o), y = (uint32_t)(b[1] + o), zz = (uint32_t)(b[2] + o);
+            X[o] = 64u * (x >> 2) + (x & 3u);
+            Y[o] = ry * (y >> 2) + 4u * (y & 3u);
+            z[o] = rz * (zz >> 2) + 16u * (zz & 3u);
+        }
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) xy[ox + 3 * oy] = X[ox] + Y[oy];
+    }
+};
+
 // Velocities of the block arena [a, a+8)^3 from LDS, other nodes from HBM.
 struct ArenaFetch {
-    static constexpr bool kArena = true;
+    static constexpr bool kArena = true, kGlobal = false;
     const float4 *lds;
     const float4 *gv;
     int ax, ay, az;
@@ -48,6 +74,7 @@ struct ArenaFetch {
 // The arena only, for stencils known to lie inside it: no per-node test,
 // so the 27 LDS reads issue back to back.
 struct ArenaOnlyFetch {
+    static constexpr bool kArena = true, kGlobal = false;
     const float4 *lds;
     int ax, ay, az;
     __device__ __forceinline__ float4 operator()(const GridP &, int x, int y, int z) const
@@ -76,6 +103,8 @@ __device__ __forceinline__ void gather_grid(const GridP &g, const Stencil &s, co
     for (int r = 0; r < 3; ++r) v[r] = -0.0f;     // -0 + x == x: the first add folds away
 #pragma unroll
     for (int k = 0; k < 9; ++k) { M[k] = -0.0f; L[k] = -0.0f; }
+    StencilSlots sl;
+    if constexpr (Fetch::kGlobal) sl.init(g, s.b);
 #pragma unroll
     for (int ox = 0; ox < 3; ++ox) {
         float Y0[3] = {-0.f, -0.f, -0.f}, Y1[3] = {-0.f, -0.f, -0.f}, Y2[3] = {-0.f, -0.f, -0.f};
@@ -85,7 +114,9 @@ __device__ __forceinline__ void gather_grid(const GridP &g, const Stencil &s, co
             float S0[3] = {-0.f, -0.f, -0.f}, S1[3] = {-0.f, -0.f, -0.f}, S2[3] = {-0.f, -0.f, -0.f};
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
-                const float4 nv = fetch(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz);
+                float4 nv;
+                if constexpr (Fetch::kGlobal) nv = fetch.gv[sl.xy[ox + 3 * oy] + sl.z[oz]];
+                else nv = fetch(g, s.b[0] + ox, s.b[1] + oy, s.b[2] + oz);
                 const float vi[3] = {nv.x, nv.y, nv.z};
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {

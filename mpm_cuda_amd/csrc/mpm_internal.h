@@ -37,12 +37,6 @@ constexpr int BLK_NODES = 64;
 // own 4, one below and three above), staged in logical (z, y, x) order.
 constexpr int ARENA = 8;
 constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
-// A staging slot holds a tile's arena in logical node order followed by a
-// 512-bit mask of the nodes that received mass (16 words in the last 4
-// float4s): only those nodes are written, so the rest of the slot keeps
-// whatever an earlier step left there, and every reader tests the mask.
-constexpr int STAGE_MASK_F4 = ARENA_NODES / 128;
-constexpr int STAGE_STRIDE = ARENA_NODES + STAGE_MASK_F4;      // float4 per staging slot
 
 struct GridP {
     int nx, ny, nz;

@@ -82,6 +82,57 @@ __global__ void __launch_bounds__(256) k_probe_copy(const float4 *__restrict__ s
         gstore4(dst, i, gload4(src, i));
 }
 
+// LDS scatter probe: one-wave workgroups each add, for `reps` x 64
+// particles, 27 node updates of 4 floats (a particle's P2G stencil in a 4^3
+// block) into an 8 KB arena.  Lane cells: mode bit 0 clear -- lane l owns
+// cell l (no two lanes on one address in an instruction); set -- a hashed
+// cell (collisions).  Mode bit 1 clear: float atomics (ds_add_f32, SoA
+// arena); set: plain float4 read-modify-write (lanes of one cell overwrite
+// each other's adds: a rate measurement, the sums are not used).  Every
+// arena index is below 4 * 512.
+constexpr int LP_NODES = 512;
+__device__ __forceinline__ uint32_t lp_hash(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ int lp_node(int cell, int o)
+{
+    const int cx = cell & 3, cy = (cell >> 2) & 3, cz = cell >> 4;
+    return (cx + o % 3 + 1) + 8 * (cy + (o / 3) % 3 + 1) + 64 * (cz + o / 9 + 1);     // < 438
+}
+template <bool ATOMIC, bool HASHED>
+__global__ void __launch_bounds__(64) k_probe_lds(float *out, int reps)
+{
+    __shared__ float A[4 * LP_NODES];
+    const int lane = threadIdx.x;
+    for (int k = lane; k < 4 * LP_NODES; k += 64) A[k] = 0.f;
+    __syncthreads();
+    float v = 1.0f + lane * 1e-3f;
+    for (int r = 0; r < reps; ++r) {
+        const int cell = HASHED ? (int)(lp_hash(lane + 64u * (r + reps * blockIdx.x)) & 63u) : lane;
+#pragma unroll
+        for (int o = 0; o < 27; ++o) {
+            const int n = lp_node(cell, o);
+            if (ATOMIC) {
+                atomicAdd(&A[n], v);
+                atomicAdd(&A[LP_NODES + n], 0.5f * v);
+                atomicAdd(&A[2 * LP_NODES + n], 0.25f * v);
+                atomicAdd(&A[3 * LP_NODES + n], 0.125f * v);
+            } else {
+                float4 *A4 = (float4 *)A;
+                float4 a = A4[n];
+                a.x += v; a.y += 0.5f * v; a.z += 0.25f * v; a.w += 0.125f * v;
+                A4[n] = a;
+                __asm__ volatile("" ::: "memory");
+            }
+        }
+        v += 1e-6f;
+    }
+    __syncthreads();
+    for (int k = lane; k < 4 * LP_NODES; k += 64) out[(int64_t)blockIdx.x * 4 * LP_NODES + k] = A[k];
+}
+
 // RAII set of device temporaries for the probes.
 struct DevBufs {
     std::vector<void *> p;
@@ -145,6 +196,41 @@ int mpm_probe_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs)
     std::sort(ms.begin(), ms.end());
     const double med = ms[ms.size() / 2];
     *gbs = 2.0 * (double)bytes / (med * 1e-3) / 1e9;
+    return MPM_OK;
+}
+
+int mpm_probe_lds_scatter(int32_t mode, int32_t reps, double *ms)
+{
+    if (!ms) return set_error(MPM_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 3) return set_error(MPM_ERR_INVALID, "mode out of range [0,3]");
+    if (reps < 1 || reps > 4096) return set_error(MPM_ERR_INVALID, "reps out of range [1,4096]");
+    constexpr int wgs = 8192;
+    DevBufs b;
+    float *out = nullptr;
+    MPM_HIP(b.alloc(&out, (size_t)wgs * 4 * LP_NODES * sizeof(float)));
+    auto launch = [&](int rr) {
+        switch (mode) {
+        case 0: hipLaunchKernelGGL((k_probe_lds<true, false>), dim3(wgs), dim3(64), 0, 0, out, rr); break;
+        case 1: hipLaunchKernelGGL((k_probe_lds<true, true>), dim3(wgs), dim3(64), 0, 0, out, rr); break;
+        case 2: hipLaunchKernelGGL((k_probe_lds<false, false>), dim3(wgs), dim3(64), 0, 0, out, rr); break;
+        default: hipLaunchKernelGGL((k_probe_lds<false, true>), dim3(wgs), dim3(64), 0, 0, out, rr); break;
+        }
+        return hipGetLastError();
+    };
+    MPM_HIP(launch(1));                                   // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MPM_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    float t = 0.f;
+    if (err == hipSuccess) err = hipEventRecord(e0, 0);
+    if (err == hipSuccess) err = launch(reps);
+    if (err == hipSuccess) err = hipEventRecord(e1, 0);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    MPM_HIP(err);
+    *ms = t;
     return MPM_OK;
 }
 

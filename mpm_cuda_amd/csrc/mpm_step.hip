@@ -123,26 +123,19 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
             }
             // A node lies in exactly 8 of the 27 arenas: per axis its own
             // block's (d = 0) and the one below (d = -1, local k <= 2) or
-            // above (d = +1, k = 3).  The 8 mask words are loaded first, then
-            // the nodes their bits mark (every load of a stage issued before
-            // any is used).
+            // above (d = +1, k = 3).  All 8 loads are issued before any is
+            // used.
             const int sx = kx == 3 ? 1 : -1, sy = ky == 3 ? 1 : -1, sz = kz == 3 ? 1 : -1;
-            int64_t at[8];
-            uint32_t bit[8];
+            float4 v[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int dx = (e & 1) ? sx : 0, dy = (e & 2) ? sy : 0, dz = (e & 4) ? sz : 0;
                 const int32_t t = __shfl(my_t, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
                 const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
-                const int q = (az * ARENA + ay) * ARENA + ax;
                 const int64_t ts = t >= 0 ? t : 0;
-                at[e] = ts * STAGE_STRIDE + q;
-                const uint32_t *mask = (const uint32_t *)(staging + ts * STAGE_STRIDE + ARENA_NODES);
-                bit[e] = t >= 0 ? (gp(mask)[q >> 5] >> (q & 31)) & 1u : 0u;
+                v[e] = gload4(staging, ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax);
+                if (t < 0) v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            float4 v[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = bit[e] ? gload4(staging, at[e]) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int e = 0; e < 8; ++e) { a.x += v[e].x; a.y += v[e].y; a.z += v[e].z; a.w += v[e].w; }
         }

@@ -397,6 +397,21 @@ def test_copy_bandwidth_probe(gpu):
         capi.probe_copy_bandwidth(1 << 20, 0)
 
 
+def test_lds_scatter_probe(gpu):
+    """The LDS scatter probe behind the P2G design: every mode runs and the
+    float atomics (ds_add_f32) are slower than a plain float4
+    read-modify-write of the same stencil; its argument checks."""
+    ms = {m: min(capi.probe_lds_scatter(m, 64) for _ in range(2)) for m in capi.LDS_PROBE_MODES}
+    per = {m: t * 1e-3 * 2.4e9 * 256 / (8192 * 64 * 64) for m, t in ms.items()}
+    for m, t in ms.items():
+        print(f"mode {m} ({capi.LDS_PROBE_MODES[m]}): {t:.3f} ms, {per[m]:.1f} CU-clk/particle at 2.4 GHz")
+    assert all(t > 0.0 for t in ms.values())
+    assert ms[0] > ms[2]
+    for bad in ((4, 64), (-1, 64), (0, 0), (0, 5000)):
+        with pytest.raises(capi.MpmError):
+            capi.probe_lds_scatter(*bad)
+
+
 @pytest.mark.parametrize("type_", [capi.JELLO, capi.SNOW], ids=["jello", "snow"])
 def test_stress_paths_against_oracle(gpu, type_):
     """The device stress on both of its paths -- the polar rotation by Newton
