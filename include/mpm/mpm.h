@@ -181,9 +181,13 @@ int mpm_create(const mpm_config *cfg, mpm_sim **out);
  * new group's index through *group (may be NULL). */
 int mpm_add_object(mpm_sim *sim, const mpm_object *obj, int32_t *group);
 
+/* Most particles one simulator holds (all groups together): 2^30 - 1. */
+#define MPM_MAX_PARTICLES ((1ll << 30) - 1)
+
 /* Add a group from explicit particles (positions xyz[3n]; vel[3n] or NULL
  * to use v0; particle_volume <= 0 means h^3).  initParticles,
- * MPMSimulator.cpp:10-120: F = I, Fp = I / J = 1, C = 0. */
+ * MPMSimulator.cpp:10-120: F = I, Fp = I / J = 1, C = 0.  MPM_ERR_INVALID
+ * when the simulator would hold more than MPM_MAX_PARTICLES. */
 int mpm_add_particles(mpm_sim *sim, const mpm_material *mat, int64_t n, const float *xyz,
                       const float *vel, const double v0[3], double particle_volume, int32_t *group);
 

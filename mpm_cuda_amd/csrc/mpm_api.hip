@@ -251,7 +251,8 @@ int mpm_add_particles(mpm_sim *s, const mpm_material *mat, int64_t n, const floa
     if (rc) return rc;
     if (n <= 0 || !xyz) return set_error(MPM_ERR_INVALID, "no particles");
     if ((rc = flush_spill(s))) return rc;     // a P2G phase's spill scatter reads the state it saw
-    if (s->total + n > (1ll << 31) - 1) return set_error(MPM_ERR_INVALID, "too many particles");
+    // particle offsets inside a plane are 32-bit byte offsets on the device
+    if (s->total + n > MPM_MAX_PARTICLES) return set_error(MPM_ERR_INVALID, "too many particles");
     // every particle's stencil must lie inside the grid (MPMSimulator.cpp:210-222)
     const GridP &g = s->g;
     const float o[3] = {g.ox, g.oy, g.oz};

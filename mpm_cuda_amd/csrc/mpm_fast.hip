@@ -635,12 +635,15 @@ __device__ __forceinline__ void sort_chunk_wave(TileKeys &L, int nc)
 
 // Phase C, one particle given its record in registers (fx, q0, hB and G as
 // xx, yy, zz, xy, xz, yz; water: the diagonal value in Gs[0]): add its 27
-// node contributions to acc (node ox + 3 oy + 9 oz).  The sum is factored
-// per axis:
-//   p_c(o) = Nz (alpha_c + oz beta_c) + Dz gamma_c
-// with alpha, beta, gamma depending on (ox, oy) only, N the B-spline weights
-// and D = dN/dx.  Offsets enter through kadd/kmul (no multiply by a
-// constant 0, which the compiler must keep); water's G is diagonal.
+// node contributions to acc (node ox + 3 oy + 9 oz).  With N the B-spline
+// weights, D = dN/dx and w = Nx Ny Nz, node o receives m w and
+//   p_c(o) = w (q0_c + hB_c . o) + G_c . grad w
+//          = Nz alpha_c(ox, oy) + (Nx Ny) zeta_c(oz)
+//   alpha_c = Ny (Nx (q0_c + ox hB_c0) + G_c0 Dx) + Nx (oy Ny hB_c1 + G_c1 Dy)
+//   zeta_c  = oz Nz hB_c2 + G_c2 Dz
+// so each node costs one FMA for the mass and two per momentum component.
+// Offsets enter through kadd (no multiply by a constant 0, which the
+// compiler must keep); water's G is diagonal.
 __device__ __forceinline__ float kadd(int k, float a, float b) { return k == 0 ? a : (k == 1 ? a + b : a + (b + b)); }
 
 template <int MAT>
@@ -661,22 +664,28 @@ __device__ __forceinline__ void accum_regs(const float fx[3], const float q0[3],
     // G_ck is structurally zero for water off the diagonal
 #define GZ(c, k) (W && (c) != (k))
     float N[3][3], D[3][3];
+    bspline3_wave(fx, N, D);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        bspline_axis_fast<float>(fx[a], N[a], D[a]);
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int k = 0; k < 3; ++k) D[a][k] *= inv_h;
-    }
-    // ey[oy][c] = N_y oy hB_c1 + G_c1 D_y ; eyz marks a structural zero
-    float ey[3][3];
+    // ey[oy][c] = oy Ny hB_c1 + G_c1 Dy and ez[oz][c] = oz Nz hB_c2 + G_c2 Dz
+    // (zeta); eyz marks a structural zero
+    float ey[3][3], ez[3][3];
 #pragma unroll
-    for (int oy = 0; oy < 3; ++oy)
+    for (int o = 0; o < 3; ++o)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float gd = GZ(c, 1) ? 0.0f : G[3 * c + 1] * D[1][oy];
-            const float nb = oy == 0 ? 0.0f : N[1][oy] * (oy == 1 ? hB[3 * c + 1] : hB[3 * c + 1] + hB[3 * c + 1]);
-            ey[oy][c] = oy == 0 ? gd : (GZ(c, 1) ? nb : nb + gd);
+            const float gy = GZ(c, 1) ? 0.0f : G[3 * c + 1] * D[1][o];
+            const float ny = o == 0 ? 0.0f : N[1][o] * (o == 1 ? hB[3 * c + 1] : hB[3 * c + 1] + hB[3 * c + 1]);
+            ey[o][c] = o == 0 ? gy : (GZ(c, 1) ? ny : ny + gy);
+            const float gz = GZ(c, 2) ? 0.0f : G[3 * c + 2] * D[2][o];
+            const float nz = o == 0 ? 0.0f : N[2][o] * (o == 1 ? hB[3 * c + 2] : hB[3 * c + 2] + hB[3 * c + 2]);
+            ez[o][c] = o == 0 ? gz : (GZ(c, 2) ? nz : nz + gz);
         }
+    float mz[3];
+#pragma unroll
+    for (int oz = 0; oz < 3; ++oz) mz[oz] = m * N[2][oz];
 #pragma unroll
     for (int ox = 0; ox < 3; ++ox) {
         float sx[3];
@@ -688,25 +697,22 @@ __device__ __forceinline__ void accum_regs(const float fx[3], const float q0[3],
 #pragma unroll
         for (int oy = 0; oy < 3; ++oy) {
             const float nxy = N[0][ox] * N[1][oy];
-            const float mxy = nxy * m;
-            float al[3], be[3], ga[3];
+            float al[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const bool eyz = oy == 0 && GZ(c, 1);
                 al[c] = eyz ? N[1][oy] * sx[c] : fmaf(N[1][oy], sx[c], N[0][ox] * ey[oy][c]);
-                be[c] = nxy * hB[3 * c + 2];
-                ga[c] = GZ(c, 2) ? 0.0f : nxy * G[3 * c + 2];
             }
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
                 float4 &A = acc[ox + 3 * oy + 9 * oz];
-                const float nz = N[2][oz], dz = D[2][oz];
-                A.x = fmaf(mxy, nz, A.x);
+                const float nz = N[2][oz];
+                A.x = fmaf(nxy, mz[oz], A.x);
                 float *Ac[3] = {&A.y, &A.z, &A.w};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    float v = fmaf(nz, kadd(oz, al[c], be[c]), *Ac[c]);
-                    if (!GZ(c, 2)) v = fmaf(dz, ga[c], v);
+                    float v = fmaf(nz, al[c], *Ac[c]);
+                    if (!(oz == 0 && GZ(c, 2))) v = fmaf(nxy, ez[oz][c], v);
                     *Ac[c] = v;
                 }
             }
@@ -785,12 +791,43 @@ __device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int bat
     flush_item(arena, has, bin, acc);
 }
 
+// The P2G record straight from a loaded particle (FAST form): fx, hB = hm C
+// with hm = h m, q0 = m v - hB fx, and G (xx, yy, zz, xy, xz, yz) = -dt V
+// tau, water: G[0] = dt V p J (MPMCUDA.cu:345-353, 391-414).
+template <int MAT>
+__device__ __forceinline__ void record_from_raw(const GridP &g, const MatP &mat, float dt, float hm, const RawP &r,
+                                                float fx[3], float q0[3], float hB[9], float Gs[6])
+{
+    int b[3];
+    (void)base_fx(g, r.x, b, fx);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hB[k] = hm * r.C[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float q = mat.mass * r.v[c];
+        q = fmaf(-hB[3 * c], fx[0], q);
+        q = fmaf(-hB[3 * c + 1], fx[1], q);
+        q0[c] = fmaf(-hB[3 * c + 2], fx[2], q);
+    }
+    if (MAT == MPM_WATER) {
+        const float J = r.s[0];
+        const float pr = mat.eos_k * (powf(J, -mat.eos_gamma) - 1.0f);   // MPMCUDA.cu:353
+        Gs[0] = dt * mat.vol * pr * J;
+#pragma unroll
+        for (int k = 1; k < 6; ++k) Gs[k] = 0.0f;
+    } else {
+        const float c = -dt * mat.vol;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Gs[k] = c * r.s[k];
+    }
+}
+
 // Phase C for one wave, FAST form: the item's particles are loaded again
 // from the particle planes (the tile's lines are in cache from phase A or
 // arrive now; no LDS records) and their records built in registers.
 template <int MAT>
 __device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena, int batch, const GridP &g,
-                                                     const MatP &mat, const PartView &pv, int i0, float dt)
+                                                     const MatP &mat, const PartView &pv, int i0, float dt, float hm)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t it = (uint32_t)(batch * 64 + lane);
@@ -813,13 +850,9 @@ __device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena,
         const RawP raw = nxt;
         if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), nxt);
         if (r < cnt) {
-            Scatter sc;
-            scatter_from_raw<MAT>(raw, mat, dt, sc);
-            Stencil st;
-            make_stencil(g, raw.x, st);
-            float q0[3], hB[9], Gs[6];
-            make_record<MAT>(g, st, sc, q0, hB, Gs);
-            accum_regs<MAT>(st.fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
+            float fx[3], q0[3], hB[9], Gs[6];
+            record_from_raw<MAT>(g, mat, dt, hm, raw, fx, q0, hB, Gs);
+            accum_regs<MAT>(fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
         }
     }
     flush_item(arena, has, bin, acc);
@@ -885,6 +918,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     uint32_t bits = 0;
     const TileGeo t = tile_geo(g, d.z);
     const bool staged = d.w < staging_cap;
+    const float hm = (1.0f / g.inv_h) * mat.mass;
     const int lane = threadIdx.x & 63;
     for (int k = lane; k < A_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
@@ -911,7 +945,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
         // B and C
         sort_chunk_wave(L, nc);
         const int nb = (int)((L.nitems + 63u) >> 6);
-        for (int b = 0; b < nb; ++b) scatter_items_reload<MAT>(L, L.arena, b, g, mat, pv, (int)(d.x + c0), dt);
+        for (int b = 0; b < nb; ++b) scatter_items_reload<MAT>(L, L.arena, b, g, mat, pv, (int)(d.x + c0), dt, hm);
         __asm__ volatile("" ::: "memory");
     }
     // the arena into the tile's staging slot (logical node order); mark the

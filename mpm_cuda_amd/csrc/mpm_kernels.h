@@ -8,6 +8,15 @@ namespace mpm {
 
 constexpr float kFallbackRatio = 1e-4f;   // sigma_min / sigma_max below this -> FP64 SVD
 
+// Particle i of a group's plane: the plane's first element of the group is
+// a uniform address and i < 2^30 (the particle limit), so the byte offset is
+// a 32-bit lane value and a load takes the scalar-base form (no 64-bit
+// address arithmetic per load).
+__device__ __forceinline__ MPM_GLOBAL float *plane_at(const PartView &pv, int plane, uint32_t i)
+{
+    MPM_GLOBAL char *base = (MPM_GLOBAL char *)gp(pv.P + (int64_t)plane * pv.cap + pv.off);
+    return (MPM_GLOBAL float *)(base + (i << 2));
+}
 __device__ __forceinline__ float ld(const PartView &pv, int plane, int i)
 {
     return gp(pv.P)[(int64_t)plane * pv.cap + pv.off + i];
@@ -38,20 +47,55 @@ __device__ __forceinline__ void mark_blocks(const GridP &g, const int b[3], uint
                 gp(flags)[((int64_t)bz * g.nby + by) * g.nbx + bx] = 1u;
 }
 
-__device__ __forceinline__ uint32_t make_stencil(const GridP &g, const float x[3], Stencil &s)
+// The three axes' weights and derivatives: the closed form of
+// bspline_axis_fast when every active lane's fx lies in [0.5, 1.5] (no base
+// node clamped: the usual case), chosen once for the wave, so the common
+// path carries none of the general form's per-lane selects; per lane
+// otherwise.  Same values either way.
+__device__ __forceinline__ void bspline3_wave(const float fx[3], float w[3][3], float dw[3][3])
+{
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) in = in && fx[a] >= 0.5f && fx[a] <= 1.5f;
+    if (__all(in)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float a0 = 1.5f - fx[a], d1 = fx[a] - 1.0f, a2 = fx[a] - 0.5f;
+            w[a][0] = 0.5f * a0 * a0;
+            w[a][1] = 0.75f - d1 * d1;
+            w[a][2] = 0.5f * a2 * a2;
+            dw[a][0] = -a0;
+            dw[a][1] = -2.0f * d1;
+            dw[a][2] = a2;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) bspline_axis_fast<float>(fx[a], w[a], dw[a]);
+    }
+}
+
+// Base node (clamped so the 3x3x3 support stays in the grid) and the
+// position relative to it in cells, fx = (x - o) * inv_h - b (one rounding).
+__device__ __forceinline__ uint32_t base_fx(const GridP &g, const float x[3], int b[3], float fx[3])
 {
     uint32_t bits = 0;
     const float o[3] = {g.ox, g.oy, g.oz};
     const int n[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int b = base_index_f32(x[a], o[a], g.inv_h);
-        if (b > n[a] - 3) { b = n[a] - 3; bits |= MPM_H_STENCIL_OOB; }
-        s.b[a] = b;
-        // relative coordinate, one rounding: (x - o) * inv_h - b
-        s.fx[a] = fmaf(x[a] - o[a], g.inv_h, -(float)b);
-        bspline_axis_fast<float>(s.fx[a], s.w[a], s.dw[a]);
+        int v = base_index_f32(x[a], o[a], g.inv_h);
+        if (v > n[a] - 3) { v = n[a] - 3; bits |= MPM_H_STENCIL_OOB; }
+        b[a] = v;
+        fx[a] = fmaf(x[a] - o[a], g.inv_h, -(float)v);
     }
+    return bits;
+}
+
+__device__ __forceinline__ uint32_t make_stencil(const GridP &g, const float x[3], Stencil &s)
+{
+    const uint32_t bits = base_fx(g, x, s.b, s.fx);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bspline_axis_fast<float>(s.fx[a], s.w[a], s.dw[a]);
     return bits;
 }
 
@@ -196,7 +240,7 @@ struct RawP {
 // loads then take the scalar-base form and share one offset register.
 __device__ __forceinline__ float ld_plane(const PartView &pv, int plane, uint32_t i)
 {
-    return gp(pv.P + (int64_t)plane * pv.cap + pv.off)[i];
+    return *plane_at(pv, plane, i);
 }
 template <int MAT>
 __device__ __forceinline__ void load_raw(const PartView &pv, int i, RawP &r)
