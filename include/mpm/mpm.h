@@ -97,7 +97,7 @@ typedef struct mpm_config {
     int32_t svd_sweeps;       /* FP32 Jacobi sweeps (plain solve) */
     int32_t warm_start;       /* MPM_WARM_* */
     int32_t warm_sweeps;      /* Jacobi sweeps after a warm start */
-    int32_t tile;             /* FAST: particles per P2G tile, 0 = tuned default */
+    int32_t tile;             /* FAST: P2G particles per pass over a tile: 256, 384 or 512; 0 = 384 */
     int32_t threads;          /* threads per workgroup for particle kernels, 0 = default */
     int32_t dt_mode;          /* MPM_DT_*; the dt given to mpm_step is max_dt */
     float cfl;                /* CFL number of the controller, in (0, 1] (0.5) */
@@ -374,6 +374,7 @@ typedef struct mpm_batch_shape {
     int32_t g2p_threads;      /* G2P workgroup size: 64, 128, 192 or 256 */
     int32_t sort_every;       /* steps between re-sorts of every scene */
     int32_t use_graph;        /* 1: replay captured hipGraphs; 0: launch the stages directly */
+    int32_t p2g_tile;         /* P2G particles per pass over a tile: 256, 384 or 512 (0 = 384) */
 } mpm_batch_shape;
 
 void mpm_batch_shape_default(mpm_batch_shape *shape);
@@ -383,9 +384,10 @@ void mpm_batch_shape_default(mpm_batch_shape *shape);
 int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *shape, mpm_batch **out);
 
 /* Times 4 launch-width candidates of P2G, grid update and G2P on live steps
- * (advances every scene by 4 * reps steps) and keeps the fastest of each.
- * ms_out (NULL or 12 floats): median ms per stage (P2G, grid, G2P) and
- * candidate. */
+ * and keeps the fastest of each, then times the 3 P2G tile shapes (256, 384,
+ * 512 particles per pass) with those widths and keeps the fastest; advances
+ * every scene by 7 * reps steps.  ms_out (NULL or 15 floats): median ms per
+ * stage (P2G, grid, G2P) and width candidate, then P2G ms per tile shape. */
 int mpm_batch_autotune(mpm_batch *b, int32_t reps, float *ms_out);
 
 int mpm_batch_get_shape(const mpm_batch *b, mpm_batch_shape *out);

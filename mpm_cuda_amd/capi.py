@@ -55,7 +55,7 @@ class MpmObject(C.Structure):
 
 class MpmBatchShape(C.Structure):
     _fields_ = [("p2g_workgroups", C.c_int32), ("grid_workgroups", C.c_int32), ("g2p_threads", C.c_int32),
-                ("sort_every", C.c_int32), ("use_graph", C.c_int32)]
+                ("sort_every", C.c_int32), ("use_graph", C.c_int32), ("p2g_tile", C.c_int32)]
 
 
 class Mpm2dConfig(C.Structure):
@@ -337,7 +337,8 @@ class Sim:
 
 
 BATCH_STAGES = ("p2g", "grid", "g2p")
-BATCH_CANDIDATES = {"p2g": (1024, 2048, 4096, 8192), "grid": (1024, 2048, 4096, 8192), "g2p": (64, 128, 192, 256)}
+BATCH_CANDIDATES = {"p2g": (1024, 2048, 4096, 8192), "grid": (1024, 2048, 4096, 8192), "g2p": (64, 128, 192, 256),
+                    "tile": (256, 384, 512)}
 
 
 def batch_shape(**kw) -> MpmBatchShape:
@@ -378,11 +379,13 @@ class Batch:
         self.close()
 
     def autotune(self, reps: int = 3) -> dict:
-        """Median ms per stage and candidate; keeps the fastest candidate of each stage."""
-        ms = np.zeros(12, np.float32)
+        """Median ms per stage and candidate (launch widths of P2G, grid update
+        and G2P, then P2G per tile shape); keeps the fastest of each."""
+        ms = np.zeros(15, np.float32)
         check(lib().mpm_batch_autotune(self._h, int(reps), ms.ctypes.data_as(C.c_void_p)))
-        ms = ms.reshape(3, 4)
-        return {st: dict(zip(BATCH_CANDIDATES[st], ms[i].tolist())) for i, st in enumerate(BATCH_STAGES)}
+        out = {st: dict(zip(BATCH_CANDIDATES[st], ms[4 * i:4 * i + 4].tolist())) for i, st in enumerate(BATCH_STAGES)}
+        out["tile"] = dict(zip(BATCH_CANDIDATES["tile"], ms[12:15].tolist()))
+        return out
 
     @property
     def shape(self) -> MpmBatchShape:

@@ -191,6 +191,7 @@ int mpm_create(const mpm_config *cfg, mpm_sim **out)
         return set_error(MPM_ERR_INVALID, "grid too large");
     if (cfg->form != MPM_FORM_PLAIN && cfg->form != MPM_FORM_FAST && cfg->form != MPM_FORM_FUSED)
         return set_error(MPM_ERR_INVALID, "unknown form");
+    if (!tile_shape_valid(cfg->tile)) return set_error(MPM_ERR_INVALID, "tile must be 0, 256, 384 or 512");
     if (cfg->bc_mode != MPM_BC_AXIS && cfg->bc_mode != MPM_BC_STICKY) return set_error(MPM_ERR_INVALID, "unknown bc_mode");
     if (cfg->svd_sweeps < 1 || cfg->svd_sweeps > 16) return set_error(MPM_ERR_INVALID, "svd_sweeps out of range [1,16]");
     if (cfg->dt_mode < MPM_DT_FIXED || cfg->dt_mode > MPM_DT_CFL_DEFORM) return set_error(MPM_ERR_INVALID, "unknown dt_mode");

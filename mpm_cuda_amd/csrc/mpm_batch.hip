@@ -31,6 +31,8 @@ constexpr int NCAND = 4;
 const int P2G_CAND[NCAND] = {1024, 2048, 4096, 8192};   // one-wave workgroups
 const int GRID_CAND[NCAND] = {1024, 2048, 4096, 8192};
 const int G2P_CAND[NCAND] = {64, 128, 192, 256};
+constexpr int NTILE = 3;
+const int TILE_CAND[NTILE] = {256, 384, 512};            // P2G particles per pass over a tile
 
 // What the captured graphs depend on, per scene.
 struct SceneSig {
@@ -61,7 +63,7 @@ struct mpm_batch {
     std::vector<SceneSig> sig;          // signature the tables and graphs were built for
     bool built = false;
     hipGraphExec_t step_exec[2] = {}, sort_exec[2] = {};
-    int captured_p2g = 0, captured_grid = 0, captured_g2p = 0;
+    int captured_p2g = 0, captured_grid = 0, captured_g2p = 0, captured_tile = 0;
     int parity = 0;
     int64_t steps_since_sort = 0;
     int64_t total = 0;
@@ -123,7 +125,7 @@ int check(int rc, const char *what) { return rc ? hip_fail((hipError_t)rc, what)
 
 // One step of every scene on the batch stream (parity p tables).  With
 // `ev` (6 events) the P2G, grid-update and G2P launches are bracketed.
-int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, hipEvent_t *ev)
+int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int tile, hipEvent_t *ev)
 {
     hipStream_t st = b->stream;
     const BatchSlice *tab = b->tab[p];
@@ -132,7 +134,7 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, hipEve
     if ((rc = check(launch_batch_zero(st, tab, n), "batch_zero"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[0], st));
     for (int m = MPM_JELLO; m <= MPM_WATER; ++m)
-        if ((rc = check(launch_p2g_batch(st, m, tab, b->types.data(), n, p2g), "p2g_batch"))) return rc;
+        if ((rc = check(launch_p2g_batch(st, m, tab, b->types.data(), n, p2g, tile), "p2g_batch"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[1], st));
     if ((rc = check(launch_spill_batch(st, tab, n, SPILL_WGS), "spill_batch"))) return rc;
     if ((rc = check(launch_compact_batch(st, tab, n, b->compact_wgs), "compact_batch"))) return rc;
@@ -201,7 +203,7 @@ int capture(mpm_batch *b, hipGraphExec_t *out, bool sort, int p)
     MPM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     int rc = sort ? enqueue_sort(b, p)
                   : enqueue_step(b, p, b->shape.p2g_workgroups, b->shape.grid_workgroups, b->shape.g2p_threads,
-                                 nullptr);
+                                 b->shape.p2g_tile, nullptr);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &graph);
     for (size_t k = 0; k < b->sims.size(); ++k) set_book(b->sims[k], book[k]);
@@ -277,6 +279,7 @@ int build(mpm_batch *b)
     b->captured_p2g = b->shape.p2g_workgroups;
     b->captured_grid = b->shape.grid_workgroups;
     b->captured_g2p = b->shape.g2p_threads;
+    b->captured_tile = b->shape.p2g_tile;
     return MPM_OK;
 }
 
@@ -287,6 +290,7 @@ int validate_shape(const mpm_batch_shape &s)
     if (s.g2p_threads < 64 || s.g2p_threads > 256 || s.g2p_threads % 64)
         return set_error(MPM_ERR_INVALID, "g2p_threads must be 64, 128, 192 or 256");
     if (s.sort_every < 1) return set_error(MPM_ERR_INVALID, "sort_every must be >= 1");
+    if (s.p2g_tile != 0 && !tile_shape_valid(s.p2g_tile)) return set_error(MPM_ERR_INVALID, "p2g_tile must be 256, 384 or 512");
     return MPM_OK;
 }
 
@@ -317,7 +321,8 @@ int prepare(mpm_batch *b)
     for (size_t k = 0; same && k < n; ++k) same = sig_of(b, k) == b->sig[k];
     if (same && b->shape.use_graph)
         same = b->step_exec[0] && b->captured_p2g == b->shape.p2g_workgroups &&
-               b->captured_grid == b->shape.grid_workgroups && b->captured_g2p == b->shape.g2p_threads;
+               b->captured_grid == b->shape.grid_workgroups && b->captured_g2p == b->shape.g2p_threads &&
+               b->captured_tile == b->shape.p2g_tile;
     if (!same) {
         for (mpm_sim *s : b->sims) MPM_HIP(hipStreamSynchronize(s->stream));
         int rc = build(b);
@@ -368,7 +373,7 @@ void after_step(mpm_batch *b)
 }
 
 // Sort when due, then one step; graphs or direct launches.
-int one_step(mpm_batch *b, int p2g, int grid, int g2p_threads, hipEvent_t *ev, bool graphs)
+int one_step(mpm_batch *b, int p2g, int grid, int g2p_threads, int tile, hipEvent_t *ev, bool graphs)
 {
     if (b->steps_since_sort >= b->shape.sort_every) {
         if (graphs) {
@@ -385,7 +390,7 @@ int one_step(mpm_batch *b, int p2g, int grid, int g2p_threads, hipEvent_t *ev, b
     if (graphs) {
         MPM_HIP(hipGraphLaunch(b->step_exec[b->parity], b->stream));
     } else {
-        int rc = enqueue_step(b, b->parity, p2g, grid, g2p_threads, ev);
+        int rc = enqueue_step(b, b->parity, p2g, grid, g2p_threads, tile, ev);
         if (rc) return rc;
     }
     after_step(b);
@@ -415,6 +420,7 @@ void mpm_batch_shape_default(mpm_batch_shape *s)
     s->g2p_threads = 256;
     s->sort_every = 24;
     s->use_graph = 1;
+    s->p2g_tile = 384;
 }
 
 int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *shape, mpm_batch **out)
@@ -498,7 +504,8 @@ int mpm_batch_run(mpm_batch *b, int32_t nsteps, float *step_ms)
     const bool graphs = b->shape.use_graph != 0;
     for (int32_t k = 0; k < nsteps; ++k) {
         if (ev) MPM_HIP(hipEventRecord(ev[k], b->stream));
-        rc = one_step(b, b->shape.p2g_workgroups, b->shape.grid_workgroups, b->shape.g2p_threads, nullptr, graphs);
+        rc = one_step(b, b->shape.p2g_workgroups, b->shape.grid_workgroups, b->shape.g2p_threads, b->shape.p2g_tile,
+                      nullptr, graphs);
         if (rc) return rc;
     }
     if (ev) MPM_HIP(hipEventRecord(ev[nsteps], b->stream));
@@ -512,9 +519,11 @@ int mpm_batch_run(mpm_batch *b, int32_t nsteps, float *step_ms)
 
 // Times each stage's launch-width candidates on live steps (rounds of
 // NCAND steps, candidate c in step c of every round) and keeps the fastest
-// median per stage.  Advances every scene by reps * NCAND steps.  ms_out
-// (optional, 3 x NCAND): median milliseconds of P2G, grid update and G2P
-// per candidate.
+// median per stage; then, with those widths, the P2G tile shapes (rounds of
+// NTILE steps) and keeps the fastest.  Advances every scene by
+// reps * (NCAND + NTILE) steps.  ms_out (optional, 3 x NCAND + NTILE):
+// median milliseconds of P2G, grid update and G2P per width candidate, then
+// of P2G per tile shape.
 int mpm_batch_autotune(mpm_batch *b, int32_t reps, float *ms_out)
 {
     if (!b) return set_error(MPM_ERR_INVALID, "batch is null");
@@ -522,32 +531,33 @@ int mpm_batch_autotune(mpm_batch *b, int32_t reps, float *ms_out)
     int rc = prepare(b);
     if (rc) return rc;
     const size_t per = 6;
-    hipEvent_t *ev = timing_events(b, (size_t)reps * NCAND * per);
+    hipEvent_t *ev = timing_events(b, (size_t)reps * (NCAND + NTILE) * per);
     if (!ev) return set_error(MPM_ERR_HIP, "hipEventCreate failed");
+    const int tile0 = tile_shape(b->shape.p2g_tile);
     for (int32_t r = 0; r < reps; ++r)
         for (int c = 0; c < NCAND; ++c) {
             hipEvent_t *e = ev + ((size_t)r * NCAND + c) * per;
-            rc = one_step(b, P2G_CAND[c], GRID_CAND[c], G2P_CAND[c], e, false);
+            rc = one_step(b, P2G_CAND[c], GRID_CAND[c], G2P_CAND[c], tile0, e, false);
             if (rc) return rc;
         }
-    if ((rc = release(b))) return rc;
     MPM_HIP(hipStreamSynchronize(b->stream));
-    float med[3][NCAND];
-    for (int c = 0; c < NCAND; ++c) {
-        std::vector<float> t[3];
+    // median of the span [e[k0], e[k0 + 1]] over the rounds
+    auto median = [&](size_t first, size_t stride, int k0, float &out) -> int {
+        std::vector<float> t;
         for (int32_t r = 0; r < reps; ++r) {
-            hipEvent_t *e = ev + ((size_t)r * NCAND + c) * per;
-            float a = 0, g = 0, q = 0;
-            MPM_HIP(hipEventElapsedTime(&a, e[0], e[1]));
-            MPM_HIP(hipEventElapsedTime(&g, e[2], e[3]));
-            MPM_HIP(hipEventElapsedTime(&q, e[4], e[5]));
-            t[0].push_back(a); t[1].push_back(g); t[2].push_back(q);
+            hipEvent_t *e = ev + (first + (size_t)r * stride) * per;
+            float v = 0;
+            MPM_HIP(hipEventElapsedTime(&v, e[k0], e[k0 + 1]));
+            t.push_back(v);
         }
-        for (int j = 0; j < 3; ++j) {
-            std::sort(t[j].begin(), t[j].end());
-            med[j][c] = t[j][t[j].size() / 2];
-        }
-    }
+        std::sort(t.begin(), t.end());
+        out = t[t.size() / 2];
+        return MPM_OK;
+    };
+    float med[3][NCAND];
+    for (int c = 0; c < NCAND; ++c)
+        for (int j = 0; j < 3; ++j)
+            if ((rc = median((size_t)c, NCAND, 2 * j, med[j][c]))) return rc;
     int best[3] = {0, 0, 0};
     for (int j = 0; j < 3; ++j)
         for (int c = 1; c < NCAND; ++c)
@@ -555,9 +565,29 @@ int mpm_batch_autotune(mpm_batch *b, int32_t reps, float *ms_out)
     b->shape.p2g_workgroups = P2G_CAND[best[0]];
     b->shape.grid_workgroups = GRID_CAND[best[1]];
     b->shape.g2p_threads = G2P_CAND[best[2]];
-    if (ms_out)
+    // tile shapes with the chosen widths
+    const size_t t0 = (size_t)reps * NCAND;
+    for (int32_t r = 0; r < reps; ++r)
+        for (int c = 0; c < NTILE; ++c) {
+            hipEvent_t *e = ev + (t0 + (size_t)r * NTILE + c) * per;
+            rc = one_step(b, b->shape.p2g_workgroups, b->shape.grid_workgroups, b->shape.g2p_threads, TILE_CAND[c], e,
+                          false);
+            if (rc) return rc;
+        }
+    if ((rc = release(b))) return rc;
+    MPM_HIP(hipStreamSynchronize(b->stream));
+    float tmed[NTILE];
+    int tbest = 0;
+    for (int c = 0; c < NTILE; ++c) {
+        if ((rc = median(t0 + (size_t)c, NTILE, 0, tmed[c]))) return rc;
+        if (tmed[c] < tmed[tbest]) tbest = c;
+    }
+    b->shape.p2g_tile = TILE_CAND[tbest];
+    if (ms_out) {
         for (int j = 0; j < 3; ++j)
             for (int c = 0; c < NCAND; ++c) ms_out[j * NCAND + c] = med[j][c];
+        for (int c = 0; c < NTILE; ++c) ms_out[3 * NCAND + c] = tmed[c];
+    }
     return MPM_OK;
 }
 

@@ -359,7 +359,10 @@ constexpr int P2G_THREADS = 64;                  // FAST form: one wave per tile
 static_assert(ARENA_NODES % TILE_THREADS == 0, "arena nodes per thread must be whole");
 constexpr int CHUNK = 384;          // particles per pass over a tile (a full 4^3-cell block holds ~320 at 5 per cell)
 static_assert(CHUNK % TILE_THREADS == 0, "whole phase-A passes");
-constexpr int P2G_PASSES = CHUNK / P2G_THREADS;  // FAST phase-A passes over a chunk
+// FAST form: particles per pass over a tile, chosen per simulator
+// (mpm_config.tile) or per batch (its autotuner) among these
+constexpr int TILE_SHAPES[3] = {256, 384, 512};
+constexpr int TILE_DEFAULT = 384;
 constexpr int ITEM_CAP = 4;         // particles per lane item
 constexpr int NBIN = 216;           // 6^3 base positions inside an arena
 constexpr uint32_t NO_BIN = 255u;
@@ -407,16 +410,19 @@ struct TileLds {
     uint32_t nitems;
 };
 
-// LDS of a FAST-form tile (one wave): bin keys, the sort and the arena
-// (12.5 KB; the particle records stay in HBM / cache).
+// LDS of a FAST-form tile (one wave) of CH particles per pass: bin keys,
+// the sort and the arena (11.4 / 12.5 / 13.7 KB for CH = 256 / 384 / 512;
+// the particle records stay in HBM / cache).
+template <int CH>
 struct TileKeys {
+    static_assert(CH % 64 == 0 && CH <= 4095, "whole 64-particle passes; 12-bit item positions");
     float4 arena[A_SLOTS];
-    uint32_t items[CHUNK];
-    uint16_t perm[CHUNK];
-    uint16_t rnk[CHUNK];
+    uint32_t items[CH];
+    uint16_t perm[CH];
+    uint16_t rnk[CH];
     uint16_t cnt[NBIN];
     uint16_t start[NBIN];       // each bin's first sorted position
-    uint8_t key[CHUNK];
+    uint8_t key[CH];
     uint32_t nitems;
 };
 
@@ -598,7 +604,8 @@ __device__ __forceinline__ void sort_chunk(TileLds &L, int nc)
 // Phase B, FAST form (the tile's one wave): the same sort, in slot order
 // within each bin.  One wave's LDS operations execute in order, so the
 // steps need no barrier; the compiler barriers keep their order.
-__device__ __forceinline__ void sort_chunk_wave(TileKeys &L, int nc)
+template <class TK>
+__device__ __forceinline__ void sort_chunk_wave(TK &L, int nc)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
@@ -825,8 +832,8 @@ __device__ __forceinline__ void record_from_raw(const GridP &g, const MatP &mat,
 // Phase C for one wave, FAST form: the item's particles are loaded again
 // from the particle planes (the tile's lines are in cache from phase A or
 // arrive now; no LDS records) and their records built in registers.
-template <int MAT>
-__device__ __forceinline__ void scatter_items_reload(TileKeys &L, float4 *arena, int batch, const GridP &g,
+template <int MAT, class TK>
+__device__ __forceinline__ void scatter_items_reload(TK &L, float4 *arena, int batch, const GridP &g,
                                                      const MatP &mat, const PartView &pv, int i0, float dt, float hm)
 {
     const int lane = threadIdx.x & 63;
@@ -910,10 +917,10 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 // of 53 KB) and no barrier, and the register file alone bounds the tiles
 // in flight per CU (8 one-wave tiles instead of 3 two-wave ones with LDS
 // records: 77 vs 106 us per 1M-particle snow step, measured).
-template <int MAT>
+template <int MAT, int CH>
 __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, const PartView &pv, const uint4 d,
                                              uint32_t staging_cap, float dt, float4 *__restrict__ staging,
-                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, TileKeys &L)
+                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, TileKeys<CH> &L)
 {
     uint32_t bits = 0;
     const TileGeo t = tile_geo(g, d.z);
@@ -921,19 +928,20 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     const float hm = (1.0f / g.inv_h) * mat.mass;
     const int lane = threadIdx.x & 63;
     for (int k = lane; k < A_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (uint32_t c0 = 0; c0 < d.y; c0 += CHUNK) {
-        const int nc = (int)min((uint32_t)CHUNK, d.y - c0);
+    constexpr int PASSES = CH / 64;
+    for (uint32_t c0 = 0; c0 < d.y; c0 += CH) {
+        const int nc = (int)min((uint32_t)CH, d.y - c0);
         // A: all position loads of the chunk first (lanes past its end reload
         // its last particle), then bin keys
-        float xs[P2G_PASSES][3];
+        float xs[PASSES][3];
 #pragma unroll
-        for (int it = 0; it < P2G_PASSES; ++it) {
+        for (int it = 0; it < PASSES; ++it) {
             const uint32_t slot = (uint32_t)min(it * 64 + lane, nc - 1);
 #pragma unroll
             for (int a = 0; a < 3; ++a) xs[it][a] = ld_plane(pv, PL_X + a, d.x + c0 + slot);
         }
 #pragma unroll
-        for (int it = 0; it < P2G_PASSES; ++it) {
+        for (int it = 0; it < PASSES; ++it) {
             if (it * 64 >= nc) break;
             const int slot = it * 64 + lane;
             const bool active = slot < nc;
@@ -965,7 +973,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     return bits;
 }
 
-template <int MAT>
+template <int MAT, int CH>
 __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_block(GridP g, MatP mat, PartView pv,
                                                             const uint4 *__restrict__ desc,
                                                             const uint32_t *__restrict__ tile_count, uint32_t desc_cap,
@@ -973,7 +981,7 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
                                                             float4 *__restrict__ staging, uint32_t *spill,
                                                             uint32_t *flags, uint32_t *counters, int clear_active)
 {
-    __shared__ TileKeys L;
+    __shared__ TileKeys<CH> L;
     const uint32_t nt = min(*tile_count, desc_cap);
     // the active-block and node counters are not used during P2G: clear
     // them here for the block compaction that follows
@@ -987,7 +995,7 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
     for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
         const uint32_t tt = xcd_spread(k8, nt8);
         if (tt >= nt) continue;
-        bits |= p2g_tile<MAT>(g, mat, pv, desc[tt], staging_cap, dt, staging, spill, flags, counters, L);
+        bits |= p2g_tile<MAT, CH>(g, mat, pv, desc[tt], staging_cap, dt, staging, spill, flags, counters, L);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
@@ -1003,10 +1011,10 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
 // tile's (256 B; LDS is allocated in granules, so any addition is checked
 // against the workgroups per CU it leaves).
 constexpr int P2G_CHUNK = 64;
-template <int MAT>
+template <int MAT, int CH>
 __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_p2g_batch(const BatchSlice *__restrict__ tab, int nslices)
 {
-    __shared__ TileKeys L;
+    __shared__ TileKeys<CH> L;
     __shared__ uint32_t s_end[P2G_CHUNK];
     __shared__ uint32_t s_total;
     if (threadIdx.x < 64) {
@@ -1027,7 +1035,7 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
         const BatchSlice &b = tab[sid];
         const uint4 d = gload_u4(b.desc, tt - (sid ? s_end[sid - 1] : 0u));
         const uint32_t bits =
-            p2g_tile<MAT>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L);
+            p2g_tile<MAT, CH>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L);
         if (bits) gatomic_or(b.counters + CT_HEALTH, bits);
     }
     if (blockIdx.x == 0)
@@ -1323,6 +1331,52 @@ int fast_ready(mpm_sim *s)
     return MPM_OK;
 }
 
+// Launchers of the FAST P2G kernels by material and tile shape.
+using BlockLaunch = void (*)(dim3, dim3, hipStream_t, const GridP &, const MatP &, const PartView &, const uint4 *,
+                             const uint32_t *, uint32_t, uint32_t, float, float4 *, uint32_t *, uint32_t *, uint32_t *);
+using BatchLaunch = void (*)(dim3, dim3, hipStream_t, const BatchSlice *, int);
+template <int MAT, int CH>
+static void launch_block(dim3 gr, dim3 bl, hipStream_t st, const GridP &g, const MatP &m, const PartView &pv,
+                         const uint4 *desc, const uint32_t *tc, uint32_t cap, uint32_t scap, float dt,
+                         float4 *staging, uint32_t *spill, uint32_t *flags, uint32_t *counters)
+{
+    hipLaunchKernelGGL((k_p2g_block<MAT, CH>), gr, bl, 0, st, g, m, pv, desc, tc, cap, scap, dt, staging, spill,
+                       flags, counters, 1);
+}
+template <int MAT, int CH>
+static void launch_batch(dim3 gr, dim3 bl, hipStream_t st, const BatchSlice *tab, int n)
+{
+    hipLaunchKernelGGL((k_p2g_batch<MAT, CH>), gr, bl, 0, st, tab, n);
+}
+static int tile_index(int tile) { return tile == 256 ? 0 : (tile == 512 ? 2 : 1); }
+static int mat_index(int mat) { return mat == MPM_JELLO ? 0 : (mat == MPM_SNOW ? 1 : 2); }
+static BlockLaunch block_launcher(int mat, int tile)
+{
+    static const BlockLaunch t[3][3] = {
+        {launch_block<MPM_JELLO, 256>, launch_block<MPM_JELLO, 384>, launch_block<MPM_JELLO, 512>},
+        {launch_block<MPM_SNOW, 256>, launch_block<MPM_SNOW, 384>, launch_block<MPM_SNOW, 512>},
+        {launch_block<MPM_WATER, 256>, launch_block<MPM_WATER, 384>, launch_block<MPM_WATER, 512>}};
+    return t[mat_index(mat)][tile_index(tile)];
+}
+static BatchLaunch batch_launcher(int mat, int tile)
+{
+    static const BatchLaunch t[3][3] = {
+        {launch_batch<MPM_JELLO, 256>, launch_batch<MPM_JELLO, 384>, launch_batch<MPM_JELLO, 512>},
+        {launch_batch<MPM_SNOW, 256>, launch_batch<MPM_SNOW, 384>, launch_batch<MPM_SNOW, 512>},
+        {launch_batch<MPM_WATER, 256>, launch_batch<MPM_WATER, 384>, launch_batch<MPM_WATER, 512>}};
+    return t[mat_index(mat)][tile_index(tile)];
+}
+
+int tile_shape(int tile) { return tile == 0 ? TILE_DEFAULT : tile; }
+
+bool tile_shape_valid(int tile)
+{
+    if (tile == 0) return true;
+    for (int t : TILE_SHAPES)
+        if (t == tile) return true;
+    return false;
+}
+
 int launch_p2g_tiles(mpm_sim *s, int group, float dt)
 {
     PartView pv = view_of(s, group);
@@ -1344,11 +1398,8 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     dim3 gr(wg), bl(P2G_THREADS);
     {
     PhaseTimer tk(s, TK_P2G);
-    switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters, 1); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters, 1); break;
-    default: hipLaunchKernelGGL(k_p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, s->staging, s->spill, s->flags, s->counters, 1); break;
-    }
+    block_launcher(m.type, tile_shape(s->cfg.tile))(gr, bl, s->stream, s->g, m, pv, desc, tc, cap, scap, dt,
+                                                    s->staging, s->spill, s->flags, s->counters);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
@@ -1559,8 +1610,10 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
     b.nblocks = s->nblocks;
 }
 
-// Material `mat` of the scenes [0, nslices) of tab, in chunks of P2G_CHUNK.
-int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs)
+// Material `mat` of the scenes [0, nslices) of tab, in chunks of P2G_CHUNK,
+// tiles of `tile` particles per pass.
+int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs,
+                     int tile)
 {
     dim3 gr(wgs), bl(P2G_THREADS);
     for (int c0 = 0; c0 < nslices; c0 += P2G_CHUNK) {
@@ -1568,11 +1621,7 @@ int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *
         bool any = false;
         for (int k = c0; k < c0 + n; ++k) any = any || types[k] == mat;
         if (!any) continue;
-        switch (mat) {
-        case MPM_JELLO: hipLaunchKernelGGL(k_p2g_batch<MPM_JELLO>, gr, bl, 0, st, tab + c0, n); break;
-        case MPM_SNOW: hipLaunchKernelGGL(k_p2g_batch<MPM_SNOW>, gr, bl, 0, st, tab + c0, n); break;
-        default: hipLaunchKernelGGL(k_p2g_batch<MPM_WATER>, gr, bl, 0, st, tab + c0, n); break;
-        }
+        batch_launcher(mat, tile_shape(tile))(gr, bl, st, tab + c0, n);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
     }

@@ -43,7 +43,11 @@ PartView view_of(mpm_sim *s, int g);
 // mpm_fast.hip); host side in mpm_batch.hip.
 void fill_batch_slice(mpm_sim *s, BatchSlice &b);
 int launch_batch_zero(hipStream_t st, const BatchSlice *tab, int nslices);
-int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs);
+int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs,
+                     int tile);
+// FAST P2G tile shapes (particles per pass over a tile): 0 = the default.
+int tile_shape(int tile);
+bool tile_shape_valid(int tile);
 int launch_spill_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
 int launch_compact_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
 int launch_grid_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);

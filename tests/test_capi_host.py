@@ -140,6 +140,14 @@ def test_bgeo_roundtrip(tmp_path):
         capi.write_bgeo(str(tmp_path / "no_dir" / "x.bgeo"), xyz)
 
 
+def test_config_tile_shape_checked_before_device():
+    """The FAST tile shape is one of 256, 384, 512 (0 = default); anything else
+    is refused before a device is touched."""
+    for bad in (1, 128, 400, 1024, -384):
+        with pytest.raises(capi.MpmError, match="tile"):
+            capi.Sim(capi.default_config(nx=32, ny=32, nz=32, h=1 / 32, tile=bad))
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
 def test_create_without_gpu_reports_error():
     with pytest.raises(capi.MpmError, match="no HIP device"):

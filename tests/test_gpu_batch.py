@@ -132,19 +132,20 @@ def test_autotune_keeps_results(gpu):
     A, B = _pairs(), _pairs()
     try:
         with capi.Batch([p.gpu for p in A]) as b:
-            ms = b.autotune(reps=2)                  # 8 steps
+            ms = b.autotune(reps=2)                  # 14 steps
             print("autotune medians (ms)", ms)
-            assert set(ms) == {"p2g", "grid", "g2p"}
+            assert set(ms) == {"p2g", "grid", "g2p", "tile"}
             for st, cand in ms.items():
-                assert len(cand) == 4 and all(v > 0 for v in cand.values())
+                assert len(cand) == len(capi.BATCH_CANDIDATES[st]) and all(v > 0 for v in cand.values())
             sh = b.shape
             assert sh.p2g_workgroups in capi.BATCH_CANDIDATES["p2g"]
             assert sh.grid_workgroups in capi.BATCH_CANDIDATES["grid"]
             assert sh.g2p_threads in capi.BATCH_CANDIDATES["g2p"]
             assert sh.p2g_workgroups == min(ms["p2g"], key=ms["p2g"].get)
+            assert sh.p2g_tile == min(ms["tile"], key=ms["tile"].get)
             b.run(6)
         for p in B:
-            for _ in range(14):
+            for _ in range(20):
                 p.gpu.step(p.dt)
         for pa, pb in zip(A, B):
             _compare(pa, pb)

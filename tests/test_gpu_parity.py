@@ -327,6 +327,30 @@ def test_dense_blocks_several_chunks(gpu, form):
         p.close()
 
 
+@pytest.mark.parametrize("tile", [256, 384, 512])
+def test_fast_tile_shapes(gpu, tile):
+    """Every FAST tile shape (particles per pass over a tile; 8 particles per
+    cell put ~512 in a block, so 256 and 384 take several passes): grid parity
+    after P2G and particle parity after three steps against the oracle."""
+    p = Pair(type_=capi.SNOW, form=capi.FORM_FAST, seed=19, ppc=8, tile=tile)
+    try:
+        em, ep = _p2g_compare(p)
+        assert em < TOL_GRID and ep < TOL_GRID, (em, ep)
+        p.gpu.phase(capi.PHASE_GRID, p.dt)
+        p.orc.grid_update(p.dt)
+        p.gpu.phase(capi.PHASE_G2P, p.dt)
+        p.orc.g2p(p.dt)
+        for _ in range(2):
+            p.gpu.step(p.dt)
+            assert p.orc.step(p.dt) == 0
+        g, o = p.gpu_particles(), p.orc_particles()
+        for k in ("x", "v", "C"):
+            assert rel(g[k], o[k]) < TOL_PART, k
+        assert rel(g["F"] - np.eye(3), o["F"] - np.eye(3)) < TOL_PART
+    finally:
+        p.close()
+
+
 def test_resort_forced_by_spills(gpu):
     """With the interval cap out of reach, particles moving 0.13 cells per
     step leave their block arenas within ~10 steps; the spill count forces
