@@ -134,31 +134,17 @@ def preroll_steps(workload, steps, warmup, preroll=-1):
     return max(0, min(175, COLLISION_END - warmup - steps))
 
 
-def svd_path_fraction(sim, steps, dt):
-    """Fraction of particle-steps that took the G2P's SVD path over the next
-    `steps` steps (stepping one at a time).  A snow particle takes it exactly
-    when its trial F leaves the clamp band, and then its J_p changes (the
-    clamp scales it by prod(sigma / sigma*)); on the elastic path J_p stays
-    bit-identical.  Also the fraction of particles plastically compressed
-    (J_p < 1 - 1e-3) at the end."""
-    from mpm_cuda_amd import capi
-    jp = sim.read(0, capi.P_JP)[:, 0].copy()
-    changed = 0
-    for _ in range(steps):
-        sim.step(dt)
-        j = sim.read(0, capi.P_JP)[:, 0]
-        changed += int(np.count_nonzero(j != jp))
-        jp = j.copy()
-    return changed / (sim.total * max(steps, 1)), float(np.mean(jp < 1 - 1e-3))
-
-
-def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0, preroll=0, svd_stats=False):
+def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0, preroll=0):
     """Throughput run (no per-kernel events in the timed region), then the
     same pre-roll + warm-up + timed steps again on a fresh simulator with HIP
     events around each phase and each phase's main kernel (the events cost
-    host time, so they stay out of the throughput run).  With svd_stats, a
-    third simulator steps the same window one step at a time to count the
-    particle-steps that took the SVD path."""
+    host time, so they stay out of the throughput run).  The device's SVD
+    counters give the particle-steps of the timed window whose G2P took the
+    Jacobi SVD (snow: trial F outside the clamp band, i.e. plastic flow),
+    how many of those started from the network's quaternion and how many
+    of those starts were rejected; J_p at the end gives the share of
+    particles compressed plastically (J_p < 1 - 1e-3)."""
+    from mpm_cuda_amd import capi
     sim, info = _make_sim(workload, form, sort_every, tile, warm_sweeps)
     n = sim.total
     dt = info["cfg"].dt
@@ -166,14 +152,17 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0, 
     sim.sync()
     sim.health()
     st0 = sim.stats()
+    sv0 = sim.svd_stats()
     t0 = time.perf_counter()
     sim.run(steps, dt)
     sim.sync()
     t1 = time.perf_counter()
     health = sim.health()
     stats = sim.stats()
+    sv1 = sim.svd_stats()
     stats["sorts_timed"] = stats["sorts"] - st0["sorts"]
-    stats["warm_rejects_timed"] = stats["warm_rejects"] - st0["warm_rejects"]
+    svd = {k: (sv1[k] - sv0[k]) % (1 << 32) for k in sv1}      # 32-bit device counters
+    plastic = float(np.mean(sim.read(0, capi.P_JP)[:, 0] < 1 - 1e-3)) if info["mat"].type == capi.SNOW else 0.0
     sim.close()
     # timing pass: identical steps, events on during the timed region
     sim, _ = _make_sim(workload, form, sort_every, tile, warm_sweeps)
@@ -187,13 +176,8 @@ def run_gpu(workload, form, steps, warmup, sort_every=0, tile=0, warm_sweeps=0, 
     ph = sim.phase_times()
     kt = sim.kernel_times()
     sim.close()
-    out = {"n": n, "seconds": t1 - t0, "phases": ph, "kernels": kt, "health": health, "stats": stats, "info": info}
-    if svd_stats and info["mat"].type == 1:
-        sim, _ = _make_sim(workload, form, sort_every, tile, warm_sweeps)
-        sim.run(preroll + warmup, dt)
-        out["svd_fraction"], out["plastic_fraction"] = svd_path_fraction(sim, steps, dt)
-        sim.close()
-    return out
+    return {"n": n, "seconds": t1 - t0, "phases": ph, "kernels": kt, "health": health, "stats": stats, "info": info,
+            "svd": svd, "svd_fraction": svd["svd"] / (n * steps), "plastic_fraction": plastic}
 
 
 def run_batch(steps, warmup, warm=False, autotune_reps=2, seq_steps=5):
@@ -216,7 +200,7 @@ def run_batch(steps, warmup, warm=False, autotune_reps=2, seq_steps=5):
     tune = b.autotune(reps=autotune_reps)
     b.run(warmup)
     b.sync()
-    st0 = [s.stats() for s in sims]
+    st0 = [s.svd_stats() for s in sims]
     t0 = time.perf_counter()
     b.run(steps)
     b.sync()
@@ -224,13 +208,13 @@ def run_batch(steps, warmup, warm=False, autotune_reps=2, seq_steps=5):
     ms = b.run(max(20, steps // 2), timed=True)
     sh = b.shape
     b.close()
-    st1 = [s.stats() for s in sims]
+    st1 = [s.svd_stats() for s in sims]
     health = 0
     for s in sims:
         health |= s.health()
     snow = [k for k, (_, info) in enumerate(scenes) if info["type"] == capi.SNOW]
-    n_snow = sum(sims[k].total for k in snow)
-    rejects = sum(st1[k]["warm_rejects"] - st0[k]["warm_rejects"] for k in snow)
+    warm_calls = sum(st1[k]["warm"] - st0[k]["warm"] for k in snow)
+    rejects = sum(st1[k]["rejected"] - st0[k]["rejected"] for k in snow)
     # each scene on its own (mpm_run on its stream), same state
     seq = []
     for s in sims:
@@ -255,7 +239,8 @@ def run_batch(steps, warmup, warm=False, autotune_reps=2, seq_steps=5):
                   "g2p_threads": sh.g2p_threads, "sort_every": sh.sort_every, "graph": bool(sh.use_graph)},
         "autotune_ms": {k: {str(c): round(v, 4) for c, v in d.items()} for k, d in tune.items()},
         "warm_start": "snow scenes, 2 sweeps" if warm else "none",
-        "warm_rejected_fraction": round(rejects / max(1, n_snow * steps), 4) if warm else None,
+        "warm_svd_calls": int(warm_calls) if warm else None,
+        "warm_rejected_fraction": round(rejects / warm_calls, 4) if warm and warm_calls else None,
         "health_bits": health,
         "unbatched": {"value": round(n / (seq.sum() * 1e-3), 1), "scene_step_ms_p50": round(float(np.percentile(seq, 50)), 4),
                       "scene_step_ms_p99": round(float(np.percentile(seq, 99)), 4),
@@ -494,7 +479,7 @@ def main():
         return
     material = {"snow_spheres": "snow", "jello_cube": "jello", "water_dam": "water"}[a.workload]
     pre = preroll_steps(a.workload, a.steps, a.warmup, a.preroll)
-    res = run_gpu(a.workload, a.form, a.steps, a.warmup, a.sort_every, a.tile, preroll=pre, svd_stats=True)
+    res = run_gpu(a.workload, a.form, a.steps, a.warmup, a.sort_every, a.tile, preroll=pre)
     n = res["n"]
     value = n * a.steps / res["seconds"]
     out = {
@@ -518,11 +503,11 @@ def main():
         "fp64_svd_fallbacks": res["stats"]["fp64_fallbacks"],
         "sorts_in_timed_steps": int(res["stats"]["sorts_timed"]),
     }
-    if "svd_fraction" in res:
-        # particle-steps of the timed window whose G2P took the Jacobi SVD
-        # (trial F outside the clamp band: plastic flow) instead of the
-        # elastic path, and the particles compressed plastically by its end
-        out["svd_path_fraction"] = round(res["svd_fraction"], 5)
+    # particle-steps of the timed window whose G2P took the Jacobi SVD (snow:
+    # trial F outside the clamp band, plastic flow) instead of the elastic
+    # path, and the particles compressed plastically by its end
+    out["svd_path_fraction"] = round(res["svd_fraction"], 5)
+    if material == "snow":
         out["plastic_fraction"] = round(res["plastic_fraction"], 5)
     # the achievable bandwidth beside the published peak
     from mpm_cuda_amd import capi
@@ -543,16 +528,21 @@ def main():
                                      "roofline": roofline(r, material, form, ksteps, a.workload)}
         # learned warm start of the G2P SVD: network quaternion + 2 Jacobi
         # sweeps, cold 4-sweep SVD for every particle above the 1e-6 bound
-        r = run_gpu(a.workload, a.form, ksteps, kwu, warm_sweeps=2, preroll=kpre, svd_stats=True)
+        # (the network runs for the particles whose previous G2P took the SVD)
+        r = run_gpu(a.workload, a.form, ksteps, kwu, warm_sweeps=2, preroll=kpre)
+        rc = run_gpu(a.workload, a.form, ksteps, kwu, preroll=kpre)
         v = r["n"] * ksteps / r["seconds"]
-        svd_calls = r.get("svd_fraction", 0.0) * r["n"] * ksteps
+        sv = r["svd"]
         out["variants"]["warm_start"] = {
             "value": round(v, 1), "ms_per_step": round(r["seconds"] / ksteps * 1e3, 4), "form": a.form,
             "warm_sweeps": 2, "timed_steps": [kpre + kwu, kpre + kwu + ksteps],
-            "svd_path_fraction": round(r.get("svd_fraction", 0.0), 5),
-            "warm_svd_calls": int(round(svd_calls)), "warm_rejects": int(r["stats"]["warm_rejects_timed"]),
-            "rejected_fraction": round(r["stats"]["warm_rejects_timed"] / svd_calls, 4) if svd_calls else None,
-            "kernel_avg_ms": roofline(r, material, a.form, ksteps, a.workload)["kernel_avg_ms"]}
+            "svd_path_fraction": round(r["svd_fraction"], 5),
+            "svd_calls": sv["svd"], "warm_svd_calls": sv["warm"], "warm_rejects": sv["rejected"],
+            "rejected_fraction": round(sv["rejected"] / sv["warm"], 4) if sv["warm"] else None,
+            "kernel_avg_ms": roofline(r, material, a.form, ksteps, a.workload)["kernel_avg_ms"],
+            "cold_same_window": {"value": round(rc["n"] * ksteps / rc["seconds"], 1),
+                                 "ms_per_step": round(rc["seconds"] / ksteps * 1e3, 4),
+                                 "kernel_avg_ms": roofline(rc, material, a.form, ksteps, a.workload)["kernel_avg_ms"]}}
         if "plain" in out["variants"]:
             out["variants"]["speedup_over_plain"] = round(value / out["variants"]["plain"]["value"], 3)
     if not a.no_2d:

@@ -183,11 +183,14 @@ int mpm_add_object(mpm_sim *sim, const mpm_object *obj, int32_t *group);
 
 /* Most particles one simulator holds (all groups together): 2^30 - 1. */
 #define MPM_MAX_PARTICLES ((1ll << 30) - 1)
+/* Most particle groups one simulator holds. */
+#define MPM_MAX_GROUPS 256
 
 /* Add a group from explicit particles (positions xyz[3n]; vel[3n] or NULL
  * to use v0; particle_volume <= 0 means h^3).  initParticles,
  * MPMSimulator.cpp:10-120: F = I, Fp = I / J = 1, C = 0.  MPM_ERR_INVALID
- * when the simulator would hold more than MPM_MAX_PARTICLES. */
+ * when the simulator would hold more than MPM_MAX_PARTICLES particles or
+ * MPM_MAX_GROUPS groups. */
 int mpm_add_particles(mpm_sim *sim, const mpm_material *mat, int64_t n, const float *xyz,
                       const float *vel, const double v0[3], double particle_volume, int32_t *group);
 
@@ -225,6 +228,14 @@ int mpm_health(mpm_sim *sim, uint32_t *bits);
  * tiles of the last step, [7] particles of the last P2G that left their
  * tile's arena and took the atomic fallback. */
 int mpm_stats(mpm_sim *sim, double stats[8]);
+
+/* SVD work of the G2P since the simulator was created (32-bit device
+ * counters, no reference counterpart): [0] particle updates that took the
+ * Jacobi SVD (snow whose trial F left the clamp band, jello whose F has a
+ * singular value outside (0.5, 2)); [1] of those, started from a network
+ * quaternion (warm start); [2] warm results rejected for the cold SVD;
+ * [3] FP64 fallbacks of every SVD (also those of the stress outside G2P). */
+int mpm_svd_stats(mpm_sim *sim, double out[4]);
 
 int mpm_synchronize(mpm_sim *sim);
 

@@ -92,6 +92,7 @@ _SIGS = {
     "mpm_read_grid": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mpm_health": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "mpm_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mpm_svd_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mpm_synchronize": (C.c_int, [C.c_void_p]),
     "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -294,6 +295,13 @@ class Sim:
         check(lib().mpm_stats(self._h, s))
         return {"steps": s[0], "active_blocks": s[1], "fp64_fallbacks": s[2], "warm_rejects": s[3],
                 "sorts": s[4], "time": s[5], "tiles": s[6], "spilled": s[7]}
+
+    def svd_stats(self) -> dict:
+        """Cumulative G2P SVD counts: SVD-path updates, warm-started ones,
+        rejected warm results, FP64 fallbacks."""
+        s = (C.c_double * 4)()
+        check(lib().mpm_svd_stats(self._h, s))
+        return {"svd": int(s[0]), "warm": int(s[1]), "rejected": int(s[2]), "fp64_fallbacks": int(s[3])}
 
     def set_timing(self, on: bool = True) -> None:
         check(lib().mpm_set_timing(self._h, 1 if on else 0))

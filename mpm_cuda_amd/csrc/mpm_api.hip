@@ -95,6 +95,22 @@ static int ensure_capacity(mpm_sim *s, int64_t need)
     return MPM_OK;
 }
 
+// Newton steps R <- (R + R^-T) / 2 from R = F that bring every singular
+// value of the band [lo, hi] to within 1e-7 of 1 (the iteration maps a
+// singular value s to (s + 1/s) / 2; one FP32 ulp at 1 is 1.2e-7).  The snow
+// clamp band (0.975, 1.0075) takes two.
+static int polar_steps_for_band(double lo, double hi)
+{
+    int steps = 1;
+    for (double s0 : {lo, hi}) {
+        double s = s0;
+        int k = 0;
+        while (std::fabs(s - 1.0) > 1e-7 && k < 8) { s = 0.5 * (s + 1.0 / s); ++k; }
+        steps = std::max(steps, k);
+    }
+    return steps;
+}
+
 static MatP make_matp(const mpm_material &m, double mass, double vol)
 {
     MatP p;
@@ -110,6 +126,7 @@ static MatP make_matp(const mpm_material &m, double mass, double vol)
     p.eos_k = (float)m.eos_k;
     p.eos_gamma = (float)m.eos_gamma;
     p.warm_sweeps = 0;
+    p.polar_steps = polar_steps_for_band(1.0 - m.theta_c, 1.0 + m.theta_s);
     // squared wave speed at rest for the step-size controller: P waves
     // (lam + 2 mu) / rho, water's bulk modulus k gamma over rho
     const double rho = mass / vol;
@@ -254,6 +271,7 @@ int mpm_add_particles(mpm_sim *s, const mpm_material *mat, int64_t n, const floa
     if ((rc = flush_spill(s))) return rc;     // a P2G phase's spill scatter reads the state it saw
     // particle offsets inside a plane are 32-bit byte offsets on the device
     if (s->total + n > MPM_MAX_PARTICLES) return set_error(MPM_ERR_INVALID, "too many particles");
+    if ((int64_t)s->groups.size() >= MAX_GROUPS) return set_error(MPM_ERR_INVALID, "at most 256 particle groups");
     // every particle's stencil must lie inside the grid (MPMSimulator.cpp:210-222)
     const GridP &g = s->g;
     const float o[3] = {g.ox, g.oy, g.oz};
@@ -518,6 +536,9 @@ int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
             std::vector<uint4> desc(nt);
             MPM_HIP(hipMemcpy(desc.data(), (const uint4 *)s->tile_desc + s->group_tile_off[k], nt * sizeof(uint4),
                               hipMemcpyDeviceToHost));
+            std::vector<int32_t> entry((size_t)s->nblocks);
+            MPM_HIP(hipMemcpy(entry.data(), s->blk_tile + (int64_t)k * s->nblocks, entry.size() * sizeof(int32_t),
+                              hipMemcpyDeviceToHost));
             std::vector<float4> arena(ARENA_NODES);
             for (uint32_t t = 0; t < nt; ++t) {
                 if (desc[t].w >= (uint32_t)s->staging_cap) continue;
@@ -525,9 +546,13 @@ int mpm_read_grid(mpm_sim *s, int32_t field, float *out)
                                   ARENA_NODES * sizeof(float4), hipMemcpyDeviceToHost));
                 const int b = (int)desc[t].z;
                 const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+                // arena faces the P2G wrote (the rest of the slot's face nodes is stale)
+                const uint32_t f = (uint32_t)entry[b] >> TILE_FACE_SHIFT;
+                auto face_ok = [&](int i, int a) { return (i != 0 || (f >> (2 * a)) & 1u) && (i != 7 || (f >> (2 * a + 1)) & 1u); };
                 for (int q = 0; q < ARENA_NODES; ++q) {
                     const int x = bx * BLK - 1 + (q & 7), y = by * BLK - 1 + ((q >> 3) & 7), z = bz * BLK - 1 + (q >> 6);
                     if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                    if (!face_ok(q & 7, 0) || !face_ok((q >> 3) & 7, 1) || !face_ok(q >> 6, 2)) continue;
                     float4 &dst = mp[node_slot(g, x, y, z)];
                     dst.x += arena[q].x; dst.y += arena[q].y; dst.z += arena[q].z; dst.w += arena[q].w;
                 }
@@ -578,6 +603,19 @@ int mpm_stats(mpm_sim *s, double st[8])
     st[5] = mpm_current_time(s);
     st[6] = c[CT_TILES];
     st[7] = s->cfg.form == MPM_FORM_FUSED ? c[CT_SPILL] : c[CT_SPILLED];
+    return MPM_OK;
+}
+
+int mpm_svd_stats(mpm_sim *s, double out[4])
+{
+    if (!s || !out) return set_error(MPM_ERR_INVALID, "null argument");
+    uint32_t c[CT_COUNT];
+    MPM_HIP(hipMemcpyAsync(c, s->counters, sizeof c, hipMemcpyDeviceToHost, s->stream));
+    MPM_HIP(hipStreamSynchronize(s->stream));
+    out[0] = c[CT_SVD];
+    out[1] = c[CT_WARM];
+    out[2] = c[CT_REJECT];
+    out[3] = c[CT_FALLBACK];
     return MPM_OK;
 }
 
@@ -692,7 +730,7 @@ void mpm_destroy(mpm_sim *s)
     void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
                     s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
                     s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
-                    s->staging, s->blk_tile, s->spill, s->dtc};
+                    s->staging, s->blk_tile, s->spill, s->dtc, s->warm_list[0], s->warm_list[1], s->warm_cnt};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (s->spill_host) (void)hipHostFree(s->spill_host);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -719,21 +757,50 @@ hipEvent_t take_event(mpm_sim *s)
     return e;
 }
 
-int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv);
+int launch_mlp_infer_list(hipStream_t st, const float *params, const PartView &pv, const uint32_t *list,
+                          const uint32_t *count, uint32_t *next_count);
 
-// Warm start of the G2P SVDs: quaternions of the current F from the network
-// (snow and jello groups), and the sweep count for the G2P kernels.
+// The two particle lists of the warm start (and their per-group counts,
+// cleared), sized for the particle capacity.
+static int ensure_warm_lists(mpm_sim *s)
+{
+    if (s->warm_list[0] && s->warm_list_cap >= s->cap) return MPM_OK;
+    for (auto &l : s->warm_list) { if (l) (void)hipFree(l); l = nullptr; }
+    s->warm_list_cap = 0;
+    for (auto &l : s->warm_list) MPM_HIP(hipMalloc(&l, (size_t)s->cap * sizeof(uint32_t)));
+    if (!s->warm_cnt) MPM_HIP(hipMalloc(&s->warm_cnt, 2 * MAX_GROUPS * sizeof(uint32_t)));
+    s->warm_list_cap = s->cap;
+    return check_launch(launch_zero_u32(s->stream, s->warm_cnt, 2 * MAX_GROUPS), "zero");
+}
+
+// Warm start of the G2P SVDs: quaternions from the network for the particles
+// whose last G2P took the SVD (snow and jello groups), the list the coming
+// G2P fills, and the sweep count for the G2P kernels.  A particle entering
+// the SVD path for the first time has no fresh quaternion and is listed for
+// the next step; between steps its planes keep the last one inferred.
 static int prepare_warm(mpm_sim *s)
 {
     const bool on = s->cfg.warm_start == MPM_WARM_MLP && s->mlp_w && s->mlp_count == mpm_mlp_param_count();
+    int rc = on ? ensure_warm_lists(s) : MPM_OK;
+    if (rc) return rc;
+    const int p = s->warm_par;
     for (size_t k = 0; k < s->groups.size(); ++k) {
         MatP &m = s->groups[k].mat;
         m.warm_sweeps = (on && m.type != MPM_WATER) ? std::max(1, s->cfg.warm_sweeps) : 0;
+        m.wlist = nullptr;
+        m.wcount = nullptr;
         if (m.warm_sweeps > 0) {
-            int rc = check_launch(launch_mlp_infer(s->stream, s->mlp_w, view_of(s, (int)k)), "mlp_infer");
+            const int64_t off = s->groups[k].off;
+            uint32_t *cnt = s->warm_cnt + 2 * k;
+            rc = check_launch(launch_mlp_infer_list(s->stream, s->mlp_w, view_of(s, (int)k), s->warm_list[p] + off,
+                                                    cnt + p, cnt + (p ^ 1)),
+                              "mlp_infer_list");
             if (rc) return rc;
+            m.wlist = s->warm_list[p ^ 1] + off;
+            m.wcount = cnt + (p ^ 1);
         }
     }
+    if (on) s->warm_par ^= 1;
     return MPM_OK;
 }
 

@@ -312,7 +312,8 @@ __global__ void k_tile_flags(const uint32_t *blk_start, const uint32_t *blk_end,
 }
 
 // One tile per non-empty block: (first particle, count, block, tile id).
-// blk_tile maps the block to its global tile id (staging slot).
+// blk_tile maps the block to its global tile id (staging slot), every arena
+// face marked written until the block's first P2G.
 __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end, const uint32_t *tile_index,
                              int nblocks, uint4 *desc, uint32_t desc_cap, int32_t *blk_tile, uint32_t tile_base)
 {
@@ -322,7 +323,7 @@ __global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end,
     uint32_t t = tile_index[b];
     if (c > 0 && t < desc_cap) {
         desc[t] = make_uint4(blk_start[b], c, (uint32_t)b, tile_base + t);
-        blk_tile[b] = (int32_t)(tile_base + t);
+        blk_tile[b] = tile_entry(tile_base + t, TILE_ALL_FACES);
     } else {
         blk_tile[b] = -1;
     }
@@ -410,13 +411,15 @@ struct TileLds {
     uint32_t nitems;
 };
 
-// LDS of a FAST-form tile (one wave) of CH particles per pass: bin keys,
-// the sort and the arena (11.4 / 12.5 / 13.7 KB for CH = 256 / 384 / 512;
-// the particle records stay in HBM / cache).
+// LDS of a FAST-form tile (one wave) of CH particles per pass: the
+// particles' fx, bin keys, the sort and the arena (14.5 / 17.1 / 19.8 KB for
+// CH = 256 / 384 / 512; the rest of the particle records stays in HBM /
+// cache).
 template <int CH>
 struct TileKeys {
     static_assert(CH % 64 == 0 && CH <= 4095, "whole 64-particle passes; 12-bit item positions");
     float4 arena[A_SLOTS];
+    float fx[3][CH];            // phase A: position relative to the base node, in cells
     uint32_t items[CH];
     uint16_t perm[CH];
     uint16_t rnk[CH];
@@ -802,11 +805,9 @@ __device__ __forceinline__ void scatter_items(TileLds &L, float4 *arena, int bat
 // with hm = h m, q0 = m v - hB fx, and G (xx, yy, zz, xy, xz, yz) = -dt V
 // tau, water: G[0] = dt V p J (MPMCUDA.cu:345-353, 391-414).
 template <int MAT>
-__device__ __forceinline__ void record_from_raw(const GridP &g, const MatP &mat, float dt, float hm, const RawP &r,
-                                                float fx[3], float q0[3], float hB[9], float Gs[6])
+__device__ __forceinline__ void record_from_raw(const MatP &mat, float dt, float hm, const RawP &r,
+                                                const float fx[3], float q0[3], float hB[9], float Gs[6])
 {
-    int b[3];
-    (void)base_fx(g, r.x, b, fx);
 #pragma unroll
     for (int k = 0; k < 9; ++k) hB[k] = hm * r.C[k];
 #pragma unroll
@@ -829,9 +830,28 @@ __device__ __forceinline__ void record_from_raw(const GridP &g, const MatP &mat,
     }
 }
 
+// v, C and the stored stress (water: J) of particle i: the FAST P2G's phase
+// C inputs besides fx, which phase A left in LDS.
+template <int MAT>
+__device__ __forceinline__ void load_vcs(const PartView &pv, int i, RawP &r)
+{
+    const uint32_t u = (uint32_t)i;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r.v[a] = ld_plane(pv, PL_V + a, u);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.C[k] = ld_plane(pv, PL_C + k, u);
+    if (MAT == MPM_WATER) {
+        r.s[0] = ld_plane(pv, PL_S, u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r.s[k] = ld_plane(pv, PL_TAU + k, u);
+    }
+}
+
 // Phase C for one wave, FAST form: the item's particles are loaded again
 // from the particle planes (the tile's lines are in cache from phase A or
-// arrive now; no LDS records) and their records built in registers.
+// arrive now; no LDS records) and their records built in registers with the
+// fx phase A computed.
 template <int MAT, class TK>
 __device__ __forceinline__ void scatter_items_reload(TK &L, float4 *arena, int batch, const GridP &g,
                                                      const MatP &mat, const PartView &pv, int i0, float dt, float hm)
@@ -852,13 +872,23 @@ __device__ __forceinline__ void scatter_items_reload(TK &L, float4 *arena, int b
     // lanes without a next particle reload a valid one (one basic block)
     auto slot_of = [&](uint32_t r) -> int { return cnt ? (int)L.perm[first + min(r, cnt - 1u)] : 0; };
     RawP nxt;
-    load_raw<MAT>(pv, i0 + slot_of(0), nxt);
+    float nfx[3];
+    int sl = slot_of(0);
+    load_vcs<MAT>(pv, i0 + sl, nxt);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) nfx[a] = L.fx[a][sl];
     for (uint32_t r = 0; r < rounds; ++r) {
         const RawP raw = nxt;
-        if (r + 1 < rounds) load_raw<MAT>(pv, i0 + slot_of(r + 1), nxt);
+        const float fx[3] = {nfx[0], nfx[1], nfx[2]};
+        if (r + 1 < rounds) {
+            sl = slot_of(r + 1);
+            load_vcs<MAT>(pv, i0 + sl, nxt);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) nfx[a] = L.fx[a][sl];
+        }
         if (r < cnt) {
-            float fx[3], q0[3], hB[9], Gs[6];
-            record_from_raw<MAT>(g, mat, dt, hm, raw, fx, q0, hB, Gs);
+            float q0[3], hB[9], Gs[6];
+            record_from_raw<MAT>(mat, dt, hm, raw, fx, q0, hB, Gs);
             accum_regs<MAT>(fx, q0, hB, Gs, g.inv_h, mat.mass, acc);
         }
     }
@@ -887,10 +917,12 @@ __device__ __forceinline__ void tile_clear(TileLds &L)
 // Sum the waves' arenas into staging slot `slot` (logical node order) and
 // mark the node blocks that received mass.
 __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const TileLds &L, float4 *staging,
-                                         uint32_t slot, bool staged, uint32_t *flags)
+                                         uint32_t slot, bool staged, uint32_t *flags, int32_t *blk_entry)
 {
     __syncthreads();
     if (staged) {
+        // the whole arena is written: every face
+        if (threadIdx.x == 0) *gp(blk_entry) = tile_entry(slot, TILE_ALL_FACES);
         const int64_t dst = (int64_t)slot * ARENA_NODES;
         for (int k = threadIdx.x; k < ARENA_NODES; k += TILE_THREADS) {
             const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
@@ -920,7 +952,8 @@ __device__ __forceinline__ void tile_end(const GridP &g, const TileGeo &t, const
 template <int MAT, int CH>
 __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, const PartView &pv, const uint4 d,
                                              uint32_t staging_cap, float dt, float4 *__restrict__ staging,
-                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, TileKeys<CH> &L)
+                                             uint32_t *spill, uint32_t *flags, uint32_t *counters, int32_t *blk_tile,
+                                             TileKeys<CH> &L)
 {
     uint32_t bits = 0;
     const TileGeo t = tile_geo(g, d.z);
@@ -929,6 +962,7 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     const int lane = threadIdx.x & 63;
     for (int k = lane; k < A_SLOTS; k += 64) L.arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     constexpr int PASSES = CH / 64;
+    uint32_t lane_faces = 0;    // arena faces this lane's particles' stencils reach (see tile_entry)
     for (uint32_t c0 = 0; c0 < d.y; c0 += CH) {
         const int nc = (int)min((uint32_t)CH, d.y - c0);
         // A: all position loads of the chunk first (lanes past its end reload
@@ -946,8 +980,24 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
             const int slot = it * 64 + lane;
             const bool active = slot < nc;
             int b[3] = {0, 0, 0};
-            if (active) bits |= base_only(g, xs[it], b);
-            bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, b, staged, spill, counters, flags);
+            float f[3];
+            if (active) {
+                bits |= base_fx(g, xs[it], b, f);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) L.fx[a][slot] = f[a];
+            }
+            const bool listed = bin_key(g, L, t, active, slot, (int)(d.x + c0) + slot, b, staged, spill, counters,
+                                        flags);
+            // a base on the arena's first (last) bin position reaches its
+            // face at index 0 (7)
+            if (active && !listed) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int rel = b[a] - (a == 0 ? t.ax : (a == 1 ? t.ay : t.az));
+                    lane_faces |= (rel == 0 ? 1u : 0u) << (2 * a);
+                    lane_faces |= (rel == ARENA - 3 ? 2u : 0u) << (2 * a);
+                }
+            }
         }
         __asm__ volatile("" ::: "memory");
         // B and C
@@ -956,18 +1006,33 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
         for (int b = 0; b < nb; ++b) scatter_items_reload<MAT>(L, L.arena, b, g, mat, pv, (int)(d.x + c0), dt, hm);
         __asm__ volatile("" ::: "memory");
     }
-    // the arena into the tile's staging slot (logical node order); mark the
-    // node blocks that received mass
+    // the arena into the tile's staging slot (logical node order): indices
+    // 1..6 of each axis and the faces some stencil reached (every other node
+    // is zero); mark the node blocks that received mass
     if (staged) {
+        uint32_t faces = 0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            if (__ballot((lane_faces >> f) & 1u)) faces |= 1u << f;
         const int64_t dst = (int64_t)d.w * ARENA_NODES;
-        for (int k = lane; k < ARENA_NODES; k += 64) {
-            const int kx = k & 7, ky = (k >> 3) & 7, kz = k >> 6;
-            const float4 v = L.arena[a_slot(kx, ky, kz)];
-            gstore4(staging, dst + k, v);
+        const int lo[3] = {(faces & 1u) ? 0 : 1, (faces & 4u) ? 0 : 1, (faces & 16u) ? 0 : 1};
+        const int hi[3] = {(faces & 2u) ? 7 : 6, (faces & 8u) ? 7 : 6, (faces & 32u) ? 7 : 6};
+        const int kx = lane & 7, ky = lane >> 3;
+        const bool row = kx >= lo[0] && kx <= hi[0] && ky >= lo[1] && ky <= hi[1];
+        // all eight z planes are read from LDS back to back (the accumulators
+        // are dead here), then the planes in range are stored
+        float4 v[ARENA];
+#pragma unroll
+        for (int kz = 0; kz < ARENA; ++kz) v[kz] = L.arena[a_slot(kx, ky, kz)];
+#pragma unroll
+        for (int kz = 0; kz < ARENA; ++kz) {
+            if (!row || kz < lo[2] || kz > hi[2]) continue;
+            gstore4(staging, dst + kz * 64 + lane, v[kz]);
             const int nx_ = t.ax + kx, ny_ = t.ay + ky, nz_ = t.az + kz;
-            if (v.x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
+            if (v[kz].x != 0.f && nx_ >= 0 && ny_ >= 0 && nz_ >= 0 && nx_ < g.nx && ny_ < g.ny && nz_ < g.nz)
                 gp(flags)[((int64_t)(nz_ >> 2) * g.nby + (ny_ >> 2)) * g.nbx + (nx_ >> 2)] = 1u;
         }
+        if (lane == 0) gp(blk_tile)[d.z] = tile_entry(d.w, faces);
     }
     __asm__ volatile("" ::: "memory");
     return bits;
@@ -979,7 +1044,8 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
                                                             const uint32_t *__restrict__ tile_count, uint32_t desc_cap,
                                                             uint32_t staging_cap, float dt,
                                                             float4 *__restrict__ staging, uint32_t *spill,
-                                                            uint32_t *flags, uint32_t *counters, int clear_active)
+                                                            uint32_t *flags, uint32_t *counters, int32_t *blk_tile,
+                                                            int clear_active)
 {
     __shared__ TileKeys<CH> L;
     const uint32_t nt = min(*tile_count, desc_cap);
@@ -995,7 +1061,8 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
     for (uint32_t k8 = blockIdx.x; k8 < nt8; k8 += gridDim.x) {
         const uint32_t tt = xcd_spread(k8, nt8);
         if (tt >= nt) continue;
-        bits |= p2g_tile<MAT, CH>(g, mat, pv, desc[tt], staging_cap, dt, staging, spill, flags, counters, L);
+        bits |= p2g_tile<MAT, CH>(g, mat, pv, desc[tt], staging_cap, dt, staging, spill, flags, counters, blk_tile,
+                                  L);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
@@ -1035,7 +1102,8 @@ __global__ void __launch_bounds__(P2G_THREADS) __attribute__((amdgpu_waves_per_e
         const BatchSlice &b = tab[sid];
         const uint4 d = gload_u4(b.desc, tt - (sid ? s_end[sid - 1] : 0u));
         const uint32_t bits =
-            p2g_tile<MAT, CH>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters, L);
+            p2g_tile<MAT, CH>(b.g, b.mat, b.pv, d, b.staging_cap, b.dt, b.staging, b.spill, b.flags, b.counters,
+                              b.blk_tile, L);
         if (bits) gatomic_or(b.counters + CT_HEALTH, bits);
     }
     if (blockIdx.x == 0)
@@ -1057,7 +1125,8 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
                                                               uint32_t desc_cap, uint32_t staging_cap, float dt,
                                                               float dt_next, const float4 *__restrict__ gv,
                                                               float4 *__restrict__ staging, uint32_t *spill,
-                                                              uint32_t *flags, uint32_t *counters, int sweeps)
+                                                              uint32_t *flags, uint32_t *counters, int sweeps,
+                                                              int32_t *blk_tile)
 {
     __shared__ TileLds L;
     __shared__ float4 garena[A_SLOTS];
@@ -1116,7 +1185,7 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
             }
             scatter_chunk<MAT>(L, nc, g.inv_h, mat.mass);
         }
-        tile_end(g, t, L, staging, d.w, staged, flags);
+        tile_end(g, t, L, staging, d.w, staged, flags, blk_tile + d.z);
     }
     if (bits) atomicOr(counters + CT_HEALTH, bits);
     if (threadIdx.x == 0 && blockIdx.x == 0) counters[CT_TILES] = nt;
@@ -1186,7 +1255,11 @@ int sort_all(mpm_sim *s)
     while ((1ll << (bits - KEY_LOW)) < s->nblocks) ++bits;
     uint64_t plane_mask = (1ull << PL_COUNT) - 1ull;
     if (!s->tau_valid) plane_mask &= ~(((1ull << 6) - 1ull) << PL_TAU);
-    plane_mask &= ~(((1ull << 4) - 1ull) << PL_Q);       // recomputed before every G2P
+    // warm-start quaternions travel with their particles while the warm start
+    // lists particles (the lists' indices are dropped: they name slots of the
+    // old order)
+    if (!s->warm_cnt) plane_mask &= ~(((1ull << 4) - 1ull) << PL_Q);
+    else if ((rc = launch_zero_u32(st, s->warm_cnt, 2 * MAX_GROUPS))) return hip_fail((hipError_t)rc, "zero");
     const int nb = (int)s->nblocks;
     for (size_t k = 0; k < s->groups.size(); ++k) {
         PartView pv = view_of(s, (int)k);
@@ -1250,7 +1323,7 @@ static bool sort_due(mpm_sim *s)
     if (s->spill_launch[slot] != want) return false;
     // wait until every group's spill kernel of that launch has published its
     // count (launch number written after the count with release order)
-    const uint32_t *seq = s->spill_host + mpm_sim::SPILL_SLOTS * 256 + slot * 256;
+    const uint32_t *seq = s->spill_host + mpm_sim::SPILL_SLOTS * MAX_GROUPS + slot * MAX_GROUPS;
     const auto t0 = std::chrono::steady_clock::now();
     for (size_t k = 0; k < s->groups.size(); ++k) {
         while (__atomic_load_n(&seq[k], __ATOMIC_ACQUIRE) != (uint32_t)want) {
@@ -1260,7 +1333,7 @@ static bool sort_due(mpm_sim *s)
     }
     s->spill_launch[slot] = -1;
     uint64_t tot = 0;
-    for (size_t k = 0; k < s->groups.size(); ++k) tot += s->spill_host[slot * 256 + k];
+    for (size_t k = 0; k < s->groups.size(); ++k) tot += s->spill_host[slot * MAX_GROUPS + k];
     if (tot <= std::max<uint64_t>(64, (uint64_t)s->total / 10000)) return false;
     s->sort_interval = std::max<int64_t>(1, s->steps_since_sort - 3);
     return true;
@@ -1271,13 +1344,13 @@ static bool sort_due(mpm_sim *s)
 static int spill_host_word(mpm_sim *s, int group, uint32_t **word)
 {
     if (!s->spill_host) {
-        constexpr size_t words = 2 * mpm_sim::SPILL_SLOTS * 256;
+        constexpr size_t words = 2 * mpm_sim::SPILL_SLOTS * MAX_GROUPS;
         MPM_HIP(hipHostMalloc((void **)&s->spill_host, words * sizeof(uint32_t), hipHostMallocCoherent));
         std::memset(s->spill_host, 0, words / 2 * sizeof(uint32_t));
         std::memset(s->spill_host + words / 2, 0xff, words / 2 * sizeof(uint32_t));   // no launch yet
     }
     const int slot = (int)(s->tile_launches % mpm_sim::SPILL_SLOTS);
-    *word = s->spill_host + slot * 256 + group;
+    *word = s->spill_host + slot * MAX_GROUPS + group;
     return MPM_OK;
 }
 
@@ -1333,15 +1406,16 @@ int fast_ready(mpm_sim *s)
 
 // Launchers of the FAST P2G kernels by material and tile shape.
 using BlockLaunch = void (*)(dim3, dim3, hipStream_t, const GridP &, const MatP &, const PartView &, const uint4 *,
-                             const uint32_t *, uint32_t, uint32_t, float, float4 *, uint32_t *, uint32_t *, uint32_t *);
+                             const uint32_t *, uint32_t, uint32_t, float, float4 *, uint32_t *, uint32_t *, uint32_t *,
+                             int32_t *);
 using BatchLaunch = void (*)(dim3, dim3, hipStream_t, const BatchSlice *, int);
 template <int MAT, int CH>
 static void launch_block(dim3 gr, dim3 bl, hipStream_t st, const GridP &g, const MatP &m, const PartView &pv,
                          const uint4 *desc, const uint32_t *tc, uint32_t cap, uint32_t scap, float dt,
-                         float4 *staging, uint32_t *spill, uint32_t *flags, uint32_t *counters)
+                         float4 *staging, uint32_t *spill, uint32_t *flags, uint32_t *counters, int32_t *blk_tile)
 {
     hipLaunchKernelGGL((k_p2g_block<MAT, CH>), gr, bl, 0, st, g, m, pv, desc, tc, cap, scap, dt, staging, spill,
-                       flags, counters, 1);
+                       flags, counters, blk_tile, 1);
 }
 template <int MAT, int CH>
 static void launch_batch(dim3 gr, dim3 bl, hipStream_t st, const BatchSlice *tab, int n)
@@ -1399,7 +1473,8 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
     {
     PhaseTimer tk(s, TK_P2G);
     block_launcher(m.type, tile_shape(s->cfg.tile))(gr, bl, s->stream, s->g, m, pv, desc, tc, cap, scap, dt,
-                                                    s->staging, s->spill, s->flags, s->counters);
+                                                    s->staging, s->spill, s->flags, s->counters,
+                                                    s->blk_tile + (int64_t)group * s->nblocks);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2g_block");
@@ -1414,7 +1489,7 @@ int launch_p2g_tiles(mpm_sim *s, int group, float dt)
         return note_spills(s, group);
     }
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp, s->flags,
-                          hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches);
+                          hw, hw + mpm_sim::SPILL_SLOTS * MAX_GROUPS, (uint32_t)s->tile_launches);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }
@@ -1426,7 +1501,7 @@ int spill_compact(mpm_sim *s)
     const PartView pv = view_of(s, 0);
     const int rc = launch_spill_compact(s->stream, s->spill_g, s->groups[0].mat, pv, s->spill_dt, s->spill,
                                         (uint32_t)pv.n, s->counters, s->grid_mp, s->flags, (int)s->nblocks,
-                                        s->list, s->spill_hw, s->spill_hw + mpm_sim::SPILL_SLOTS * 256,
+                                        s->list, s->spill_hw, s->spill_hw + mpm_sim::SPILL_SLOTS * MAX_GROUPS,
                                         s->spill_seq);
     return rc ? hip_fail((hipError_t)rc, "spill_compact") : MPM_OK;
 }
@@ -1438,7 +1513,7 @@ int flush_spill(mpm_sim *s)
     const PartView pv = view_of(s, 0);
     const int rc = launch_p2g_spill(s->stream, s->spill_g, s->groups[0].mat, pv, s->spill_dt, s->spill, (uint32_t)pv.n,
                                     s->counters, s->grid_mp, s->flags, s->spill_hw,
-                                    s->spill_hw + mpm_sim::SPILL_SLOTS * 256, s->spill_seq);
+                                    s->spill_hw + mpm_sim::SPILL_SLOTS * MAX_GROUPS, s->spill_seq);
     return rc ? hip_fail((hipError_t)rc, "p2g_spill") : MPM_OK;
 }
 
@@ -1454,6 +1529,7 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     const uint32_t cap = (uint32_t)bound;
     const uint32_t scap = (uint32_t)s->staging_cap;
     const int sw = s->cfg.svd_sweeps;
+    int32_t *bt = s->blk_tile + (int64_t)group * s->nblocks;
     int rc = launch_zero_u32(s->stream, s->counters + CT_SPILL, 1);
     if (rc) return hip_fail((hipError_t)rc, "zero");
     uint32_t *hw = nullptr;
@@ -1462,15 +1538,15 @@ int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next)
     {
     PhaseTimer tk(s, TK_G2P);
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_g2p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_g2p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
-    default: hipLaunchKernelGGL(k_g2p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_g2p2g_block<MPM_JELLO>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw, bt); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_g2p2g_block<MPM_SNOW>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw, bt); break;
+    default: hipLaunchKernelGGL(k_g2p2g_block<MPM_WATER>, gr, bl, 0, s->stream, s->g, m, pv, desc, tc, cap, scap, dt, dt_next, s->grid_v, s->staging, s->spill, s->flags, s->counters, sw, bt); break;
     }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "g2p2g_block");
     rc = launch_p2g_spill(s->stream, s->g, m, pv, dt_next, s->spill, (uint32_t)pv.n, s->counters, s->grid_mp,
-                          s->flags, hw, hw + mpm_sim::SPILL_SLOTS * 256, (uint32_t)s->tile_launches);
+                          s->flags, hw, hw + mpm_sim::SPILL_SLOTS * MAX_GROUPS, (uint32_t)s->tile_launches);
     if (rc) return hip_fail((hipError_t)rc, "p2g_spill");
     return note_spills(s, group);
 }
@@ -1591,6 +1667,8 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
     memset(&b, 0, sizeof b);
     b.g = s->g;
     b.mat = s->groups[0].mat;
+    b.mat.wlist = nullptr;          // batched scenes infer every particle's quaternion
+    b.mat.wcount = nullptr;
     b.pv = view_of(s, 0);
     b.dt = (float)s->cfg.dt;
     b.sweeps = s->cfg.svd_sweeps;

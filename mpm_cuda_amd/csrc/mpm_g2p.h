@@ -226,21 +226,20 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         if (sv_in_band<float>(Ft, 1.0f - mat.theta_c, 1.0f + mat.theta_s)) {
             // no singular value reaches the clamp: F_e = F_tr, J_p unchanged
             // (its plane is left as it is); the stress needs only the polar
-            // rotation, three Newton steps instead of the SVD
+            // rotation, a few Newton steps instead of the SVD
 #pragma unroll
             for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Ft[k]);
             if (!(Sp > 0.f)) bits |= MPM_H_DET_NONPOS;
             if (want_tau) {
                 float R[9];
-                polar_newton<float>(Ft, R, 3);
+                polar_newton<float>(Ft, R, mat.polar_steps);
                 const float hc = expf(mat.xi * (1.0f - Sp));
                 fcr_tau_polar<float>(Ft, R, mat.mu * hc, mat.lam * hc, out.tau);
             }
             out.S = Sp;
         } else {
             float U[9], sg[3], V[9];
-            if (mat.warm_sweeps > 0) svd_warm(pv, i, Ft, U, sg, V, mat.warm_sweeps, sweeps, counters);
-            else svd_checked(Ft, U, sg, V, sweeps, counters);
+            svd_g2p(mat, pv, i, Ft, U, sg, V, sweeps, counters);
             float ratio = 1.0f;
             float sc[3];
 #pragma unroll

@@ -55,6 +55,16 @@ struct MatP {
     float eos_k, eos_gamma;
     int warm_sweeps;    // > 0: G2P SVDs start from the stored quaternion (PL_Q) with this many sweeps
     float c2;           // squared elastic wave speed at rest: (lam + 2 mu) / rho, water k gamma / rho
+    // Newton steps of the polar rotation on the snow elastic path (singular
+    // values inside the clamp band): enough to bring both band edges to
+    // within 1e-7 of 1
+    int polar_steps = 3;
+    // warm start: the G2P lists the (group-local) indices of the particles
+    // that take the SVD into wlist, counted in *wcount; the next step's
+    // inference runs over that list only.  Null: no list (every particle's
+    // quaternion is inferred).
+    uint32_t *wlist = nullptr;
+    uint32_t *wcount = nullptr;
 };
 
 // Per-launch view of one group's particles.
@@ -78,8 +88,31 @@ enum Counter : int {
     CT_SPILLED = 7,     // spill count of the last block compaction (CT_SPILL is cleared there)
     CT_SPEED = 8,       // step-size controller: max_p |v_p| + c_p (float bits), since the last control
     CT_DEFORM = 9,      // step-size controller: max_p max(J, 1/J) (float bits)
+    CT_SVD = 10,        // G2P particle updates that took the Jacobi SVD (cumulative)
+    CT_WARM = 11,       // of those, started from a network quaternion (cumulative)
     CT_COUNT = 16
 };
+
+// Entry of the block -> staging slot map (blk_tile): -1 when the block has
+// no tile; otherwise the slot in bits 0..23 (slots at or above the staging
+// capacity are not staged; larger numbers saturate at 2^24 - 1) and in bits
+// 24..29 the faces of the block's 8^3 staging arena that its last P2G wrote:
+// bit 2a the low face (arena index 0) and bit 2a + 1 the high face (index 7)
+// of axis a.  Arena indices 1..6 of every axis are always written.  A
+// particle of the block whose base lies in the block writes indices 1..6
+// only; the faces carry the particles that drifted one cell out of it since
+// the last sort.
+constexpr uint32_t TILE_SLOT_MASK = 0xFFFFFFu;
+constexpr int TILE_FACE_SHIFT = 24;
+constexpr uint32_t TILE_ALL_FACES = 0x3Fu;
+__host__ __device__ inline int32_t tile_entry(uint32_t slot, uint32_t faces)
+{
+    return (int32_t)((slot < TILE_SLOT_MASK ? slot : TILE_SLOT_MASK) | (faces << TILE_FACE_SHIFT));
+}
+
+// Particle groups per simulator (per-group host words of the spill counts,
+// warm-start list counts).
+constexpr int MAX_GROUPS = MPM_MAX_GROUPS;
 
 // Device state of the step-size controller (one per simulator).
 struct DtCtl {
@@ -151,6 +184,29 @@ __device__ __forceinline__ float step_dt(const GridP &g, float dt) { return g.dt
 // patterns as unsigned integers.  The atomic is skipped when *dst already
 // holds at least v (a plain read first: once a large value is posted, most
 // waves of a launch leave the single contended word alone).
+// The active lanes of the wave (divergent code included) add their count of
+// `pred` to *dst with one atomic.
+__device__ __forceinline__ void wave_count(uint32_t *dst, bool pred)
+{
+    const uint64_t m = __ballot(pred);
+    const int first = __ffsll((unsigned long long)__ballot(1)) - 1;
+    if (m && (int)(threadIdx.x & 63) == first) (void)gatomic_add(dst, (uint32_t)__popcll(m));
+}
+
+// The active lanes of the wave append their `v` to list[*count ...) (one
+// atomic per wave; the order within a wave follows the lanes).
+__device__ __forceinline__ void wave_append(uint32_t *list, uint32_t *count, uint32_t v)
+{
+    const uint64_t m = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int first = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = gatomic_add(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, first);
+    const uint64_t below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
+    gp(list)[base + (uint32_t)__popcll(below)] = v;
+}
+
 __device__ __forceinline__ void wave_max_to(uint32_t *dst, float v)
 {
 #pragma unroll
@@ -200,7 +256,7 @@ struct BatchSlice {
     uint32_t *flags, *list, *counters, *spill;
     const uint4 *desc;              // P2G tiles of the last sort
     const uint32_t *tile_count;
-    const int32_t *blk_tile;
+    int32_t *blk_tile;              // tile_entry per block (the P2G updates its faces)
     uint32_t desc_cap, staging_cap;
     int64_t nblocks;
 };
@@ -281,7 +337,7 @@ struct mpm_sim {
     uint32_t *tile_counts = nullptr;        // per group: tiles built by the last sort
     float4 *staging = nullptr;              // per tile: 8^3-node P2G arena written back without atomics
     int64_t staging_cap = 0;                // tiles the staging buffer holds
-    int32_t *blk_tile = nullptr;            // [group][block]: staging tile of the block, -1 if none
+    int32_t *blk_tile = nullptr;            // [group][block]: tile_entry of the block (-1: no tile)
     uint32_t *spill = nullptr;              // indices of particles for the atomic P2G fallback
     bool primed = false;                    // FUSED: staging holds the scatter of the current state
     float primed_dt = 0.f;                  // FUSED: dt the primed scatter used for the force term
@@ -294,7 +350,7 @@ struct mpm_sim {
     // keeps at most two steps queued ahead of the device, and no event
     // packet sits in the stream).
     static constexpr int SPILL_SLOTS = 2;
-    uint32_t *spill_host = nullptr;         // [slot][256 groups] counts, then [slot][256 groups] launch numbers
+    uint32_t *spill_host = nullptr;         // [slot][MAX_GROUPS] counts, then [slot][MAX_GROUPS] launch numbers
     int64_t spill_launch[SPILL_SLOTS] = {-1, -1};   // tile launch number held by the slot
     int64_t tile_launches = 0;
     bool counters_clean = false;            // CT_SPILL cleared by the last grid phase (FAST form)
@@ -311,6 +367,14 @@ struct mpm_sim {
     // changes with every mpm_set_warm_start_weights
     float *mlp_w = nullptr;
     int64_t mlp_count = 0;
+    // warm start by list: warm_list[p] holds, per group at the group's
+    // offset, the particles whose last G2P took the SVD; warm_cnt[2 k + p]
+    // counts group k's entries.  The G2P appends to parity warm_par ^ 1
+    // while the inference before it reads parity warm_par.
+    uint32_t *warm_list[2] = {nullptr, nullptr};
+    uint32_t *warm_cnt = nullptr;
+    int64_t warm_list_cap = 0;
+    int warm_par = 0;
     std::vector<float> mlp_host;
     int64_t mlp_version = 0;
     // step-size controller (cfg.dt_mode != MPM_DT_FIXED)

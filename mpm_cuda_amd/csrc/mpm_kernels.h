@@ -135,7 +135,9 @@ __device__ __forceinline__ void svd_checked(const float F[9], float U[9], float 
 // R(q / |q|), `warm_sweeps` Jacobi sweeps.  The result is accepted when the
 // columns of F V are orthogonal to 1e-6 (largest |cos| between columns) and
 // F is well conditioned; otherwise the cold SVD (with its FP64 fallback)
-// replaces it and the rejection is counted.
+// replaces it and the rejection is counted.  A particle with no quaternion
+// (all zero: not inferred since its planes were cleared) takes the cold SVD
+// directly; the attempts with one are counted in CT_WARM.
 constexpr float kWarmTol = 1e-6f;
 
 __device__ __forceinline__ void svd_warm(const PartView &pv, int i, const float F[9], float U[9], float s[3],
@@ -143,17 +145,33 @@ __device__ __forceinline__ void svd_warm(const PartView &pv, int i, const float 
 {
     float q[4] = {ld(pv, PL_Q, i), ld(pv, PL_Q + 1, i), ld(pv, PL_Q + 2, i), ld(pv, PL_Q + 3, i)};
     const float qn2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-    if (qn2 > 1e-20f && qn2 < 1e20f) {
+    const bool has_q = qn2 > 0.0f;
+    wave_count(counters + CT_WARM, has_q);
+    bool ok = false;
+    if (has_q && qn2 > 1e-20f && qn2 < 1e20f) {
         const float inv = r_rsqrt(qn2);
 #pragma unroll
         for (int k = 0; k < 4; ++k) q[k] *= inv;
         float V0[9];
         quat_to_mat<float>(q, V0);
         const float off = svd3<float>(F, U, s, V, warm_sweeps, V0);
-        if (off <= kWarmTol && s[2] > kFallbackRatio * s[0]) return;
+        ok = off <= kWarmTol && s[2] > kFallbackRatio * s[0];
     }
-    gatomic_add(counters + CT_REJECT, 1u);
+    if (ok) return;
+    wave_count(counters + CT_REJECT, has_q);
     svd_checked(F, U, s, V, sweeps, counters);
+}
+
+// The Jacobi SVD of a G2P update (particle i of pv): counted in CT_SVD,
+// warm-started when the material's warm_sweeps > 0, and the particle listed
+// for the next step's inference when the material carries a list.
+__device__ __forceinline__ void svd_g2p(const MatP &m, const PartView &pv, int i, const float F[9], float U[9],
+                                        float s[3], float V[9], int sweeps, uint32_t *counters)
+{
+    wave_count(counters + CT_SVD, true);
+    if (m.warm_sweeps > 0) svd_warm(pv, i, F, U, s, V, m.warm_sweeps, sweeps, counters);
+    else svd_checked(F, U, s, V, sweeps, counters);
+    if (m.wlist) wave_append(m.wlist, m.wcount, (uint32_t)i);
 }
 
 // Kirchhoff stress of an elastic / snow particle from its deformation
@@ -178,7 +196,7 @@ __device__ __forceinline__ void stress_from_F(const MatP &m, const float F[9], f
         return;
     }
     float U[9], s[3], V[9];
-    if (warm_pv && m.warm_sweeps > 0) svd_warm(*warm_pv, warm_i, F, U, s, V, m.warm_sweeps, sweeps, counters);
+    if (warm_pv) svd_g2p(m, *warm_pv, warm_i, F, U, s, V, sweeps, counters);
     else svd_checked(F, U, s, V, sweeps, counters);
     fcr_tau_from_svd<float>(U, s, mu, lam, tau6);
 }
@@ -231,57 +249,16 @@ __device__ __forceinline__ void load_scatter(const PartView &pv, int i, const Ma
     }
 }
 
-// The FAST-form P2G inputs of one particle as loaded (x, v, C and the stored
-// stress; water: J in s[0]).
+// The FAST-form P2G inputs of one particle besides its position: v, C and
+// the stored stress (water: J in s[0]).
 struct RawP {
-    float x[3], v[3], C[9], s[6];
+    float v[3], C[9], s[6];
 };
 // Plane p of particle i as (uniform plane base) + (32-bit lane offset): the
 // loads then take the scalar-base form and share one offset register.
 __device__ __forceinline__ float ld_plane(const PartView &pv, int plane, uint32_t i)
 {
     return *plane_at(pv, plane, i);
-}
-template <int MAT>
-__device__ __forceinline__ void load_raw(const PartView &pv, int i, RawP &r)
-{
-    const uint32_t u = (uint32_t)i;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) r.x[a] = ld_plane(pv, PL_X + a, u);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) r.v[a] = ld_plane(pv, PL_V + a, u);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) r.C[k] = ld_plane(pv, PL_C + k, u);
-    if (MAT == MPM_WATER) {
-        r.s[0] = ld_plane(pv, PL_S, u);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) r.s[k] = ld_plane(pv, PL_TAU + k, u);
-    }
-}
-
-// load_scatter's arithmetic on loaded inputs (stress stored).
-template <int MAT>
-__device__ __forceinline__ void scatter_from_raw(const RawP &r, const MatP &mat, float dt, Scatter &sc)
-{
-    sc.m = mat.mass;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) sc.a[a] = mat.mass * r.v[a];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) sc.B[k] = mat.mass * r.C[k];
-    if (MAT == MPM_WATER) {
-        const float J = r.s[0];
-        const float pr = mat.eos_k * (powf(J, -mat.eos_gamma) - 1.0f);   // MPMCUDA.cu:353
-        const float c = dt * mat.vol * pr * J;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) sc.G[k] = (k % 4 == 0) ? c : 0.0f;
-    } else {
-        const float c = -dt * mat.vol;
-        sc.G[0] = c * r.s[0]; sc.G[4] = c * r.s[1]; sc.G[8] = c * r.s[2];
-        sc.G[1] = sc.G[3] = c * r.s[3];
-        sc.G[2] = sc.G[6] = c * r.s[4];
-        sc.G[5] = sc.G[7] = c * r.s[5];
-    }
 }
 
 // Contribution of one particle to stencil node (ox, oy, oz): out = (m, p).

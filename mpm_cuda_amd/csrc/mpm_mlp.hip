@@ -530,6 +530,58 @@ int launch_mlp_infer_batch(hipStream_t st, const float *params, const BatchSlice
     return (int)hipGetLastError();
 }
 
+// Warm-start quaternions of the listed particles of a group: list[0, *count)
+// holds group-local indices (the particles whose last G2P took the SVD).
+// Block 0 first clears *next_count, the list the coming G2P appends to.
+// Persistent: every workgroup loads the weights once and takes chunks of 64
+// listed particles per wave; the count is read from device memory, so the
+// grid is sized for the group and workgroups without a chunk leave before
+// loading the weights.
+__global__ void __launch_bounds__(256) k_mlp_infer_list(const float *__restrict__ params, PartView pv,
+                                                        const uint32_t *__restrict__ list,
+                                                        const uint32_t *__restrict__ count, uint32_t *next_count)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *gp(next_count) = 0u;
+    const uint32_t n = min(*gp(count), (uint32_t)pv.n);
+    const uint32_t chunks = (n + 63u) / 64u, per = gridDim.x * 4u;
+    if (blockIdx.x * 4u >= chunks) return;       // whole workgroup
+    __shared__ MlpWeightsLds W;
+    __shared__ MlpInferLds act[4];
+    load_weights(params, W, nullptr);
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    MlpInferLds &A = act[w];
+    // every wave runs the same number of chunks (barriers inside)
+    for (uint32_t c0 = blockIdx.x * 4u; c0 < chunks; c0 += per) {
+        const uint32_t j = (c0 + (uint32_t)w) * 64u + (uint32_t)l;
+        const bool ok = j < n;
+        const int i = ok ? (int)gp(list)[j] : 0;
+        float Fv[9], x[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fv[k] = ok ? ld(pv, PL_F + k, i) : 0.0f;
+        mlp_input(Fv, x);
+#pragma unroll
+        for (int k = 0; k < LD32; ++k) A.X[l][k] = (__bf16)(k < MLP_IN ? x[k] : 0.0f);
+        __syncthreads();
+        mlp_infer_wave(W, A);
+        if (ok) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) st(pv, PL_Q + k, i, A.Q[l][k]);
+        }
+        __syncthreads();
+    }
+}
+
+int launch_mlp_infer_list(hipStream_t st, const float *params, const PartView &pv, const uint32_t *list,
+                          const uint32_t *count, uint32_t *next_count)
+{
+    if (pv.n <= 0) return 0;
+    const int64_t chunks = (pv.n + 63) / 64;
+    const int wg = (int)std::min<int64_t>((chunks + 3) / 4, 256);
+    hipLaunchKernelGGL(k_mlp_infer_list, dim3(wg), dim3(256), 0, st, params, pv, list, count, next_count);
+    return (int)hipGetLastError();
+}
+
 int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv)
 {
     if (pv.n <= 0) return 0;

@@ -107,7 +107,7 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
     const int bz = (int)(b / ((uint32_t)g.nbx * (uint32_t)g.nby));
     const int64_t idx = (int64_t)b * BLK_NODES + lane;
     float4 a = gload4(gmp, idx);
-    gstore4(gmp, idx, make_float4(0.f, 0.f, 0.f, 0.f));
+    if (a.x != 0.f || a.y != 0.f || a.z != 0.f || a.w != 0.f) gstore4(gmp, idx, make_float4(0.f, 0.f, 0.f, 0.f));
     if (blk_tile) {
         for (int gi = 0; gi < ngroups; ++gi) {
             // lanes 0..26 look up the staging slots of the 27 neighbour
@@ -118,13 +118,15 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
                 const int px = bx + lane % 3 - 1, py = by + (lane / 3) % 3 - 1, pz = bz + lane / 9 - 1;
                 if (px >= 0 && py >= 0 && pz >= 0 && px < g.nbx && py < g.nby && pz < g.nbz) {
                     const int32_t t = gp(bt)[((int64_t)pz * g.nby + py) * g.nbx + px];
-                    if (t >= 0 && (uint32_t)t < staging_cap) my_t = t;
+                    if (t >= 0 && ((uint32_t)t & TILE_SLOT_MASK) < staging_cap) my_t = t;
                 }
             }
             // A node lies in exactly 8 of the 27 arenas: per axis its own
             // block's (d = 0) and the one below (d = -1, local k <= 2) or
-            // above (d = +1, k = 3).  All 8 loads are issued before any is
-            // used.
+            // above (d = +1, k = 3).  A neighbour's arena is read at index 7
+            // (k = 2, below) or 0 (k = 3, above) only when its P2G wrote that
+            // face (tile_entry); index 0 and 7 hold zero otherwise, and are
+            // not written.  All loads are issued before any is used.
             const int sx = kx == 3 ? 1 : -1, sy = ky == 3 ? 1 : -1, sz = kz == 3 ? 1 : -1;
             float4 v[8];
 #pragma unroll
@@ -132,9 +134,13 @@ __device__ __forceinline__ void grid_block(const GridP &g, float dt, uint32_t b,
                 const int dx = (e & 1) ? sx : 0, dy = (e & 2) ? sy : 0, dz = (e & 4) ? sz : 0;
                 const int32_t t = __shfl(my_t, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
                 const int ax = kx + 1 - 4 * dx, ay = ky + 1 - 4 * dy, az = kz + 1 - 4 * dz;
-                const int64_t ts = t >= 0 ? t : 0;
-                v[e] = gload4(staging, ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax);
-                if (t < 0) v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+                const uint32_t f = (uint32_t)t >> TILE_FACE_SHIFT;
+                const bool face_ok = (ax != 0 || (f & 1u)) && (ax != 7 || (f & 2u)) && (ay != 0 || (f & 4u)) &&
+                                     (ay != 7 || (f & 8u)) && (az != 0 || (f & 16u)) && (az != 7 || (f & 32u));
+                const bool use = t >= 0 && face_ok;
+                const int64_t ts = use ? (int64_t)((uint32_t)t & TILE_SLOT_MASK) : 0;
+                v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (use) v[e] = gload4(staging, ts * ARENA_NODES + (az * ARENA + ay) * ARENA + ax);
             }
 #pragma unroll
             for (int e = 0; e < 8; ++e) { a.x += v[e].x; a.y += v[e].y; a.z += v[e].z; a.w += v[e].w; }
@@ -231,11 +237,17 @@ __device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, co
     if (sd) dt_measures<MAT>(mat, o.v, o.S, sd[0], sd[1]);
 }
 
+// Jello and snow G2P fit 128 VGPRs without spills when asked for four waves
+// per SIMD (the default allocation took 131-133, three waves; 1M snow G2P
+// 45.1 vs 47.0 us, measured); water's fits by default, and asking spilled.
+template <int MAT>
+constexpr int kG2pWaves = MAT == MPM_WATER ? 1 : 4;
+
 // With the step-size controller on (g.dtp), every wave also merges its
 // particles' maxima into counters[CT_SPEED], counters[CT_DEFORM] for the
 // next step's control.
 template <int MAT>
-__global__ void __launch_bounds__(256) k_g2p(GridP g, MatP mat, PartView pv, float dt,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kG2pWaves<MAT>))) k_g2p(GridP g, MatP mat, PartView pv, float dt,
                                              const float4 *__restrict__ gv, int write_tau,
                                              int sweeps, uint32_t *counters)
 {

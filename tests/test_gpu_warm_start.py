@@ -119,13 +119,15 @@ def _svd_calls(sim, mat, n):
 @pytest.mark.parametrize("mat", [capi.JELLO, capi.SNOW], ids=["jello", "snow"])
 def test_simulation_with_warm_start(gpu, mat):
     """FAST form with the warm start (two sweeps, cold fallback) against the
-    cold start, with a third of the particles deformed so that the warm
-    path runs: snow trial F outside the clamp band (singular values in
+    cold start, with a third of the particles deformed so that the SVD path
+    runs: snow trial F outside the clamp band (singular values in
     [0.93, 1.03] against [0.975, 1.0075]), jello F outside (0.5, 2) (the
     stress takes the SVD instead of the Newton polar rotation).  Fields
-    match the cold run within the FP32 parity bars; the warm-started SVD ran
-    for at least a tenth of the particle-steps and the network's start was
-    rejected (cold SVD redone) for at most the stated share of them."""
+    match the cold run within the FP32 parity bars.  The device counters
+    agree with the SVD-path particles counted on the host; the network is
+    run for the particles whose previous G2P took the SVD, so every
+    SVD-path update after the first step starts from a quaternion, and at
+    most the stated share of those starts is rejected."""
     F = _deformations(20000, 11, spread=0.05)
     p, _ = capi.mlp_train(F, M.init_params(12), steps=300, batch=2048, lr=2e-3, seed=3)
     a = Pair(type_=mat, form=capi.FORM_FAST, seed=21)
@@ -143,25 +145,40 @@ def test_simulation_with_warm_start(gpu, mat):
         for q in (a, b):
             q.gpu.write(0, capi.P_F, F0)
         b.gpu.set_warm_start_weights(p)
-        calls = 0
+        calls, first = 0, 0
         jp = b.gpu.read(0, capi.P_JP)[:, 0].copy()
-        for _ in range(5):
+        for k in range(5):
             a.gpu.step(a.dt)
             b.gpu.step(b.dt)
             if mat == capi.SNOW:
                 j = b.gpu.read(0, capi.P_JP)[:, 0]
-                calls += int(np.count_nonzero(j != jp))
+                c = int(np.count_nonzero(j != jp))
                 jp = j.copy()
             else:
-                calls += _svd_calls(b.gpu, mat, n)
+                c = _svd_calls(b.gpu, mat, n)
+            calls += c
+            if k == 0:
+                first = b.gpu.svd_stats()["svd"]
         ga, gb = a.gpu_particles(), b.gpu_particles()
         for k in ("x", "v"):
             assert rel(gb[k], ga[k]) < 1e-5, k
         assert rel(gb["F"] - np.eye(3), ga["F"] - np.eye(3)) < 1e-4
-        rejects = b.gpu.stats()["warm_rejects"]
-        frac = rejects / max(calls, 1)
-        print(f"warm-started SVDs {calls} of {5 * n} particle-steps, rejected {rejects} ({frac:.3f})")
+        st = b.gpu.svd_stats()
+        cold = a.gpu.svd_stats()
+        frac = st["rejected"] / max(st["warm"], 1)
+        print(f"SVD-path updates {st['svd']} (host count {calls}, cold run {cold['svd']}) of {5 * n} "
+              f"particle-steps, warm-started {st['warm']}, rejected {st['rejected']} ({frac:.3f})")
         assert calls >= 0.1 * 5 * n
+        # host and device agree on the SVD path (a particle at rounding
+        # distance of the band edge may take it without a clamp)
+        assert calls <= st["svd"] <= 1.01 * calls + 10, (calls, st["svd"])
+        # the cold run takes the same path up to particles at rounding
+        # distance of the band edge (the two runs differ by rounding)
+        assert cold["warm"] == 0 and cold["rejected"] == 0
+        assert abs(cold["svd"] - st["svd"]) <= 0.01 * st["svd"] + 10, (cold["svd"], st["svd"])
+        # the first step has no quaternion yet; later SVD-path updates of
+        # particles listed by the step before start from the network's
+        assert st["warm"] >= 0.8 * (st["svd"] - first), (st, first)
         assert frac <= WARM_REJECT_BAR[mat], frac
         assert b.gpu.health() & ~capi.H_ARENA_SPILL == 0
     finally:
