@@ -730,7 +730,7 @@ void mpm_destroy(mpm_sim *s)
     void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
                     s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
                     s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
-                    s->staging, s->blk_tile, s->spill, s->dtc, s->warm_list[0], s->warm_list[1], s->warm_cnt};
+                    s->staging, s->blk_tile, s->spill, s->dtc, s->warm_list, s->warm_cnt};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (s->spill_host) (void)hipHostFree(s->spill_host);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -760,14 +760,15 @@ hipEvent_t take_event(mpm_sim *s)
 int launch_mlp_infer_list(hipStream_t st, const float *params, const PartView &pv, const uint32_t *list,
                           const uint32_t *count, uint32_t *next_count);
 
-// The two particle lists of the warm start (and their per-group counts,
-// cleared), sized for the particle capacity.
-static int ensure_warm_lists(mpm_sim *s)
+// The particle list of the warm start (and its per-group counts, cleared),
+// sized for the particle capacity.
+int ensure_warm_lists(mpm_sim *s)
 {
-    if (s->warm_list[0] && s->warm_list_cap >= s->cap) return MPM_OK;
-    for (auto &l : s->warm_list) { if (l) (void)hipFree(l); l = nullptr; }
+    if (s->warm_list && s->warm_list_cap >= s->cap) return MPM_OK;
+    if (s->warm_list) (void)hipFree(s->warm_list);
+    s->warm_list = nullptr;
     s->warm_list_cap = 0;
-    for (auto &l : s->warm_list) MPM_HIP(hipMalloc(&l, (size_t)s->cap * sizeof(uint32_t)));
+    MPM_HIP(hipMalloc(&s->warm_list, (size_t)s->cap * sizeof(uint32_t)));
     if (!s->warm_cnt) MPM_HIP(hipMalloc(&s->warm_cnt, 2 * MAX_GROUPS * sizeof(uint32_t)));
     s->warm_list_cap = s->cap;
     return check_launch(launch_zero_u32(s->stream, s->warm_cnt, 2 * MAX_GROUPS), "zero");
@@ -792,11 +793,11 @@ static int prepare_warm(mpm_sim *s)
         if (m.warm_sweeps > 0) {
             const int64_t off = s->groups[k].off;
             uint32_t *cnt = s->warm_cnt + 2 * k;
-            rc = check_launch(launch_mlp_infer_list(s->stream, s->mlp_w, view_of(s, (int)k), s->warm_list[p] + off,
+            rc = check_launch(launch_mlp_infer_list(s->stream, s->mlp_w, view_of(s, (int)k), s->warm_list + off,
                                                     cnt + p, cnt + (p ^ 1)),
                               "mlp_infer_list");
             if (rc) return rc;
-            m.wlist = s->warm_list[p ^ 1] + off;
+            m.wlist = s->warm_list + off;
             m.wcount = cnt + (p ^ 1);
         }
     }

@@ -248,6 +248,10 @@ int build(mpm_batch *b)
     const size_t n = b->sims.size();
     MPM_HIP(hipStreamSynchronize(b->stream));       // no replay still reads the tables
     group_warm(b);
+    for (mpm_sim *s : b->sims) {
+        int rc = warm_on(s) ? ensure_warm_lists(s) : MPM_OK;
+        if (rc) return rc;
+    }
     for (int p = 0; p < 2; ++p) {
         std::vector<BatchSlice> h(n);
         for (size_t k = 0; k < n; ++k) {
@@ -258,6 +262,13 @@ int build(mpm_batch *b)
             fill_batch_slice(s, h[k]);
             h[k].mat.warm_sweeps = warm_on(s) ? std::max(1, s->cfg.warm_sweeps) : 0;
             h[k].warm_group = b->warm_group[k];
+            if (warm_on(s)) {
+                // the G2P lists the particles that take the SVD; the next
+                // step's inference runs over them (k_batch_zero hands them over)
+                h[k].mat.wlist = s->warm_list;
+                h[k].mat.wcount = s->warm_cnt + 1;
+                h[k].warm_cnt = s->warm_cnt;
+            }
             s->P = P; s->P_alt = Pa; s->pid = id; s->pid_alt = ida;
         }
         MPM_HIP(hipMemcpy(b->tab[p], h.data(), n * sizeof(BatchSlice), hipMemcpyHostToDevice));

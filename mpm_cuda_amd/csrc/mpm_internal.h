@@ -252,6 +252,8 @@ struct BatchSlice {
     int sweeps;
     int warm_group;                 // warm-start weight set of the scene, -1: cold start
     int reserved;
+    uint32_t *warm_cnt;             // warm start: [0] listed particles the coming inference takes,
+                                    // [1] entries the G2P appends (mat.wlist); null when cold
     float4 *grid_mp, *grid_v, *staging;
     uint32_t *flags, *list, *counters, *spill;
     const uint4 *desc;              // P2G tiles of the last sort
@@ -367,11 +369,12 @@ struct mpm_sim {
     // changes with every mpm_set_warm_start_weights
     float *mlp_w = nullptr;
     int64_t mlp_count = 0;
-    // warm start by list: warm_list[p] holds, per group at the group's
-    // offset, the particles whose last G2P took the SVD; warm_cnt[2 k + p]
-    // counts group k's entries.  The G2P appends to parity warm_par ^ 1
-    // while the inference before it reads parity warm_par.
-    uint32_t *warm_list[2] = {nullptr, nullptr};
+    // warm start by list: warm_list holds, per group at the group's offset,
+    // the particles whose last G2P took the SVD; warm_cnt[2 k + p] counts
+    // group k's entries.  The inference of a step reads the entries counted
+    // in parity warm_par; the G2P after it (stream order) rewrites the list
+    // counting in parity warm_par ^ 1.
+    uint32_t *warm_list = nullptr;
     uint32_t *warm_cnt = nullptr;
     int64_t warm_list_cap = 0;
     int warm_par = 0;

@@ -43,6 +43,8 @@ PartView view_of(mpm_sim *s, int g);
 // mpm_fast.hip); host side in mpm_batch.hip.
 void fill_batch_slice(mpm_sim *s, BatchSlice &b);
 int launch_batch_zero(hipStream_t st, const BatchSlice *tab, int nslices);
+// Warm-start particle lists of a simulator (mpm_api.hip).
+int ensure_warm_lists(mpm_sim *s);
 int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *types, int nslices, int wgs,
                      int tile);
 // FAST P2G tile shapes (particles per pass over a tile): 0 = the default.

@@ -480,34 +480,39 @@ __global__ void k_batch_indices(uint32_t *idx, int m, uint64_t seed, uint64_t st
     idx[k] = (uint32_t)(z % n);
 }
 
-// Warm-start quaternions of every particle of a group (SoA planes).
 // Inference for the warm scenes of a batch that share one weight set
-// (BatchSlice::warm_group == group): chunks of 64 particles, each scene's
-// particles filling whole chunks, numbered scene after scene; every wave maps
-// its chunk to a scene with its own SliceScan.  Writes q to the PL_Q planes.
+// (BatchSlice::warm_group == group): the particles each scene's last G2P
+// listed (mat.wlist[0, warm_cnt[0])), in chunks of 64 numbered scene after
+// scene; every wave maps its chunk to a scene with its own SliceScan.
+// Writes q to the PL_Q planes.  Workgroups without a chunk leave before
+// loading the weights.
 __global__ void __launch_bounds__(256) k_mlp_infer_batch(const float *__restrict__ params,
                                                          const BatchSlice *__restrict__ tab, int nslices, int group)
 {
+    SliceScan sc;
+    sc.build(nslices, [&](int s) {
+        return tab[s].warm_group == group && tab[s].warm_cnt ? (gp(tab[s].warm_cnt)[0] + 63u) / 64u : 0u;
+    });
+    const uint32_t chunks = sc.total, per = gridDim.x * 4u;
+    if (blockIdx.x * 4u >= chunks) return;       // whole workgroup
     __shared__ MlpWeightsLds W;
     __shared__ MlpInferLds act[4];
     load_weights(params, W, nullptr);
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     MlpInferLds &A = act[w];
-    SliceScan sc;
-    sc.build(nslices, [&](int s) { return tab[s].warm_group == group ? ((uint32_t)tab[s].pv.n + 63u) / 64u : 0u; });
-    const uint32_t chunks = sc.total, per = gridDim.x * 4u;
     // every wave runs the same number of chunks (barriers inside)
     for (uint32_t c0 = blockIdx.x * 4u; c0 < chunks; c0 += per) {
         const uint32_t c = c0 + (uint32_t)w;
         const uint32_t cc = c < chunks ? c : 0u;
         const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(cc));
-        const PartView pv = tab[sid].pv;
-        const int i = (int)((cc - sc.start(sid)) * 64u) + l;
-        const bool ok = c < chunks && i < pv.n;
+        const BatchSlice &b = tab[sid];
+        const uint32_t j = (cc - sc.start(sid)) * 64u + (uint32_t)l;
+        const bool ok = c < chunks && j < min(gp(b.warm_cnt)[0], (uint32_t)b.pv.n);
+        const int i = ok ? (int)gp(b.mat.wlist)[j] : 0;
         float Fv[9], x[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Fv[k] = ok ? ld(pv, PL_F + k, i) : 0.0f;
+        for (int k = 0; k < 9; ++k) Fv[k] = ok ? ld(b.pv, PL_F + k, i) : 0.0f;
         mlp_input(Fv, x);
 #pragma unroll
         for (int k = 0; k < LD32; ++k) A.X[l][k] = (__bf16)(k < MLP_IN ? x[k] : 0.0f);
@@ -515,7 +520,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_batch(const float *__restrict
         mlp_infer_wave(W, A);
         if (ok) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) st(pv, PL_Q + k, i, A.Q[l][k]);
+            for (int k = 0; k < 4; ++k) st(b.pv, PL_Q + k, i, A.Q[l][k]);
         }
         __syncthreads();
     }
@@ -525,7 +530,7 @@ int launch_mlp_infer_batch(hipStream_t st, const float *params, const BatchSlice
                            int64_t chunks)
 {
     if (chunks <= 0) return 0;
-    const int wg = (int)std::min<int64_t>((chunks + 3) / 4, 2048);
+    const int wg = (int)std::min<int64_t>((chunks + 3) / 4, 256);
     hipLaunchKernelGGL(k_mlp_infer_batch, dim3(wg), dim3(256), 0, st, params, tab, nslices, group);
     return (int)hipGetLastError();
 }
@@ -579,18 +584,6 @@ int launch_mlp_infer_list(hipStream_t st, const float *params, const PartView &p
     const int64_t chunks = (pv.n + 63) / 64;
     const int wg = (int)std::min<int64_t>((chunks + 3) / 4, 256);
     hipLaunchKernelGGL(k_mlp_infer_list, dim3(wg), dim3(256), 0, st, params, pv, list, count, next_count);
-    return (int)hipGetLastError();
-}
-
-int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv)
-{
-    if (pv.n <= 0) return 0;
-    const int64_t chunks = (pv.n + 63) / 64;
-    const int wg = (int)std::min<int64_t>((chunks + 3) / 4, 2048);
-    const float *Fp = pv.P + (int64_t)PL_F * pv.cap + pv.off;
-    float *qp = pv.P + (int64_t)PL_Q * pv.cap + pv.off;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(wg), dim3(256), 0, st, params, Fp, (int64_t)1, (int64_t)pv.cap, qp,
-                       (int64_t)1, (int64_t)pv.cap, (int64_t)pv.n);
     return (int)hipGetLastError();
 }
 

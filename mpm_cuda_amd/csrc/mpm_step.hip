@@ -388,6 +388,13 @@ __global__ void k_batch_zero(const BatchSlice *__restrict__ tab, int nslices)
     gp(c)[CT_ACTIVE] = 0u;
     gp(c)[CT_NODES] = 0u;
     gp(c)[CT_SPILL] = 0u;
+    // warm start: the particles the last G2P listed go to this step's
+    // inference, and the list starts over for this step's G2P
+    uint32_t *w = tab[s].warm_cnt;
+    if (w) {
+        gp(w)[0] = gp(w)[1];
+        gp(w)[1] = 0u;
+    }
 }
 
 template <int MAT>
