@@ -730,7 +730,7 @@ void mpm_destroy(mpm_sim *s)
     void *ptrs[] = {s->grid_mp, s->grid_v, s->flags, s->list, s->counters, s->P, s->P_alt, s->pid,
                     s->pid_alt, s->keys, s->keys_alt, s->vals, s->vals_alt, s->sort_hist, s->sort_scan,
                     s->blk_count, s->blk_start, s->tile_desc, s->scan_tmp, s->tile_counts, s->mlp_w,
-                    s->staging, s->blk_tile, s->spill, s->dtc, s->warm_list, s->warm_cnt};
+                    s->staging, s->blk_tile, s->spill, s->dtc, s->warm_list, s->warm_cnt, s->g2p_perm};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (s->spill_host) (void)hipHostFree(s->spill_host);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -860,21 +860,31 @@ static int run_phase(mpm_sim *s, int phase, float dt)
         if (s->timing) rc = check_launch(launch_count_nodes(s->stream, s->list, s->counters, s->grid_v), "count");
         return rc;
     }
-    // G2P
+    // G2P; a due re-sort of the FAST form is applied by this G2P
+    if (fast) {
+        PhaseTimer t(s, 0);
+        rc = sort_for_g2p(s);
+        if (rc) return rc;
+    }
     rc = prepare_warm(s);
-    if (rc) return rc;
-    PhaseTimer t(s, 3), tk(s, TK_G2P);
-    for (size_t k = 0; k < s->groups.size() && !rc; ++k) {
-        if (fast) rc = launch_g2p_fast(s, (int)k, dt);
-        else
-            rc = check_launch(launch_g2p(s->stream, s->g, s->groups[k].mat, view_of(s, (int)k), dt, s->grid_v, 0,
-                                         s->cfg.svd_sweeps, s->counters, thr),
-                              "g2p");
+    if (rc) {
+        (void)finish_g2p_move(s, false);
+        return rc;
     }
-    if (!rc) {
-        s->tau_valid = fast;
-        if (fast) s->steps_since_sort += 1;
+    {
+        PhaseTimer t(s, 3), tk(s, TK_G2P);
+        for (size_t k = 0; k < s->groups.size() && !rc; ++k) {
+            if (fast) rc = launch_g2p_fast(s, (int)k, dt);
+            else
+                rc = check_launch(launch_g2p(s->stream, s->g, s->groups[k].mat, view_of(s, (int)k), dt, s->grid_v,
+                                             0, s->cfg.svd_sweeps, s->counters, thr),
+                                  "g2p");
+        }
     }
+    if (rc) return rc;          // a failed launch leaves the device state undefined either way
+    if ((rc = finish_g2p_move(s, true))) return rc;
+    s->tau_valid = fast;
+    if (fast) s->steps_since_sort += 1;
     return rc;
 }
 

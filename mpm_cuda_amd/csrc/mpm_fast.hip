@@ -1172,7 +1172,7 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
                 Scatter sc;
                 G2POut o;
                 if (active) {
-                    bits |= g2p_core<MAT>(g, mat, pv, i, dt, fetch, MAT != MPM_WATER, sweeps, counters, o);
+                    bits |= g2p_core<MAT>(g, mat, pv, i, pv, i, dt, fetch, MAT != MPM_WATER, sweeps, counters, o);
                     bits |= make_stencil(g, o.x, s);
                     scatter_from_out<MAT>(mat, o, dt_next, sc);
                 }
@@ -1246,7 +1246,11 @@ static int ensure_fast_buffers(mpm_sim *s)
     return MPM_OK;
 }
 
-int sort_all(mpm_sim *s)
+// Sort every group's particles by node block and build the tiles.  With
+// `defer` the particles are not moved: the order is left in g2p_perm for the
+// G2P that follows (FAST form), which writes its updated particles in that
+// order -- the re-sort then costs no pass of its own over the 31 planes.
+int sort_all(mpm_sim *s, bool defer)
 {
     int rc = ensure_fast_buffers(s);
     if (rc) return rc;
@@ -1255,11 +1259,19 @@ int sort_all(mpm_sim *s)
     while ((1ll << (bits - KEY_LOW)) < s->nblocks) ++bits;
     uint64_t plane_mask = (1ull << PL_COUNT) - 1ull;
     if (!s->tau_valid) plane_mask &= ~(((1ull << 6) - 1ull) << PL_TAU);
+    if (defer && s->g2p_perm_cap < s->cap) {
+        if (s->g2p_perm) (void)hipFree(s->g2p_perm);
+        s->g2p_perm = nullptr;
+        s->g2p_perm_cap = 0;
+        MPM_HIP(hipMalloc(&s->g2p_perm, (size_t)s->cap * sizeof(uint32_t)));
+        s->g2p_perm_cap = s->cap;
+    }
     // warm-start quaternions travel with their particles while the warm start
     // lists particles (the lists' indices are dropped: they name slots of the
-    // old order)
+    // old order); a deferred order keeps the lists until the G2P moves the
+    // particles (its inference reads the old slots, its list the new ones)
     if (!s->warm_cnt) plane_mask &= ~(((1ull << 4) - 1ull) << PL_Q);
-    else if ((rc = launch_zero_u32(st, s->warm_cnt, 2 * MAX_GROUPS))) return hip_fail((hipError_t)rc, "zero");
+    else if (!defer && (rc = launch_zero_u32(st, s->warm_cnt, 2 * MAX_GROUPS))) return hip_fail((hipError_t)rc, "zero");
     const int nb = (int)s->nblocks;
     for (size_t k = 0; k < s->groups.size(); ++k) {
         PartView pv = view_of(s, (int)k);
@@ -1274,8 +1286,12 @@ int sort_all(mpm_sim *s)
             return hip_fail((hipError_t)rc, "radix_sort");
         const uint32_t *perm = in_alt ? s->vals_alt : s->vals;
         const uint32_t *skeys = in_alt ? s->keys_alt : s->keys;
-        hipLaunchKernelGGL(k_permute, dim3(round_xcd((uint32_t)nblk(n, 256))), dim3(256), 0, st, s->P, s->P_alt, s->cap, pv.off, n, perm,
-                           s->pid, s->pid_alt, plane_mask);
+        if (defer)
+            MPM_HIP(hipMemcpyAsync(s->g2p_perm + pv.off, perm, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                   st));
+        else
+            hipLaunchKernelGGL(k_permute, dim3(round_xcd((uint32_t)nblk(n, 256))), dim3(256), 0, st, s->P, s->P_alt,
+                               s->cap, pv.off, n, perm, s->pid, s->pid_alt, plane_mask);
         // tiles: one per non-empty block, numbered in block order
         hipLaunchKernelGGL(k_block_runs, dim3(nblk(n, 256)), dim3(256), 0, st, skeys, n, s->blk_start, blk_end);
         hipLaunchKernelGGL(k_tile_flags, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_start, blk_end, s->scan_tmp,
@@ -1288,8 +1304,12 @@ int sort_all(mpm_sim *s)
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "sort kernels");
     }
-    std::swap(s->P, s->P_alt);
-    std::swap(s->pid, s->pid_alt);
+    if (defer) {
+        s->g2p_move_pending = true;
+    } else {
+        std::swap(s->P, s->P_alt);
+        std::swap(s->pid, s->pid_alt);
+    }
     s->sorted_valid = true;
     s->steps_since_sort = 0;
     s->sorted_at_launch = s->tile_launches;
@@ -1308,12 +1328,14 @@ int sort_all(mpm_sim *s)
 // steady flow whole layers of particles leave their arenas together); an
 // interval that ends without spills lets the next one grow by a step, up to
 // sort_every.
-static bool sort_due(mpm_sim *s)
+// `ahead`: steps that will have completed when the new order is first used
+// (1 for a re-sort the coming G2P applies).
+static bool sort_due(mpm_sim *s, int ahead = 0)
 {
     if (!s->sorted_valid) return true;
     const int64_t cap = s->cfg.sort_every;
     if (s->sort_interval <= 0 || s->sort_interval > cap) s->sort_interval = cap;
-    if (s->steps_since_sort >= s->sort_interval) {
+    if (s->steps_since_sort + ahead >= s->sort_interval) {
         s->sort_interval = std::min<int64_t>(cap, s->sort_interval + 1);
         return true;
     }
@@ -1368,7 +1390,7 @@ static int note_spills(mpm_sim *s, int group)
 
 int sort_if_due(mpm_sim *s)
 {
-    if (sort_due(s)) return sort_all(s);
+    if (sort_due(s)) return sort_all(s, false);
     return MPM_OK;
 }
 
@@ -1376,7 +1398,7 @@ int fast_prepare(mpm_sim *s)
 {
     int rc = MPM_OK;
     if (sort_due(s)) {
-        rc = sort_all(s);
+        rc = sort_all(s, false);
         if (rc) return rc;
     }
     if (!s->tau_valid) {
@@ -1393,7 +1415,7 @@ int fast_prepare(mpm_sim *s)
 int fast_ready(mpm_sim *s)
 {
     int rc = MPM_OK;
-    if (!s->sorted_valid && (rc = sort_all(s))) return rc;
+    if (!s->sorted_valid && (rc = sort_all(s, false))) return rc;
     if (!s->tau_valid) {
         for (size_t k = 0; k < s->groups.size(); ++k) {
             rc = launch_compute_tau(s->stream, s->groups[k].mat, view_of(s, (int)k), s->cfg.svd_sweeps, s->counters);
@@ -1706,11 +1728,57 @@ int launch_p2g_batch(hipStream_t st, int mat, const BatchSlice *tab, const int *
     return 0;
 }
 
+// FAST-form G2P of one group; after a deferred re-sort it writes the group
+// in the new order into the other buffer (finish_g2p_move swaps them).
 int launch_g2p_fast(mpm_sim *s, int group, float dt)
 {
-    int rc = launch_g2p(s->stream, s->g, s->groups[group].mat, view_of(s, group), dt, s->grid_v, 1,
-                        s->cfg.svd_sweeps, s->counters, s->cfg.threads);
+    const PartView pv = view_of(s, group);
+    G2PMove mv{s->g2p_perm + pv.off, pv, s->pid, s->pid_alt, s->warm_cnt ? 1 : 0};
+    mv.po.P = s->P_alt;
+    mv.po.pid = s->pid_alt;
+    int rc = launch_g2p(s->stream, s->g, s->groups[group].mat, pv, dt, s->grid_v, 1, s->cfg.svd_sweeps, s->counters,
+                        s->cfg.threads, s->g2p_move_pending ? &mv : nullptr);
     return rc ? hip_fail((hipError_t)rc, "g2p_fast") : MPM_OK;
+}
+
+// A due FAST-form re-sort, deferred to the G2P about to run (particles that
+// a P2G of this step listed for the atomic fallback are scattered first:
+// the list names slots of the current order).
+int sort_for_g2p(mpm_sim *s)
+{
+    if (!s->sorted_valid || s->g2p_move_pending || !sort_due(s, 1)) return MPM_OK;
+    int rc = flush_spill(s);
+    return rc ? rc : sort_all(s, true);
+}
+
+// After every group's G2P of a deferred re-sort: the written buffer becomes
+// the particles.  With `moved` false (the G2P did not run) the deferred
+// order is applied by the permutation kernel instead, so that the tiles and
+// the particles agree again.
+int finish_g2p_move(mpm_sim *s, bool moved)
+{
+    if (!s->g2p_move_pending) return MPM_OK;
+    s->g2p_move_pending = false;
+    if (!moved) {
+        uint64_t plane_mask = (1ull << PL_COUNT) - 1ull;
+        if (!s->tau_valid) plane_mask &= ~(((1ull << 6) - 1ull) << PL_TAU);
+        if (!s->warm_cnt) plane_mask &= ~(((1ull << 4) - 1ull) << PL_Q);
+        else {
+            int rc = launch_zero_u32(s->stream, s->warm_cnt, 2 * MAX_GROUPS);
+            if (rc) return hip_fail((hipError_t)rc, "zero");
+        }
+        for (size_t k = 0; k < s->groups.size(); ++k) {
+            const PartView pv = view_of(s, (int)k);
+            if (pv.n <= 0) continue;
+            hipLaunchKernelGGL(k_permute, dim3(round_xcd((uint32_t)nblk(pv.n, 256))), dim3(256), 0, s->stream, s->P,
+                               s->P_alt, s->cap, pv.off, pv.n, s->g2p_perm + pv.off, s->pid, s->pid_alt, plane_mask);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "permute");
+    }
+    std::swap(s->P, s->P_alt);
+    std::swap(s->pid, s->pid_alt);
+    return MPM_OK;
 }
 
 }  // namespace mpm

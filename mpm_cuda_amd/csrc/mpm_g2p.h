@@ -168,10 +168,13 @@ __device__ __forceinline__ void dt_measures(const MatP &m, const float v[3], flo
     deform = MAT == MPM_WATER ? 0.0f : (J > 0.0f ? fmaxf(J, 1.0f / J) : 1e30f);
 }
 
+// Particle i of pv is updated into slot io of po: the same slot, or (the G2P
+// that applies a re-sort) its slot in the new order in the other buffer,
+// where every plane the state needs is written.
 template <int MAT, class Fetch>
-__device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, const PartView &pv, int i, float dt,
-                                             const Fetch &fetch, bool want_tau, int sweeps, uint32_t *counters,
-                                             G2POut &out)
+__device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, const PartView &pv, int i,
+                                             const PartView &po, int io, float dt, const Fetch &fetch, bool want_tau,
+                                             int sweeps, uint32_t *counters, G2POut &out)
 {
     float x[3] = {ld(pv, PL_X, i), ld(pv, PL_X + 1, i), ld(pv, PL_X + 2, i)};
     // every particle load is issued up front (one memory latency, not one
@@ -201,12 +204,12 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
 #pragma unroll
         for (int c = 0; c < 3; ++c) C[3 * r + c] = c4 * (M[3 * r + c] - s.fx[c] * v[r]);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) { st(pv, PL_V + r, i, v[r]); out.v[r] = v[r]; }
+    for (int r = 0; r < 3; ++r) { st(po, PL_V + r, io, v[r]); out.v[r] = v[r]; }
     const bool c_is_gradv = (MAT == MPM_WATER && mat.water_c_mode == MPM_WATER_C_GRADV);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         out.C[k] = c_is_gradv ? L[k] : C[k];
-        st(pv, PL_C + k, i, out.C[k]);
+        st(po, PL_C + k, io, out.C[k]);
     }
     float A[9];                                                          // I + dt grad v
 #pragma unroll
@@ -215,10 +218,10 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         float Fn[9];
         matmul3<float>(A, F, Fn);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Fn[k]);
+        for (int k = 0; k < 9; ++k) st(po, PL_F + k, io, Fn[k]);
         const float Jn = det3<float>(Fn);
         if (!(Jn > 0.f)) bits |= MPM_H_DET_NONPOS;
-        if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters, &pv, i);
+        if (want_tau) stress_from_F<MPM_JELLO>(mat, Fn, 1.0f, sweeps, out.tau, counters, &pv, i, io);
         out.S = Jn;
     } else if (MAT == MPM_SNOW) {                                        // :620-709
         float Ft[9];
@@ -228,7 +231,8 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
             // (its plane is left as it is); the stress needs only the polar
             // rotation, a few Newton steps instead of the SVD
 #pragma unroll
-            for (int k = 0; k < 9; ++k) st(pv, PL_F + k, i, Ft[k]);
+            for (int k = 0; k < 9; ++k) st(po, PL_F + k, io, Ft[k]);
+            if (po.P != pv.P) st(po, PL_S, io, Sp);        // moved: the plane goes along
             if (!(Sp > 0.f)) bits |= MPM_H_DET_NONPOS;
             if (want_tau) {
                 float R[9];
@@ -239,7 +243,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
             out.S = Sp;
         } else {
             float U[9], sg[3], V[9];
-            svd_g2p(mat, pv, i, Ft, U, sg, V, sweeps, counters);
+            svd_g2p(mat, pv, i, io, Ft, U, sg, V, sweeps, counters);
             float ratio = 1.0f;
             float sc[3];
 #pragma unroll
@@ -255,12 +259,12 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    st(pv, PL_F + 3 * r + c, i,
+                    st(po, PL_F + 3 * r + c, io,
                        Ft[3 * r + c] + (U[3 * r] * ds[0] * V[3 * c] + U[3 * r + 1] * ds[1] * V[3 * c + 1] +
                                         U[3 * r + 2] * ds[2] * V[3 * c + 2]));
             // det Fp_new = det(Fe_new^-1 Ftr Fp) = Jp * prod(s / s*)  (:671-678)
             float Jp = Sp * ratio;
-            st(pv, PL_S, i, Jp);
+            st(po, PL_S, io, Jp);
             if (!(Jp > 0.f)) bits |= MPM_H_DET_NONPOS;
             if (want_tau) {
                 float hc = expf(mat.xi * (1.0f - Jp));
@@ -270,7 +274,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
         }
     } else {                                                             // :710-713
         float J = (1.0f + dt * (L[0] + L[4] + L[8])) * Sp;
-        st(pv, PL_S, i, J);
+        st(po, PL_S, io, J);
         out.S = J;
     }
     const float lo[3] = {g.ox, g.oy, g.oz};
@@ -278,7 +282,7 @@ __device__ __forceinline__ uint32_t g2p_core(const GridP &g, const MatP &mat, co
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float xn = x[a] + dt * v[a];                                     // :717-719
-        st(pv, PL_X + a, i, xn);
+        st(po, PL_X + a, io, xn);
         out.x[a] = xn;
         float hi = lo[a] + g.h * (float)(nn[a] - 1);
         if (!(xn >= lo[a] && xn <= hi)) bits |= MPM_H_LEFT_DOMAIN;       // :722-739

@@ -345,6 +345,13 @@ struct mpm_sim {
     float primed_dt = 0.f;                  // FUSED: dt the primed scatter used for the force term
     bool sorted_valid = false;
     int64_t steps_since_sort = 0;
+    // FAST form re-sort applied by the G2P after it: the sort leaves the
+    // particle order of every group here (group-local sources at the group's
+    // offset) and the G2P writes the updated particles in that order into the
+    // other buffer
+    uint32_t *g2p_perm = nullptr;
+    int64_t g2p_perm_cap = 0;
+    bool g2p_move_pending = false;
     // spill counts of the tile P2G launches, written to pinned host memory
     // by the spill kernel with the launch number after them; a re-sort is
     // brought forward when they grow.  Two slots: the count of the launch two

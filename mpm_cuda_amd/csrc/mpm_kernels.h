@@ -162,16 +162,17 @@ __device__ __forceinline__ void svd_warm(const PartView &pv, int i, const float 
     svd_checked(F, U, s, V, sweeps, counters);
 }
 
-// The Jacobi SVD of a G2P update (particle i of pv): counted in CT_SVD,
-// warm-started when the material's warm_sweeps > 0, and the particle listed
-// for the next step's inference when the material carries a list.
-__device__ __forceinline__ void svd_g2p(const MatP &m, const PartView &pv, int i, const float F[9], float U[9],
-                                        float s[3], float V[9], int sweeps, uint32_t *counters)
+// The Jacobi SVD of a G2P update (particle i of pv, written to slot io):
+// counted in CT_SVD, warm-started when the material's warm_sweeps > 0, and
+// the particle listed (by its new slot) for the next step's inference when
+// the material carries a list.
+__device__ __forceinline__ void svd_g2p(const MatP &m, const PartView &pv, int i, int io, const float F[9],
+                                        float U[9], float s[3], float V[9], int sweeps, uint32_t *counters)
 {
     wave_count(counters + CT_SVD, true);
     if (m.warm_sweeps > 0) svd_warm(pv, i, F, U, s, V, m.warm_sweeps, sweeps, counters);
     else svd_checked(F, U, s, V, sweeps, counters);
-    if (m.wlist) wave_append(m.wlist, m.wcount, (uint32_t)i);
+    if (m.wlist) wave_append(m.wlist, m.wcount, (uint32_t)io);
 }
 
 // Kirchhoff stress of an elastic / snow particle from its deformation
@@ -179,7 +180,7 @@ __device__ __forceinline__ void svd_g2p(const MatP &m, const PartView &pv, int i
 template <int MAT>
 __device__ __forceinline__ void stress_from_F(const MatP &m, const float F[9], float Jp, int sweeps,
                                               float tau6[6], uint32_t *counters, const PartView *warm_pv = nullptr,
-                                              int warm_i = 0)
+                                              int warm_i = 0, int warm_io = 0)
 {
     float mu = m.mu, lam = m.lam;
     if (MAT == MPM_SNOW) {
@@ -196,7 +197,7 @@ __device__ __forceinline__ void stress_from_F(const MatP &m, const float F[9], f
         return;
     }
     float U[9], s[3], V[9];
-    if (warm_pv) svd_g2p(m, *warm_pv, warm_i, F, U, s, V, sweeps, counters);
+    if (warm_pv) svd_g2p(m, *warm_pv, warm_i, warm_io, F, U, s, V, sweeps, counters);
     else svd_checked(F, U, s, V, sweeps, counters);
     fcr_tau_from_svd<float>(U, s, mu, lam, tau6);
 }

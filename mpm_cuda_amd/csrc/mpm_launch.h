@@ -14,8 +14,18 @@ int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters,
 int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t *list,
                        uint32_t *counters, float4 *gmp, float4 *gv, int nblocks, const float4 *staging,
                        const int32_t *blk_tile, int ngroups, uint32_t staging_cap, int clear_spill = 0);
+// A G2P that applies a re-sort: slot j of po receives particle perm[j] of
+// the view, its id (pid -> pid_out) and, with copy_q, its quaternion.
+struct G2PMove {
+    const uint32_t *perm;
+    PartView po;
+    const uint32_t *pid;
+    uint32_t *pid_out;
+    int copy_q;
+};
 int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
-               const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads);
+               const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads,
+               const G2PMove *mv = nullptr);
 int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sweeps, uint32_t *counters);
 int launch_p2g_spill(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
                      const uint32_t *list, uint32_t list_cap, uint32_t *counters, float4 *grid, uint32_t *flags,
@@ -35,7 +45,9 @@ int spill_compact(mpm_sim *s);                // grid phase after a one-group ti
 int flush_spill(mpm_sim *s);                  // spill scatter of a tile P2G whose grid phase did not run
 int launch_g2p_fast(mpm_sim *s, int group, float dt);
 int launch_g2p2g(mpm_sim *s, int group, float dt, float dt_next);   // FUSED form
-int sort_all(mpm_sim *s);                     // sort every group now and rebuild the tiles
+int sort_all(mpm_sim *s, bool defer = false);  // sort every group now and rebuild the tiles
+int sort_for_g2p(mpm_sim *s);                 // a due re-sort, applied by the coming FAST G2P
+int finish_g2p_move(mpm_sim *s, bool moved);  // after that G2P (or instead of it): the particles in the new order
 int fast_ready(mpm_sim *s);                   // sorted once, stored stress current
 PartView view_of(mpm_sim *s, int g);
 

@@ -221,17 +221,18 @@ int launch_count_nodes(hipStream_t st, const uint32_t *list, uint32_t *counters,
 // state is stored for the next P2G, so a FAST-form step needs one SVD.
 // With sd, the step-size controller's measures of the updated particle are
 // returned in sd[0] (speed) and sd[1] (deformation).
+// Particle i of pv goes to slot io of po (see g2p_core).
 template <int MAT>
-__device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, const PartView &pv, int i, float dt,
-                                             const float4 *__restrict__ gv, bool write_tau, int sweeps,
-                                             uint32_t *counters, float *sd = nullptr)
+__device__ __forceinline__ void g2p_particle(const GridP &g, const MatP &mat, const PartView &pv, int i,
+                                             const PartView &po, int io, float dt, const float4 *__restrict__ gv,
+                                             bool write_tau, int sweeps, uint32_t *counters, float *sd = nullptr)
 {
     G2POut o;
     GlobalFetch f{gv};
-    uint32_t bits = g2p_core<MAT>(g, mat, pv, i, dt, f, write_tau && MAT != MPM_WATER, sweeps, counters, o);
+    uint32_t bits = g2p_core<MAT>(g, mat, pv, i, po, io, dt, f, write_tau && MAT != MPM_WATER, sweeps, counters, o);
     if (write_tau && MAT != MPM_WATER) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) st(pv, PL_TAU + k, i, o.tau[k]);
+        for (int k = 0; k < 6; ++k) st(po, PL_TAU + k, io, o.tau[k]);
     }
     if (bits) gatomic_or(counters + CT_HEALTH, bits);
     if (sd) dt_measures<MAT>(mat, o.v, o.S, sd[0], sd[1]);
@@ -245,23 +246,40 @@ constexpr int kG2pWaves = MAT == MPM_WATER ? 1 : 4;
 
 // With the step-size controller on (g.dtp), every wave also merges its
 // particles' maxima into counters[CT_SPEED], counters[CT_DEFORM] for the
-// next step's control.
+// next step's control.  With a permutation (the re-sort of this step, FAST
+// form) thread j updates particle perm[j] of pv into slot j of po (the other
+// particle buffer), with its id and, when copy_q, its warm-start quaternion:
+// the G2P itself moves the particles into their new order.
 template <int MAT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kG2pWaves<MAT>))) k_g2p(GridP g, MatP mat, PartView pv, float dt,
                                              const float4 *__restrict__ gv, int write_tau,
-                                             int sweeps, uint32_t *counters)
+                                             int sweeps, uint32_t *counters, const uint32_t *__restrict__ perm,
+                                             PartView po, const uint32_t *__restrict__ pid, uint32_t *pid_out,
+                                             int copy_q)
 {
     const uint32_t nb = ((uint32_t)pv.n + blockDim.x - 1) / blockDim.x;
     const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(nb));      // launched with round_xcd(nb) workgroups
     if (blk >= nb) return;                                             // whole workgroup
-    const int i = (int)(blk * blockDim.x + threadIdx.x);
+    const int j = (int)(blk * blockDim.x + threadIdx.x);
+    int i = j;
+    if (perm && j < pv.n) {
+        const uint32_t src = gp(perm)[j];
+        i = src < (uint32_t)pv.n ? (int)src : j;     // always a valid index for a permutation
+        gp(pid_out)[pv.off + j] = gp(pid)[pv.off + i];
+        if (copy_q) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) st(po, PL_Q + k, j, ld(pv, PL_Q + k, i));
+        }
+    }
+    PartView out = pv;                           // a value: a reference chosen at run time would go to scratch
+    if (perm) out = po;
     if (!g.dtp) {
-        if (i < pv.n) g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters);
+        if (j < pv.n) g2p_particle<MAT>(g, mat, pv, i, out, j, dt, gv, write_tau != 0, sweeps, counters);
         return;
     }
     dt = step_dt(g, dt);
     float sd[2] = {0.0f, 0.0f};
-    if (i < pv.n) g2p_particle<MAT>(g, mat, pv, i, dt, gv, write_tau != 0, sweeps, counters, sd);
+    if (j < pv.n) g2p_particle<MAT>(g, mat, pv, i, out, j, dt, gv, write_tau != 0, sweeps, counters, sd);
     wave_max_to(counters + CT_SPEED, sd[0]);
     wave_max_to(counters + CT_DEFORM, sd[1]);
 }
@@ -472,7 +490,7 @@ __global__ void __launch_bounds__(256) k_g2p_batch(const BatchSlice *__restrict_
     const BatchSlice &b = tab[sid];
     const int i = (int)((blk - sc.start(sid)) * blockDim.x + threadIdx.x);
     if (i >= b.pv.n) return;
-    g2p_particle<MAT>(b.g, b.mat, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters);
+    g2p_particle<MAT>(b.g, b.mat, b.pv, i, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,14 +537,16 @@ int launch_grid_update(hipStream_t st, const GridP &g, float dt, const uint32_t 
 }
 
 int launch_g2p(hipStream_t st, const GridP &g, const MatP &m, const PartView &pv, float dt,
-               const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads)
+               const float4 *gv, int write_tau, int sweeps, uint32_t *counters, int threads, const G2PMove *mv)
 {
     if (pv.n <= 0) return 0;
     dim3 gr(round_xcd((uint32_t)nblk(pv.n, threads))), bl(threads);
+    const G2PMove none{nullptr, pv, nullptr, nullptr, 0};
+    const G2PMove &v = mv ? *mv : none;
     switch (m.type) {
-    case MPM_JELLO: hipLaunchKernelGGL(k_g2p<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
-    case MPM_SNOW: hipLaunchKernelGGL(k_g2p<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
-    default: hipLaunchKernelGGL(k_g2p<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters); break;
+    case MPM_JELLO: hipLaunchKernelGGL(k_g2p<MPM_JELLO>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters, v.perm, v.po, v.pid, v.pid_out, v.copy_q); break;
+    case MPM_SNOW: hipLaunchKernelGGL(k_g2p<MPM_SNOW>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters, v.perm, v.po, v.pid, v.pid_out, v.copy_q); break;
+    default: hipLaunchKernelGGL(k_g2p<MPM_WATER>, gr, bl, 0, st, g, m, pv, dt, gv, write_tau, sweeps, counters, v.perm, v.po, v.pid, v.pid_out, v.copy_q); break;
     }
     return (int)hipGetLastError();
 }
