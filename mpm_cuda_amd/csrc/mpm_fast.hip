@@ -1324,19 +1324,24 @@ int sort_all(mpm_sim *s, bool defer)
 // the tile launch two back; waiting for its copy never idles the device
 // (a later step is still queued) and bounds how far the host runs ahead.
 // The interval adapts: a re-sort forced by spills after k steps sets the
-// next interval to k - 3 (the count arrives two launches late, and in a
-// steady flow whole layers of particles leave their arenas together); an
-// interval that ends without spills lets the next one grow by a step, up to
-// sort_every.
+// interval, and the longest one known to stay clear of spills, to k - 3 (the
+// count arrives late, and in a steady flow whole layers of particles leave
+// their arenas together, so the burst would come back at the same age); the
+// interval then stays at that bound, which every fourth interval-called
+// re-sort raises by a step (up to sort_every), so a flow that slows down is
+// followed and a burst is met about once per dozen re-sorts instead of once
+// per three or four.
 // `ahead`: steps that will have completed when the new order is first used
 // (1 for a re-sort the coming G2P applies).
 static bool sort_due(mpm_sim *s, int ahead = 0)
 {
     if (!s->sorted_valid) return true;
     const int64_t cap = s->cfg.sort_every;
-    if (s->sort_interval <= 0 || s->sort_interval > cap) s->sort_interval = cap;
+    if (s->sort_safe <= 0 || s->sort_safe > cap) s->sort_safe = cap;
+    if (s->sort_interval <= 0 || s->sort_interval > s->sort_safe) s->sort_interval = s->sort_safe;
     if (s->steps_since_sort + ahead >= s->sort_interval) {
-        s->sort_interval = std::min<int64_t>(cap, s->sort_interval + 1);
+        if (++s->sort_clean % 4 == 0) s->sort_safe = std::min<int64_t>(cap, s->sort_safe + 1);
+        s->sort_interval = std::min<int64_t>(s->sort_safe, s->sort_interval + 1);
         return true;
     }
     const int64_t want = s->tile_launches - mpm_sim::SPILL_SLOTS;
@@ -1357,7 +1362,8 @@ static bool sort_due(mpm_sim *s, int ahead = 0)
     uint64_t tot = 0;
     for (size_t k = 0; k < s->groups.size(); ++k) tot += s->spill_host[slot * MAX_GROUPS + k];
     if (tot <= std::max<uint64_t>(64, (uint64_t)s->total / 10000)) return false;
-    s->sort_interval = std::max<int64_t>(1, s->steps_since_sort - 3);
+    s->sort_interval = s->sort_safe = std::max<int64_t>(1, s->steps_since_sort - 3);
+    s->sort_clean = 0;
     return true;
 }
 

@@ -370,7 +370,9 @@ struct mpm_sim {
     uint32_t *spill_hw = nullptr;
     uint32_t spill_seq = 0;
     int64_t sorted_at_launch = 0;           // tile_launches when the last sort ran
-    int64_t sort_interval = 0;              // learned interval: spills ended the last one after this many steps
+    int64_t sort_interval = 0;              // steps between re-sorts (learned from the spill counts)
+    int64_t sort_safe = 0;                  // longest interval since the last spill-forced re-sort (0: sort_every)
+    int64_t sort_clean = 0;                 // re-sorts in a row that the interval, not spills, called for
     int last_phase = -1;
     // warm-start MLP weights (device), a host copy and a version that
     // changes with every mpm_set_warm_start_weights
