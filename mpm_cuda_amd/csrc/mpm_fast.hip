@@ -1402,7 +1402,9 @@ int sort_if_due(mpm_sim *s)
 
 int fast_prepare(mpm_sim *s)
 {
-    int rc = MPM_OK;
+    // a re-sort whose G2P did not complete: apply its order before the tiles are used
+    int rc = finish_g2p_move(s, false);
+    if (rc) return rc;
     if (sort_due(s)) {
         rc = sort_all(s, false);
         if (rc) return rc;

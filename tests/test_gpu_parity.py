@@ -269,6 +269,65 @@ def test_two_groups_match_plain(gpu, form):
             s.close()
 
 
+def test_resorts_applied_by_g2p_three_groups_warm(gpu):
+    """Jello, snow and water groups on one grid with the learned warm start,
+    re-sorted every second step: the FAST form's re-sorts are taken in the
+    G2P phase and applied by the G2P itself (every group written in its new
+    order into the other buffer, with ids, J_p / J and quaternions).  After
+    16 steps every field of every group, read back in the original particle
+    order, agrees with the plain form; the J_p plane of elastic snow
+    particles (not rewritten by a plain G2P) moved with them."""
+    from mpm_cuda_amd import warm_start
+    from tests.scenes import MAT_PARAMS, jittered_box
+    n_grid = 64
+    h = 1.0 / n_grid
+    sims = []
+    try:
+        for f in (capi.FORM_PLAIN, capi.FORM_FAST):
+            cfg = capi.default_config(h=h, nx=n_grid, ny=n_grid, nz=n_grid, dt=1e-4, form=f, sort_every=2,
+                                      warm_start=capi.WARM_MLP, warm_sweeps=2)
+            s = capi.Sim(cfg)
+            sims.append(s)
+            s.set_warm_start_weights(warm_start.load_weights())
+            boxes = ((capi.JELLO, 0.22, 0.38, 3.0), (capi.SNOW, 0.42, 0.58, -2.0), (capi.WATER, 0.62, 0.78, -3.0))
+            for k, (type_, lo, hi, vx) in enumerate(boxes):
+                E, nu, rho = MAT_PARAMS[type_]
+                x = jittered_box((lo, 0.3, 0.3), (hi, 0.5, 0.5), h, 4, 7 + k)
+                v = np.tile(np.array([[vx, 0.5, 0.0]], np.float32), (x.shape[0], 1))
+                s.add_particles(capi.material(E, nu, rho, type_), x, vel=v, volume=h ** 3)
+            # snow: a third of the particles compressed past the clamp band, so
+            # both the SVD (warm) path and the elastic path run; J_p away from 1
+            n = s.group_size(1)
+            rng = np.random.default_rng(5)
+            F = np.tile(np.eye(3, dtype=np.float32), (n, 1, 1))
+            pick = rng.random(n) < 1 / 3
+            F[pick] *= np.float32(0.96)
+            s.write(1, capi.P_F, F.reshape(n, 9))
+            s.write(1, capi.P_JP, (0.9 + 0.2 * rng.random((n, 1))).astype(np.float32))
+        a, b = sims
+        for _ in range(16):
+            a.step(1e-4)
+            b.step(1e-4)
+        assert b.stats()["sorts"] >= 8
+        for g in (0, 1, 2):
+            fields = {0: (capi.P_POS, capi.P_VEL, capi.P_C, capi.P_F), 1: (capi.P_POS, capi.P_VEL, capi.P_C, capi.P_F,
+                      capi.P_JP), 2: (capi.P_POS, capi.P_VEL, capi.P_C, capi.P_JP)}[g]
+            for field in fields:
+                ga, gb = a.read(g, field), b.read(g, field)
+                if field == capi.P_C:
+                    e = np.abs(gb - ga).max() * h / max(np.abs(a.read(g, capi.P_VEL)).max(), 1e-6)
+                else:
+                    e = rel(gb, ga)
+                print("group", g, "field", field, f"{e:.2e}")
+                assert e < 1e-5, (g, field, e)
+        sv = b.svd_stats()
+        print("fast", sv, "plain", a.svd_stats())
+        assert sv["svd"] > 0 and sv["warm"] > 0
+    finally:
+        for s in sims:
+            s.close()
+
+
 def test_fused_dt_change_and_rewrite(gpu):
     """FUSED form re-primes its scatter when dt changes or the state is
     rewritten; results equal the FAST form doing the same."""
