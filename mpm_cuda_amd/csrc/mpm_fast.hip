@@ -79,24 +79,21 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const uint32_t *in, int 
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_partials(uint32_t *partial, int np)
-{
-    __shared__ uint32_t lds[40];
-    uint32_t carry = 0;
-    for (int c = 0; c < np; c += 1024) {
-        int i = c + threadIdx.x;
-        uint32_t v = i < np ? partial[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_excl_scan(v, tot, lds);
-        if (i < np) partial[i] = carry + ex;
-        carry += tot;
-    }
-}
-
+// Each workgroup starts from the sum of the partials before it (one wave
+// adds them up), so the scan takes two launches.
 __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint32_t *out, int n,
                                                        const uint32_t *partial, uint32_t *total_out)
 {
     __shared__ uint32_t lds[40];
+    __shared__ uint32_t wg_base;
+    if (threadIdx.x < 64) {
+        uint32_t b = 0;
+        for (int j = (int)threadIdx.x; j < (int)blockIdx.x; j += 64) b += partial[j];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) b += __shfl_xor(b, d);
+        if (threadIdx.x == 0) wg_base = b;
+    }
+    __syncthreads();
     int i0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
@@ -106,7 +103,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint3
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = partial[blockIdx.x] + block_excl_scan(s, tot, lds);
+    uint32_t run = wg_base + block_excl_scan(s, tot, lds);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         if (i0 + k < n) {
@@ -124,7 +121,6 @@ static int scan_excl(hipStream_t st, const uint32_t *in, uint32_t *out, int n, u
     int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nt == 0) return 0;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(SCAN_T), 0, st, in, n, partial);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partial, nt);
     hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SCAN_T), 0, st, in, out, n, partial, total_out);
     return (int)hipGetLastError();
 }
@@ -132,10 +128,35 @@ static int scan_excl(hipStream_t st, const uint32_t *in, uint32_t *out, int n, u
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort with DB-bit digits (8, or 9 so that 17-18-bit block
 // keys take two passes instead of three).  A tile of RS_TILE keys is handled
-// by 4 waves, each owning a contiguous run of 1024 keys processed 64 at a
-// time; equal digits inside a wave are matched with DB ballots, so ranks
-// follow input order and the sort is stable.
-constexpr int RS_T = 256, RS_WAVE_KEYS = 1024, RS_ITERS = RS_WAVE_KEYS / 64, RS_TILE = 4 * RS_WAVE_KEYS;
+// by one workgroup of RS_WAVES waves, each owning a contiguous run of
+// RS_WAVE_KEYS keys processed 64 at a time; equal digits inside a wave are
+// matched with DB ballots, so ranks follow input order and the sort is
+// stable.  A pass is three launches: the digit histogram of every tile
+// (hist[d * ntiles + t]), the scan of each digit's column over the tiles
+// with the digit totals (k_rs_spine), and the scatter, whose workgroups
+// scan the totals into digit bases themselves.  Sixteen waves per tile keep
+// each wave's rank chain short, and the ~250 tiles of a 1M-key sort put
+// about four waves on every SIMD.
+constexpr int RS_WAVES = 16, RS_T = 64 * RS_WAVES, RS_WAVE_KEYS = 256, RS_ITERS = RS_WAVE_KEYS / 64,
+              RS_TILE = RS_WAVES * RS_WAVE_KEYS;
+constexpr int RS_MAX_DIGITS = 512;      // digits of up to 9 bits: the size of the totals buffer
+
+static inline int radix_digit_bits(int bits) { return (bits > 16 && bits <= 18) ? 9 : 8; }
+
+// Adds every active lane's digit d to the LDS histogram h: the lanes that
+// share the first active lane's digit (most of a wave, for nearly sorted
+// keys) as one add, the others one each.
+__device__ __forceinline__ void hist_add(uint32_t *h, bool active, uint32_t d)
+{
+    const uint64_t act = __ballot(active);
+    if (!act) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t dl = (uint32_t)__shfl((int)d, leader);
+    const uint64_t same = __ballot(active && d == dl);
+    if (lane == leader) atomicAdd(&h[dl], (uint32_t)__popcll(same));
+    else if (active && d != dl) atomicAdd(&h[d], 1u);
+}
 
 template <int DB>
 __global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t dmask,
@@ -146,66 +167,100 @@ __global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, i
     __shared__ uint32_t h[NB];
     for (int d = threadIdx.x; d < NB; d += RS_T) h[d] = 0;
     __syncthreads();
-    int base = blockIdx.x * RS_TILE;
-    for (int k = 0; k < RS_TILE / RS_T; ++k) {
-        int i = base + k * RS_T + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & MASK], 1u);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t dg[RS_ITERS];
+#pragma unroll
+    for (int it = 0; it < RS_ITERS; ++it) {
+        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        dg[it] = i < n ? (keys[i] >> shift) & MASK : 0u;
     }
+#pragma unroll
+    for (int it = 0; it < RS_ITERS; ++it)
+        hist_add(h, blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane < n, dg[it]);
     __syncthreads();
     for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
+// Workgroup d (one wave) replaces digit d's column hist[d * ntiles + t] by
+// its exclusive prefix over the tiles t and writes the digit's total.
+__global__ void __launch_bounds__(64) k_rs_spine(uint32_t *hist, int ntiles, uint32_t *totals)
+{
+    const int lane = threadIdx.x;
+    uint32_t *col = hist + (int64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += 64) {
+        const int t = t0 + lane;
+        const uint32_t v = t < ntiles ? col[t] : 0u;
+        const uint32_t inc = wave_incl_scan(v);
+        if (t < ntiles) col[t] = carry + inc - v;
+        carry += __shfl(inc, 63);
+    }
+    if (lane == 0) totals[blockIdx.x] = carry;
+}
+
+// Scatter of one pass; vin null means the values are the input positions.
 template <int DB>
 __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                      uint32_t *vout, int n, int shift, uint32_t dmask,
-                                                     const uint32_t *hist, int ntiles)
+                                                     const uint32_t *hist, const uint32_t *totals, int ntiles)
 {
     constexpr int NB = 1 << DB;
+    static_assert(NB <= RS_T, "one digit per thread");
     const uint32_t MASK = dmask & (NB - 1);
-    __shared__ uint32_t cnt[4][NB];
-    for (int k = 0; k < 4; ++k)
-        for (int d = threadIdx.x; d < NB; d += RS_T) cnt[k][d] = 0;
+    __shared__ uint32_t cnt[RS_WAVES][NB];
+    __shared__ uint32_t lds[40];
+    const int t = threadIdx.x;
+    for (int j = t; j < RS_WAVES * NB; j += RS_T) (&cnt[0][0])[j] = 0;
+    // this tile's first output position of digit t: the digit's base (the
+    // totals of the smaller digits) plus its count in the tiles before
+    const uint32_t tot_t = t < NB ? totals[t] : 0u;
+    uint32_t all;
+    uint32_t first = block_excl_scan(tot_t, all, lds);
+    if (t < NB) first += hist[(int64_t)t * ntiles + blockIdx.x];
     __syncthreads();
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = t >> 6, lane = t & 63;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     uint32_t key[RS_ITERS], val[RS_ITERS], rank[RS_ITERS];
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
-        bool valid = i < n;
-        uint32_t k = valid ? kin[i] : 0u;
-        uint32_t d = (k >> shift) & MASK;
+        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        key[it] = i < n ? kin[i] : 0u;
+        val[it] = i < n ? (vin ? vin[i] : (uint32_t)i) : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < RS_ITERS; ++it) {
+        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const bool valid = i < n;
+        const uint32_t d = (key[it] >> shift) & MASK;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < DB; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
-        uint32_t below = (uint32_t)__popcll(m & lt);
+        const uint32_t below = (uint32_t)__popcll(m & lt);
         uint32_t prev = 0;
         if (valid) prev = cnt[w][d];
         if (valid && below == 0) cnt[w][d] = prev + (uint32_t)__popcll(m);
-        key[it] = k;
-        val[it] = valid ? vin[i] : 0u;
         rank[it] = prev + below;
+        __asm__ volatile("" ::: "memory");
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < NB; d += RS_T) {
-        uint32_t run = 0;
-        for (int k = 0; k < 4; ++k) {
-            uint32_t t = cnt[k][d];
-            cnt[k][d] = run;
-            run += t;
+    if (t < NB) {
+        uint32_t run = first;
+        for (int k = 0; k < RS_WAVES; ++k) {
+            const uint32_t c = cnt[k][t];
+            cnt[k][t] = run;
+            run += c;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         if (i < n) {
-            uint32_t d = (key[it] >> shift) & MASK;
-            uint32_t dst = hist[(int64_t)d * ntiles + blockIdx.x] + cnt[w][d] + rank[it];
+            const uint32_t dst = cnt[w][(key[it] >> shift) & MASK] + rank[it];
             if (dst < (uint32_t)n) {      // always true for a consistent histogram
                 kout[dst] = key[it];
                 vout[dst] = val[it];
@@ -214,35 +269,60 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
     }
 }
 
-// Sorts (keys, vals) of length n by the low `bits` key bits.  Returns through
-// `in_alt` whether the result ended in the *_alt buffers.  `hist` holds
-// 512 x ntiles entries.
-static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
-                      int n, int bits, uint32_t *hist, uint32_t *partial, bool &in_alt)
+struct RadixResult {
+    uint32_t *keys, *vals;
+};
+
+// Sorts (keys, vals) of length n by the low `bits` key bits, ping-ponging
+// with the *_alt buffers.  `hist` holds 512 x ceil(n / RS_TILE) entries,
+// `totals` RS_MAX_DIGITS.  With hist0_ready the first pass's digit
+// histogram is already in `hist` (radix_digit_bits(bits)-bit digits, written
+// by the caller); with iota_vals the values are the input positions (vals is
+// not read); vals_out, when given (distinct from the other buffers), takes
+// the last pass's values.  res names the sorted keys and values.
+static int radix_sort_ex(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                         int n, int bits, uint32_t *hist, uint32_t *totals, bool hist0_ready, bool iota_vals,
+                         uint32_t *vals_out, RadixResult &res)
 {
-    in_alt = false;
-    if (n <= 1) return 0;
-    int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    res = RadixResult{keys, vals};
+    if (n <= 0) return 0;
+    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
     uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
-    const int db = (bits > 16 && bits <= 18) ? 9 : 8;
+    const int db = radix_digit_bits(bits);
     for (int shift = 0; shift < bits; shift += db) {
+        const bool first = shift == 0, last = shift + db >= bits;
         const int w = std::min(db, bits - shift);
         const uint32_t dmask = (w >= 32) ? ~0u : ((1u << w) - 1u);
-        if (db == 9) hipLaunchKernelGGL(k_rs_hist<9>, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, dmask, hist, ntiles);
-        else hipLaunchKernelGGL(k_rs_hist<8>, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, dmask, hist, ntiles);
-        int rc = scan_excl(st, hist, hist, (1 << db) * ntiles, partial, nullptr);
-        if (rc) return rc;
-        if (db == 9)
-            hipLaunchKernelGGL(k_rs_scatter<9>, dim3(ntiles), dim3(RS_T), 0, st, ka, va, kb, vb, n, shift, dmask, hist,
-                               ntiles);
-        else
-            hipLaunchKernelGGL(k_rs_scatter<8>, dim3(ntiles), dim3(RS_T), 0, st, ka, va, kb, vb, n, shift, dmask, hist,
-                               ntiles);
+        uint32_t *vdst = (last && vals_out) ? vals_out : vb;
+        const uint32_t *vsrc = (first && iota_vals) ? nullptr : va;
+        if (db == 9) {
+            if (!(first && hist0_ready))
+                hipLaunchKernelGGL(k_rs_hist<9>, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, dmask, hist, ntiles);
+            hipLaunchKernelGGL(k_rs_spine, dim3(1 << 9), dim3(64), 0, st, hist, ntiles, totals);
+            hipLaunchKernelGGL(k_rs_scatter<9>, dim3(ntiles), dim3(RS_T), 0, st, ka, vsrc, kb, vdst, n, shift, dmask,
+                               hist, totals, ntiles);
+        } else {
+            if (!(first && hist0_ready))
+                hipLaunchKernelGGL(k_rs_hist<8>, dim3(ntiles), dim3(RS_T), 0, st, ka, n, shift, dmask, hist, ntiles);
+            hipLaunchKernelGGL(k_rs_spine, dim3(1 << 8), dim3(64), 0, st, hist, ntiles, totals);
+            hipLaunchKernelGGL(k_rs_scatter<8>, dim3(ntiles), dim3(RS_T), 0, st, ka, vsrc, kb, vdst, n, shift, dmask,
+                               hist, totals, ntiles);
+        }
+        res = RadixResult{kb, vdst};
         std::swap(ka, kb);
         std::swap(va, vb);
-        in_alt = !in_alt;
     }
     return (int)hipGetLastError();
+}
+
+// The plain form: in_alt tells whether the result ended in the *_alt buffers.
+static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                      int n, int bits, uint32_t *hist, uint32_t *totals, bool &in_alt)
+{
+    RadixResult r{};
+    const int rc = radix_sort_ex(st, keys, vals, keys_alt, vals_alt, n, bits, hist, totals, false, false, nullptr, r);
+    in_alt = r.keys != keys;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -264,16 +344,43 @@ __device__ __forceinline__ void base_of(const GridP &g, const PartView &pv, int 
     }
 }
 
-__global__ void k_block_keys(GridP g, PartView pv, uint32_t *keys, uint32_t *vals)
+// Block keys of the particles (linear index of the 4^3-node block holding
+// the base node) and the radix sort's first digit histogram (DB-bit digits,
+// dmask the first digit's bits) of every RS_TILE-particle tile, laid out as
+// k_rs_hist writes it.  The workgroups also clear the nb-entry block arrays
+// z0, z1, z2 that the run and tile kernels of this sort fill.
+template <int DB>
+__global__ void __launch_bounds__(RS_T) k_block_keys_hist(GridP g, PartView pv, uint32_t *keys, uint32_t dmask,
+                                                          uint32_t *hist, int ntiles, uint32_t *z0, uint32_t *z1,
+                                                          uint32_t *z2, int nb)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pv.n) return;
-    int b[3];
-    base_of(g, pv, i, b);
-    uint32_t blk = ((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
-                   (uint32_t)(b[0] >> 2);
-    keys[i] = blk << KEY_LOW;
-    vals[i] = (uint32_t)i;
+    constexpr int NB = 1 << DB;
+    const uint32_t MASK = dmask & (NB - 1);
+    __shared__ uint32_t h[NB];
+    for (int d = threadIdx.x; d < NB; d += RS_T) h[d] = 0;
+    for (int j = blockIdx.x * RS_T + threadIdx.x; j < nb; j += gridDim.x * RS_T) {
+        z0[j] = 0u;
+        z1[j] = 0u;
+        z2[j] = 0u;
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int it = 0; it < RS_ITERS; ++it) {
+        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const bool valid = i < pv.n;
+        uint32_t key = 0;
+        if (valid) {
+            int b[3];
+            base_of(g, pv, i, b);
+            key = (((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
+                   (uint32_t)(b[0] >> 2)) << KEY_LOW;
+            keys[i] = key;
+        }
+        hist_add(h, valid, key & MASK);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
 // Gathers every permuted SoA plane and the particle ids.
@@ -294,21 +401,20 @@ __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, i
     pida[off + j] = pid[off + src];
 }
 
-// First and one-past-last sorted position of every block (untouched blocks
-// keep the zeros written before).
-__global__ void k_block_runs(const uint32_t *__restrict__ skeys, int n, uint32_t *blk_start, uint32_t *blk_end)
+// First and one-past-last sorted position of every block, and has[b] = 1
+// for every non-empty block (all three cleared before, untouched blocks
+// keep the zeros).
+__global__ void k_block_runs(const uint32_t *__restrict__ skeys, int n, uint32_t *blk_start, uint32_t *blk_end,
+                             uint32_t *has)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t b = skeys[i] >> KEY_LOW;
-    if (i == 0 || (skeys[i - 1] >> KEY_LOW) != b) blk_start[b] = (uint32_t)i;
+    if (i == 0 || (skeys[i - 1] >> KEY_LOW) != b) {
+        blk_start[b] = (uint32_t)i;
+        has[b] = 1u;
+    }
     if (i == n - 1 || (skeys[i + 1] >> KEY_LOW) != b) blk_end[b] = (uint32_t)(i + 1);
-}
-
-__global__ void k_tile_flags(const uint32_t *blk_start, const uint32_t *blk_end, uint32_t *has, int nblocks)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nblocks) has[b] = blk_end[b] > blk_start[b] ? 1u : 0u;
 }
 
 // One tile per non-empty block: (first particle, count, block, tile id).
@@ -1207,8 +1313,8 @@ static int ensure_fast_buffers(mpm_sim *s)
         for (void *p : ptrs) if (p) (void)hipFree(p);
         int64_t ntiles = (nmax + RS_TILE - 1) / RS_TILE;
         int64_t hist_len = 512 * ntiles;     // digits of up to 9 bits
-        int64_t scan_words = std::max<int64_t>((hist_len + SCAN_TILE - 1) / SCAN_TILE,
-                                               (s->nblocks + SCAN_TILE - 1) / SCAN_TILE) + 16;
+        // scan partials of the tile numbering, or the radix digit totals
+        int64_t scan_words = std::max<int64_t>(RS_MAX_DIGITS, (s->nblocks + SCAN_TILE - 1) / SCAN_TILE + 16);
         MPM_HIP(hipMalloc(&s->keys, nmax * 4));
         MPM_HIP(hipMalloc(&s->keys_alt, nmax * 4));
         MPM_HIP(hipMalloc(&s->vals, nmax * 4));
@@ -1273,29 +1379,34 @@ int sort_all(mpm_sim *s, bool defer)
     if (!s->warm_cnt) plane_mask &= ~(((1ull << 4) - 1ull) << PL_Q);
     else if (!defer && (rc = launch_zero_u32(st, s->warm_cnt, 2 * MAX_GROUPS))) return hip_fail((hipError_t)rc, "zero");
     const int nb = (int)s->nblocks;
+    const int db = radix_digit_bits(bits);
+    const uint32_t dmask0 = (1u << std::min(db, bits)) - 1u;
     for (size_t k = 0; k < s->groups.size(); ++k) {
         PartView pv = view_of(s, (int)k);
         int n = pv.n;
         uint32_t *blk_end = s->blk_count;
-        if ((rc = launch_zero_u32(st, s->blk_start, nb)) || (rc = launch_zero_u32(st, blk_end, nb)))
-            return hip_fail((hipError_t)rc, "zero");
-        hipLaunchKernelGGL(k_block_keys, dim3(nblk(n, 256)), dim3(256), 0, st, s->g, pv, s->keys, s->vals);
-        bool in_alt = false;
-        if ((rc = radix_sort(st, s->keys, s->vals, s->keys_alt, s->vals_alt, n, bits, s->sort_hist, s->sort_scan,
-                             in_alt)))
-            return hip_fail((hipError_t)rc, "radix_sort");
-        const uint32_t *perm = in_alt ? s->vals_alt : s->vals;
-        const uint32_t *skeys = in_alt ? s->keys_alt : s->keys;
-        if (defer)
-            MPM_HIP(hipMemcpyAsync(s->g2p_perm + pv.off, perm, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                                   st));
+        // keys, first digit histogram, cleared block arrays; the sort's
+        // values are the particles' current slots
+        const int ntiles = nblk(n, RS_TILE);
+        if (db == 9)
+            hipLaunchKernelGGL(k_block_keys_hist<9>, dim3(ntiles), dim3(RS_T), 0, st, s->g, pv, s->keys, dmask0,
+                               s->sort_hist, ntiles, s->blk_start, blk_end, s->scan_tmp, nb);
         else
+            hipLaunchKernelGGL(k_block_keys_hist<8>, dim3(ntiles), dim3(RS_T), 0, st, s->g, pv, s->keys, dmask0,
+                               s->sort_hist, ntiles, s->blk_start, blk_end, s->scan_tmp, nb);
+        // a deferred order goes straight to the G2P's permutation
+        RadixResult r{};
+        if ((rc = radix_sort_ex(st, s->keys, s->vals, s->keys_alt, s->vals_alt, n, bits, s->sort_hist, s->sort_scan,
+                                true, true, defer ? s->g2p_perm + pv.off : nullptr, r)))
+            return hip_fail((hipError_t)rc, "radix_sort");
+        const uint32_t *perm = r.vals;
+        const uint32_t *skeys = r.keys;
+        if (!defer)
             hipLaunchKernelGGL(k_permute, dim3(round_xcd((uint32_t)nblk(n, 256))), dim3(256), 0, st, s->P, s->P_alt,
                                s->cap, pv.off, n, perm, s->pid, s->pid_alt, plane_mask);
         // tiles: one per non-empty block, numbered in block order
-        hipLaunchKernelGGL(k_block_runs, dim3(nblk(n, 256)), dim3(256), 0, st, skeys, n, s->blk_start, blk_end);
-        hipLaunchKernelGGL(k_tile_flags, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_start, blk_end, s->scan_tmp,
-                           nb);
+        hipLaunchKernelGGL(k_block_runs, dim3(nblk(n, 256)), dim3(256), 0, st, skeys, n, s->blk_start, blk_end,
+                           s->scan_tmp);
         if ((rc = scan_excl(st, s->scan_tmp, s->scan_tmp, nb, s->sort_scan, s->tile_counts + k)))
             return hip_fail((hipError_t)rc, "scan");
         hipLaunchKernelGGL(k_emit_tiles, dim3(nblk(nb, 256)), dim3(256), 0, st, s->blk_start, blk_end, s->scan_tmp,
@@ -1589,7 +1700,7 @@ int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, 
     if (bits < 1 || bits > 32) return set_error(MPM_ERR_INVALID, "bits out of range [1,32]");
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     int64_t hist_len = 512 * ntiles;         // digits of up to 9 bits
-    int64_t scan_words = (hist_len + SCAN_TILE - 1) / SCAN_TILE + 16;
+    int64_t scan_words = RS_MAX_DIGITS;     // digit totals
     uint32_t *k = nullptr, *ka = nullptr, *v = nullptr, *va = nullptr, *hist = nullptr, *scan = nullptr;
     hipStream_t st = nullptr;
     int rc = MPM_OK;
@@ -1618,21 +1729,21 @@ int probe_sort(const uint32_t *keys_h, int64_t n, int bits, uint32_t *sorted_h, 
 
 // ---------------------------------------------------------------------------
 // Sort that allocates nothing (mpm_sort_keys_noalloc): the ping-pong key and
-// value buffers, the digit histograms and the scan partials of the radix
+// value buffers, the digit histograms and the digit totals of the radix
 // sort live in one fixed device array, `scratch`, sized for
 // MPM_SORT_NOALLOC_MAX_KEYS keys (about 8.9 MB, held by every GPU the
 // library runs on until the process ends).  Host code never passes the
 // array's name to a kernel: each device's address of it is looked up once
 // (hipGetSymbolAddress with that device current) and kept per device.
 // Nothing in it needs to start at zero -- the histogram kernel writes every
-// entry it later reads, and so do the scan kernels -- but every call on one
+// entry it later reads, and so does the column-scan kernel -- but every call on one
 // GPU shares it, so calls that overlap on one GPU (different streams or
 // threads) race.
 constexpr int64_t NA_KEYS = MPM_SORT_NOALLOC_MAX_KEYS;
 constexpr int64_t NA_TILES = (NA_KEYS + RS_TILE - 1) / RS_TILE;
 constexpr int64_t NA_HIST = 512 * NA_TILES;                              // digits of up to 9 bits
-constexpr int64_t NA_PARTIAL = (NA_HIST + SCAN_TILE - 1) / SCAN_TILE + 16;
-constexpr int64_t NA_WORDS = 2 * NA_KEYS + NA_HIST + NA_PARTIAL;
+constexpr int64_t NA_TOTALS = RS_MAX_DIGITS;
+constexpr int64_t NA_WORDS = 2 * NA_KEYS + NA_HIST + NA_TOTALS;
 static __device__ __align__(256) uint32_t scratch[NA_WORDS];
 
 constexpr int NA_MAX_DEVICES = 64;
@@ -1673,14 +1784,14 @@ int sort_keys_noalloc(const uint32_t *keys, int64_t n, int bits, uint32_t *sorte
     uint32_t *base = nullptr;
     int rc = noalloc_scratch(&base);
     if (rc) return rc;
-    uint32_t *ka = base, *va = base + NA_KEYS, *hist = base + 2 * NA_KEYS, *partial = hist + NA_HIST;
+    uint32_t *ka = base, *va = base + NA_KEYS, *hist = base + 2 * NA_KEYS, *totals = hist + NA_HIST;
     hipError_t e = hipSuccess;
     if (keys != sorted) e = hipMemcpyAsync(sorted, keys, (size_t)n * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "copy keys");
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, perm, (int)n);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "iota");
     bool in_alt = false;
-    rc = radix_sort(st, sorted, perm, ka, va, (int)n, bits, hist, partial, in_alt);
+    rc = radix_sort(st, sorted, perm, ka, va, (int)n, bits, hist, totals, in_alt);
     if (rc) return hip_fail((hipError_t)rc, "radix_sort");
     if (in_alt) {
         if ((e = hipMemcpyAsync(sorted, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess)

@@ -71,6 +71,19 @@ def test_radix_sort_stable(gpu):
     srt, perm = capi.probe_sort(keys, 17)
     ref = np.argsort(keys & ((1 << 17) - 1), kind="stable")
     assert np.array_equal(perm, ref.astype(np.uint32))
+    # a re-sort's keys: sorted runs of ~300 equal block keys with 3% of the
+    # particles moved to a neighbouring block (runs of equal digits fill
+    # whole waves)
+    for n, bits in [(1_000_003, 17), (262_144, 15), (4097, 9)]:
+        keys = np.sort(rng.integers(0, 1 << bits, size=n // 300 + 1, dtype=np.int64))
+        keys = np.repeat(keys, 300)[:n]
+        moved = rng.random(n) < 0.03
+        keys[moved] += rng.choice([-1, 1, -48, 48, -2304, 2304], size=int(moved.sum()))
+        keys = np.clip(keys, 0, (1 << bits) - 1).astype(np.uint32)
+        srt, perm = capi.probe_sort(keys, bits)
+        ref = np.argsort(keys, kind="stable")
+        assert np.array_equal(perm, ref.astype(np.uint32)), (n, bits)
+        assert np.array_equal(srt, keys[ref])
 
 
 def test_svd_vs_lapack_fixtures(gpu):
