@@ -317,6 +317,13 @@ int mpm_probe_point_in_mesh(const char *obj_path, const double *pts, int64_t n, 
  * launches after one warm-up, in GB/s. */
 int mpm_probe_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs);
 
+/* Bandwidth of a structure-of-arrays pass shaped like a particle update: n
+ * elements (1 to 2^28), planes_in FP32 planes read and planes_out written
+ * (1 to 64 each), one element per thread; median over `reps` timed launches
+ * after one warm-up, bytes read + written per second in GB/s.  (A 1M-particle
+ * FAST G2P moves 13 planes in and 31 out.) */
+int mpm_probe_soa_bandwidth(int64_t n, int32_t planes_in, int32_t planes_out, int32_t reps, double *gbs);
+
 /* LDS scatter rate on the device: 8,192 one-wave workgroups each add, for
  * reps x 64 particles, a 27-node x 4-float P2G stencil into an 8 KB LDS
  * arena; *ms is the launch's duration.  mode 0: float atomics (ds_add_f32),

@@ -117,6 +117,7 @@ _SIGS = {
     "mpm_probe_sort": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "mpm_sort_keys_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "mpm_probe_soa_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "mpm_probe_lds_scatter": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "mpm_probe_stress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
                                    C.c_int32, C.c_void_p]),
@@ -611,6 +612,14 @@ def probe_copy_bandwidth(bytes_: int = 1 << 30, reps: int = 10) -> float:
     """GB/s of a streaming device copy (read + write), median of reps."""
     g = C.c_double()
     check(lib().mpm_probe_copy_bandwidth(int(bytes_), int(reps), C.byref(g)))
+    return g.value
+
+
+def probe_soa_bandwidth(n: int = 1_000_000, planes_in: int = 13, planes_out: int = 31, reps: int = 10) -> float:
+    """GB/s (read + write) of a structure-of-arrays pass: n elements,
+    planes_in FP32 planes read, planes_out written, median of reps."""
+    g = C.c_double()
+    check(lib().mpm_probe_soa_bandwidth(int(n), int(planes_in), int(planes_out), int(reps), C.byref(g)))
     return g.value
 
 

@@ -84,6 +84,21 @@ __global__ void __launch_bounds__(256) k_probe_copy(const float4 *__restrict__ s
     if (i < n4) gstore4(dst, i, gload4(src, i));
 }
 
+// Structure-of-arrays pass shaped like a particle update: element i reads
+// plane p (< nin) at in[p * n + i] and writes plane q (< nout) at
+// out[q * n + i], one element per thread.
+__global__ void __launch_bounds__(256) k_probe_soa(const float *__restrict__ in, float *__restrict__ out, int64_t n,
+                                                   int nin, int nout)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.f;
+#pragma unroll 8
+    for (int p = 0; p < nin; ++p) acc += gp(in)[p * n + i];
+#pragma unroll 8
+    for (int q = 0; q < nout; ++q) gp(out)[q * n + i] = acc + (float)q;
+}
+
 // LDS scatter probe: one-wave workgroups each add, for `reps` x 64
 // particles, 27 node updates of 4 floats (a particle's P2G stencil in a 4^3
 // block) into an 8 KB arena.  Lane cells: mode bit 0 clear -- lane l owns
@@ -198,6 +213,42 @@ int mpm_probe_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs)
     std::sort(ms.begin(), ms.end());
     const double med = ms[ms.size() / 2];
     *gbs = 2.0 * (double)bytes / (med * 1e-3) / 1e9;
+    return MPM_OK;
+}
+
+int mpm_probe_soa_bandwidth(int64_t n, int32_t planes_in, int32_t planes_out, int32_t reps, double *gbs)
+{
+    if (!gbs) return set_error(MPM_ERR_INVALID, "null argument");
+    if (n < 1 || n > (1ll << 28)) return set_error(MPM_ERR_INVALID, "n out of range [1, 2^28]");
+    if (planes_in < 1 || planes_in > 64 || planes_out < 1 || planes_out > 64)
+        return set_error(MPM_ERR_INVALID, "planes out of range [1,64]");
+    if (reps < 1 || reps > 100) return set_error(MPM_ERR_INVALID, "reps out of range [1,100]");
+    DevBufs b;
+    float *in = nullptr, *out = nullptr;
+    MPM_HIP(b.alloc(&in, (size_t)n * planes_in * sizeof(float)));
+    MPM_HIP(b.alloc(&out, (size_t)n * planes_out * sizeof(float)));
+    MPM_HIP(hipMemset(in, 0, (size_t)n * planes_in * sizeof(float)));
+    const unsigned wg = (unsigned)((n + 255) / 256);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MPM_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    std::vector<float> ms;
+    for (int r = 0; r <= reps && err == hipSuccess; ++r) {            // the first launch is a warm-up
+        if ((err = hipEventRecord(e0, 0)) != hipSuccess) break;
+        hipLaunchKernelGGL(k_probe_soa, dim3(wg), dim3(256), 0, 0, in, out, n, planes_in, planes_out);
+        if ((err = hipGetLastError()) != hipSuccess) break;
+        if ((err = hipEventRecord(e1, 0)) != hipSuccess) break;
+        if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+        float t = 0.f;
+        if ((err = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) break;
+        if (r > 0) ms.push_back(t);
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    MPM_HIP(err);
+    std::sort(ms.begin(), ms.end());
+    const double med = ms[ms.size() / 2];
+    *gbs = (double)n * (planes_in + planes_out) * sizeof(float) / (med * 1e-3) / 1e9;
     return MPM_OK;
 }
 

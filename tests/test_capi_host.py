@@ -148,6 +148,17 @@ def test_config_tile_shape_checked_before_device():
             capi.Sim(capi.default_config(nx=32, ny=32, nz=32, h=1 / 32, tile=bad))
 
 
+def test_bandwidth_probe_arguments_checked_before_device():
+    """The bandwidth probes refuse bad sizes before touching a device."""
+    for args in ((0, 13, 31), (1 << 29, 13, 31), (1000, 0, 31), (1000, 13, 65)):
+        with pytest.raises(capi.MpmError, match="range"):
+            capi.probe_soa_bandwidth(*args, reps=5)
+    with pytest.raises(capi.MpmError, match="reps"):
+        capi.probe_soa_bandwidth(1000, 13, 31, reps=0)
+    with pytest.raises(capi.MpmError, match="multiple of 16"):
+        capi.probe_copy_bandwidth(1000, 5)
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
 def test_create_without_gpu_reports_error():
     with pytest.raises(capi.MpmError, match="no HIP device"):

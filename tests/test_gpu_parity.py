@@ -493,6 +493,17 @@ def test_copy_bandwidth_probe(gpu):
         capi.probe_copy_bandwidth(1 << 20, 0)
 
 
+def test_soa_bandwidth_probe(gpu):
+    """The structure-of-arrays probe (a G2P-shaped pass: 13 planes read, 31
+    written, 1M elements): a plausible figure.  Its 176 MB fit the MALL, so
+    the rate may exceed the HBM peak."""
+    g = capi.probe_soa_bandwidth(1_000_000, 13, 31, 5)
+    print(f"soa 13->31 {g:.1f} GB/s")
+    assert 500.0 < g < 40000.0
+    with pytest.raises(capi.MpmError):
+        capi.probe_soa_bandwidth(1000, 0, 31, 5)
+
+
 def test_lds_scatter_probe(gpu):
     """The LDS scatter probe behind the P2G design: every mode runs and the
     float atomics (ds_add_f32) are slower than a plain float4
