@@ -123,6 +123,27 @@ int g2p_wgs(const mpm_batch *b, int mat, int threads)
 
 int check(int rc, const char *what) { return rc ? hip_fail((hipError_t)rc, what) : MPM_OK; }
 
+// The launches of every material present (`launch(stream, material)`), as
+// parallel branches: forked from the batch stream onto the stream of each
+// material's first scene and joined back (the materials' scenes are
+// disjoint, so one launch's tail overlaps the others' work).
+template <class L>
+int fork_materials(mpm_batch *b, L launch)
+{
+    MPM_HIP(hipEventRecord(b->ev_fork, b->stream));
+    for (int m = MPM_JELLO; m <= MPM_WATER; ++m) {
+        size_t k = 0;
+        while (k < b->types.size() && b->types[k] != m) ++k;
+        if (k == b->types.size()) continue;
+        hipStream_t ms = b->sims[k]->stream;
+        MPM_HIP(hipStreamWaitEvent(ms, b->ev_fork, 0));
+        if (int rc = launch(ms, m)) return rc;
+        MPM_HIP(hipEventRecord(b->ev_join[k], ms));
+        MPM_HIP(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
+    }
+    return MPM_OK;
+}
+
 // One step of every scene on the batch stream (parity p tables).  With
 // `ev` (6 events) the P2G, grid-update and G2P launches are bracketed.
 int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int tile, hipEvent_t *ev)
@@ -133,8 +154,10 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int ti
     int rc;
     if ((rc = check(launch_batch_zero(st, tab, n), "batch_zero"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[0], st));
-    for (int m = MPM_JELLO; m <= MPM_WATER; ++m)
-        if ((rc = check(launch_p2g_batch(st, m, tab, b->types.data(), n, p2g, tile), "p2g_batch"))) return rc;
+    rc = fork_materials(b, [&](hipStream_t ms, int m) {
+        return check(launch_p2g_batch(ms, m, tab, b->types.data(), n, p2g, tile), "p2g_batch");
+    });
+    if (rc) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[1], st));
     if ((rc = check(launch_spill_batch(st, tab, n, SPILL_WGS), "spill_batch"))) return rc;
     if ((rc = check(launch_compact_batch(st, tab, n, b->compact_wgs), "compact_batch"))) return rc;
@@ -146,10 +169,11 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int ti
                         "mlp_infer_batch")))
             return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[4], st));
-    for (int m = MPM_JELLO; m <= MPM_WATER; ++m) {
+    rc = fork_materials(b, [&](hipStream_t ms, int m) {
         const int wgs = g2p_wgs(b, m, g2p_threads);
-        if (wgs && (rc = check(launch_g2p_batch(st, m, tab, n, wgs, g2p_threads), "g2p_batch"))) return rc;
-    }
+        return wgs ? check(launch_g2p_batch(ms, m, tab, n, wgs, g2p_threads), "g2p_batch") : MPM_OK;
+    });
+    if (rc) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[5], st));
     return MPM_OK;
 }
