@@ -69,6 +69,10 @@ struct mpm_batch {
     int64_t total = 0;
     int compact_wgs = 0;
     std::vector<int> types;             // material of each scene
+    // scenes are kept ordered by material: material m holds table entries
+    // [mat_begin[m], mat_begin[m] + mat_count[m]); its block compaction
+    // takes mat_compact_wgs[m] workgroups
+    int mat_begin[3] = {0, 0, 0}, mat_count[3] = {0, 0, 0}, mat_compact_wgs[3] = {0, 0, 0};
     // warm-start weight sets: scenes with equal weights share one inference launch
     std::vector<const float *> warm_params; // device weights of each set
     std::vector<int64_t> warm_chunks;       // 64-particle chunks of each set
@@ -144,14 +148,56 @@ int fork_materials(mpm_batch *b, L launch)
     return MPM_OK;
 }
 
-// One step of every scene on the batch stream (parity p tables).  With
-// `ev` (6 events) the P2G, grid-update and G2P launches are bracketed.
+// The stages of one material's scenes (table range [s0, s0 + cnt)) on
+// stream ms: counter reset, P2G, spill scatter, block compaction, grid
+// update and, with `g2p`, the G2P.
+int enqueue_material(mpm_batch *b, hipStream_t ms, int m, const BatchSlice *tab, int s0, int cnt, int p2g, int grid,
+                     int g2p_threads, int tile, bool g2p)
+{
+    const BatchSlice *t = tab + s0;
+    int rc;
+    if ((rc = check(launch_batch_zero(ms, t, cnt), "batch_zero"))) return rc;
+    if ((rc = check(launch_p2g_batch(ms, m, t, b->types.data() + s0, cnt, p2g, tile), "p2g_batch"))) return rc;
+    if ((rc = check(launch_spill_batch(ms, t, cnt, SPILL_WGS), "spill_batch"))) return rc;
+    if ((rc = check(launch_compact_batch(ms, t, cnt, b->mat_compact_wgs[m]), "compact_batch"))) return rc;
+    if ((rc = check(launch_grid_batch(ms, t, cnt, grid), "grid_batch"))) return rc;
+    if (g2p) {
+        const int wgs = g2p_wgs(b, m, g2p_threads);
+        if (wgs && (rc = check(launch_g2p_batch(ms, m, t, cnt, wgs, g2p_threads), "g2p_batch"))) return rc;
+    }
+    return MPM_OK;
+}
+
+// One step of every scene (parity p tables).  Without `ev` each material's
+// scenes run their whole step as one branch beside the other materials'
+// (with a warm start the branches join for the inference launches, which
+// span materials, and fork again for the G2P).  With `ev` (6 events, the
+// autotuner's stage timing) the stages follow one another on the batch
+// stream, only the materials' P2G and G2P launches side by side, and the
+// P2G, grid-update and G2P stages are bracketed.
 int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int tile, hipEvent_t *ev)
 {
     hipStream_t st = b->stream;
     const BatchSlice *tab = b->tab[p];
     const int n = (int)b->sims.size();
     int rc;
+    if (!ev) {
+        const bool warm = !b->warm_params.empty();
+        rc = fork_materials(b, [&](hipStream_t ms, int m) {
+            return enqueue_material(b, ms, m, tab, b->mat_begin[m], b->mat_count[m], p2g, grid, g2p_threads, tile,
+                                    !warm);
+        });
+        if (rc || !warm) return rc;
+        for (size_t w = 0; w < b->warm_params.size(); ++w)
+            if ((rc = check(launch_mlp_infer_batch(st, b->warm_params[w], tab, n, (int)w, b->warm_chunks[w]),
+                            "mlp_infer_batch")))
+                return rc;
+        return fork_materials(b, [&](hipStream_t ms, int m) {
+            const int wgs = g2p_wgs(b, m, g2p_threads);
+            const BatchSlice *t = tab + b->mat_begin[m];
+            return wgs ? check(launch_g2p_batch(ms, m, t, b->mat_count[m], wgs, g2p_threads), "g2p_batch") : MPM_OK;
+        });
+    }
     if ((rc = check(launch_batch_zero(st, tab, n), "batch_zero"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[0], st));
     rc = fork_materials(b, [&](hipStream_t ms, int m) {
@@ -481,6 +527,11 @@ int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *sha
     }
     mpm_batch *b = new mpm_batch();
     b->sims.assign(sims, sims + n);
+    // ordered by material (stable): each material's scenes form one range of
+    // the tables, so a step can run the materials side by side
+    std::stable_sort(b->sims.begin(), b->sims.end(), [](const mpm_sim *x, const mpm_sim *y) {
+        return x->groups[0].mat.type < y->groups[0].mat.type;
+    });
     b->shape = sh;
     auto fail = [&](int code) { mpm_batch_destroy(b); return code; };
     for (mpm_sim *s : b->sims) {
@@ -494,6 +545,12 @@ int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *sha
         b->P0.push_back(s->P); b->P1.push_back(s->P_alt);
         b->pid0.push_back(s->pid); b->pid1.push_back(s->pid_alt);
         b->compact_wgs += (int)((s->nblocks + 255) / 256);
+    }
+    for (int k = 0; k < n; ++k) {
+        const int m = b->types[k];
+        if (b->mat_count[m] == 0) b->mat_begin[m] = k;
+        b->mat_count[m] += 1;
+        b->mat_compact_wgs[m] += (int)((b->sims[k]->nblocks + 255) / 256);
     }
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(hip_fail(e, "stream"));
