@@ -73,6 +73,9 @@ struct mpm_batch {
     // [mat_begin[m], mat_begin[m] + mat_count[m]); its block compaction
     // takes mat_compact_wgs[m] workgroups
     int mat_begin[3] = {0, 0, 0}, mat_count[3] = {0, 0, 0}, mat_compact_wgs[3] = {0, 0, 0};
+    // warm-start sets present among each material's scenes, with their
+    // 64-particle chunks there
+    std::vector<std::pair<int, int64_t>> mat_sets[3];
     // warm-start weight sets: scenes with equal weights share one inference launch
     std::vector<const float *> warm_params; // device weights of each set
     std::vector<int64_t> warm_chunks;       // 64-particle chunks of each set
@@ -148,11 +151,11 @@ int fork_materials(mpm_batch *b, L launch)
     return MPM_OK;
 }
 
-// The stages of one material's scenes (table range [s0, s0 + cnt)) on
-// stream ms: counter reset, P2G, spill scatter, block compaction, grid
-// update and, with `g2p`, the G2P.
+// The step of one material's scenes (table range [s0, s0 + cnt)) on stream
+// ms: counter reset, P2G, spill scatter, block compaction, grid update, the
+// warm-start inference of each weight set among them, G2P.
 int enqueue_material(mpm_batch *b, hipStream_t ms, int m, const BatchSlice *tab, int s0, int cnt, int p2g, int grid,
-                     int g2p_threads, int tile, bool g2p)
+                     int g2p_threads, int tile)
 {
     const BatchSlice *t = tab + s0;
     int rc;
@@ -161,43 +164,30 @@ int enqueue_material(mpm_batch *b, hipStream_t ms, int m, const BatchSlice *tab,
     if ((rc = check(launch_spill_batch(ms, t, cnt, SPILL_WGS), "spill_batch"))) return rc;
     if ((rc = check(launch_compact_batch(ms, t, cnt, b->mat_compact_wgs[m]), "compact_batch"))) return rc;
     if ((rc = check(launch_grid_batch(ms, t, cnt, grid), "grid_batch"))) return rc;
-    if (g2p) {
-        const int wgs = g2p_wgs(b, m, g2p_threads);
-        if (wgs && (rc = check(launch_g2p_batch(ms, m, t, cnt, wgs, g2p_threads), "g2p_batch"))) return rc;
-    }
+    for (const auto &ws : b->mat_sets[m])
+        if ((rc = check(launch_mlp_infer_batch(ms, b->warm_params[ws.first], t, cnt, ws.first, ws.second),
+                        "mlp_infer_batch")))
+            return rc;
+    const int wgs = g2p_wgs(b, m, g2p_threads);
+    if (wgs && (rc = check(launch_g2p_batch(ms, m, t, cnt, wgs, g2p_threads), "g2p_batch"))) return rc;
     return MPM_OK;
 }
 
 // One step of every scene (parity p tables).  Without `ev` each material's
-// scenes run their whole step as one branch beside the other materials'
-// (with a warm start the branches join for the inference launches, which
-// span materials, and fork again for the G2P).  With `ev` (6 events, the
-// autotuner's stage timing) the stages follow one another on the batch
-// stream, only the materials' P2G and G2P launches side by side, and the
-// P2G, grid-update and G2P stages are bracketed.
+// scenes run their whole step as one branch beside the other materials'.
+// With `ev` (6 events, the autotuner's stage timing) the stages follow one
+// another on the batch stream, only the materials' P2G and G2P launches side
+// by side, and the P2G, grid-update and G2P stages are bracketed.
 int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int tile, hipEvent_t *ev)
 {
     hipStream_t st = b->stream;
     const BatchSlice *tab = b->tab[p];
     const int n = (int)b->sims.size();
     int rc;
-    if (!ev) {
-        const bool warm = !b->warm_params.empty();
-        rc = fork_materials(b, [&](hipStream_t ms, int m) {
-            return enqueue_material(b, ms, m, tab, b->mat_begin[m], b->mat_count[m], p2g, grid, g2p_threads, tile,
-                                    !warm);
-        });
-        if (rc || !warm) return rc;
-        for (size_t w = 0; w < b->warm_params.size(); ++w)
-            if ((rc = check(launch_mlp_infer_batch(st, b->warm_params[w], tab, n, (int)w, b->warm_chunks[w]),
-                            "mlp_infer_batch")))
-                return rc;
+    if (!ev)
         return fork_materials(b, [&](hipStream_t ms, int m) {
-            const int wgs = g2p_wgs(b, m, g2p_threads);
-            const BatchSlice *t = tab + b->mat_begin[m];
-            return wgs ? check(launch_g2p_batch(ms, m, t, b->mat_count[m], wgs, g2p_threads), "g2p_batch") : MPM_OK;
+            return enqueue_material(b, ms, m, tab, b->mat_begin[m], b->mat_count[m], p2g, grid, g2p_threads, tile);
         });
-    }
     if ((rc = check(launch_batch_zero(st, tab, n), "batch_zero"))) return rc;
     if (ev) MPM_HIP(hipEventRecord(ev[0], st));
     rc = fork_materials(b, [&](hipStream_t ms, int m) {
@@ -310,6 +300,17 @@ void group_warm(mpm_batch *b)
         }
         b->warm_group[k] = g;
         b->warm_chunks[g] += (s->groups[0].n + 63) / 64;
+    }
+    for (int m = 0; m < 3; ++m) {
+        b->mat_sets[m].clear();
+        for (int k = b->mat_begin[m]; k < b->mat_begin[m] + b->mat_count[m]; ++k) {
+            const int g = b->warm_group[k];
+            if (g < 0) continue;
+            size_t j = 0;
+            while (j < b->mat_sets[m].size() && b->mat_sets[m][j].first != g) ++j;
+            if (j == b->mat_sets[m].size()) b->mat_sets[m].push_back({g, 0});
+            b->mat_sets[m][j].second += (b->sims[k]->groups[0].n + 63) / 64;
+        }
     }
 }
 
