@@ -73,6 +73,7 @@ struct mpm_batch {
     // [mat_begin[m], mat_begin[m] + mat_count[m]); its block compaction
     // takes mat_compact_wgs[m] workgroups
     int mat_begin[3] = {0, 0, 0}, mat_count[3] = {0, 0, 0}, mat_compact_wgs[3] = {0, 0, 0};
+    int permute_wgs = 0;                // workgroups of the batched re-sort move
     // warm-start sets present among each material's scenes, with their
     // 64-particle chunks there
     std::vector<std::pair<int, int64_t>> mat_sets[3];
@@ -214,9 +215,11 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int ti
     return MPM_OK;
 }
 
-// Re-sort every scene from parity p into parity 1 - p: one branch per scene
-// on its own stream, joined back into the batch stream.  Host views of the
-// scenes are left pointing at parity 1 - p.
+// Re-sort every scene from parity p into parity 1 - p: keys, order and
+// tiles on one branch per scene (its own stream), joined back into the batch
+// stream, then one launch moves every scene's particles into the parity
+// 1 - p buffers in its new order (the warm-start lists, which name old
+// slots, are dropped).  The caller points the host views at parity 1 - p.
 int enqueue_sort(mpm_batch *b, int p)
 {
     MPM_HIP(hipEventRecord(b->ev_fork, b->stream));
@@ -224,12 +227,15 @@ int enqueue_sort(mpm_batch *b, int p)
         mpm_sim *s = b->sims[k];
         set_parity(b, k, p);
         MPM_HIP(hipStreamWaitEvent(s->stream, b->ev_fork, 0));
-        int rc = sort_all(s);
+        int rc = sort_all(s, true);
+        if (!rc && s->warm_cnt) rc = check(launch_zero_u32(s->stream, s->warm_cnt, 2 * MAX_GROUPS), "zero");
+        s->g2p_move_pending = false;        // the move below applies the order
         if (rc) return rc;
         MPM_HIP(hipEventRecord(b->ev_join[k], s->stream));
         MPM_HIP(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
     }
-    return MPM_OK;
+    return check(launch_permute_batch(b->stream, b->tab[p], b->tab[p ^ 1], (int)b->sims.size(), b->permute_wgs),
+                 "permute_batch");
 }
 
 void drop_graphs(mpm_batch *b)
@@ -322,6 +328,15 @@ int build(mpm_batch *b)
     for (mpm_sim *s : b->sims) {
         int rc = warm_on(s) ? ensure_warm_lists(s) : MPM_OK;
         if (rc) return rc;
+        // the re-sort order the batched move reads (allocated here, never
+        // inside a captured sort)
+        if (s->g2p_perm_cap < s->cap) {
+            if (s->g2p_perm) (void)hipFree(s->g2p_perm);
+            s->g2p_perm = nullptr;
+            s->g2p_perm_cap = 0;
+            MPM_HIP(hipMalloc(&s->g2p_perm, (size_t)s->cap * sizeof(uint32_t)));
+            s->g2p_perm_cap = s->cap;
+        }
     }
     for (int p = 0; p < 2; ++p) {
         std::vector<BatchSlice> h(n);
@@ -547,6 +562,9 @@ int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *sha
         b->pid0.push_back(s->pid); b->pid1.push_back(s->pid_alt);
         b->compact_wgs += (int)((s->nblocks + 255) / 256);
     }
+    uint32_t pblocks = 0;
+    for (mpm_sim *s : b->sims) pblocks += (uint32_t)((s->groups[0].n + 255) / 256);
+    b->permute_wgs = (int)round_xcd(pblocks);
     for (int k = 0; k < n; ++k) {
         const int m = b->types[k];
         if (b->mat_count[m] == 0) b->mat_begin[m] = k;

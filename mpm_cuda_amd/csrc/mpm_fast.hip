@@ -1827,6 +1827,8 @@ void fill_batch_slice(mpm_sim *s, BatchSlice &b)
     b.desc_cap = (uint32_t)tile_bound(s, s->groups[0].n);
     b.staging_cap = (uint32_t)s->staging_cap;
     b.nblocks = s->nblocks;
+    b.perm = s->g2p_perm;
+    b.copy_q = s->warm_cnt ? 1 : 0;
 }
 
 // Material `mat` of the scenes [0, nslices) of tab, in chunks of P2G_CHUNK,

@@ -251,7 +251,8 @@ struct BatchSlice {
     float dt;
     int sweeps;
     int warm_group;                 // warm-start weight set of the scene, -1: cold start
-    int reserved;
+    int copy_q;                     // a re-sort moves the warm-start quaternions too
+    const uint32_t *perm;           // the scene's re-sort order: slot j takes particle perm[j]
     uint32_t *warm_cnt;             // warm start: [0] listed particles the coming inference takes,
                                     // [1] entries the G2P appends (mat.wlist); null when cold
     float4 *grid_mp, *grid_v, *staging;

@@ -64,6 +64,7 @@ int tile_shape(int tile);
 bool tile_shape_valid(int tile);
 int launch_spill_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
 int launch_compact_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
+int launch_permute_batch(hipStream_t st, const BatchSlice *src, const BatchSlice *dst, int nslices, int wgs);
 int launch_grid_batch(hipStream_t st, const BatchSlice *tab, int nslices, int wgs);
 int launch_g2p_batch(hipStream_t st, int mat, const BatchSlice *tab, int nslices, int wgs, int threads);
 int launch_mlp_infer(hipStream_t st, const float *params, const PartView &pv);

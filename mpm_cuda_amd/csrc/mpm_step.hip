@@ -492,6 +492,29 @@ __global__ void __launch_bounds__(256) k_g2p_batch(const BatchSlice *__restrict_
     g2p_particle<MAT>(b.g, b.mat, b.pv, i, b.pv, i, b.dt, b.grid_v, true, b.sweeps, b.counters);
 }
 
+// A batch's re-sort move: every scene's particles from the buffers of table
+// `src` into those of table `dst`, slot j taking particle perm[j] of the
+// scene's last sort (all planes, the quaternions when copy_q, the ids); one
+// thread per particle, each workgroup's 256 particles in one scene.
+__global__ void __launch_bounds__(256) k_permute_batch(const BatchSlice *__restrict__ src,
+                                                       const BatchSlice *__restrict__ dst, int nslices)
+{
+    SliceScan sc;
+    sc.build(nslices, [&](int s) { return ((uint32_t)src[s].pv.n + 255u) / 256u; });
+    const uint32_t blk = xcd_spread(blockIdx.x, round_xcd(sc.total));
+    if (blk >= sc.total) return;
+    const uint32_t sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(blk));
+    const PartView a = src[sid].pv, b = dst[sid].pv;
+    const int j = (int)((blk - sc.start(sid)) * 256u + threadIdx.x);
+    if (j >= a.n) return;
+    uint32_t from = gp(src[sid].perm)[j];
+    if (from >= (uint32_t)a.n) from = (uint32_t)j;        // never taken for a valid permutation
+    const bool q = src[sid].copy_q != 0;
+    for (int p = 0; p < PL_COUNT; ++p)
+        if (q || p < PL_Q || p >= PL_Q + 4) st(b, p, j, ld(a, p, (int)from));
+    gp(b.pid)[b.off + j] = gp(a.pid)[a.off + (int64_t)from];
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 static inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
@@ -589,6 +612,13 @@ int launch_compute_tau(hipStream_t st, const MatP &m, const PartView &pv, int sw
 int launch_batch_zero(hipStream_t st, const BatchSlice *tab, int nslices)
 {
     hipLaunchKernelGGL(k_batch_zero, dim3(nblk(nslices, 64)), dim3(64), 0, st, tab, nslices);
+    return (int)hipGetLastError();
+}
+
+int launch_permute_batch(hipStream_t st, const BatchSlice *src, const BatchSlice *dst, int nslices, int wgs)
+{
+    if (wgs <= 0) return 0;
+    hipLaunchKernelGGL(k_permute_batch, dim3(wgs), dim3(256), 0, st, src, dst, nslices);
     return (int)hipGetLastError();
 }
 
