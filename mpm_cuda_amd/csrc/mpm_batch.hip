@@ -74,6 +74,8 @@ struct mpm_batch {
     // takes mat_compact_wgs[m] workgroups
     int mat_begin[3] = {0, 0, 0}, mat_count[3] = {0, 0, 0}, mat_compact_wgs[3] = {0, 0, 0};
     int permute_wgs = 0;                // workgroups of the batched re-sort move
+    SortSlice *stab[2] = {};            // batched re-sort tables of both parities
+    SortBatchPlan splan;
     // warm-start sets present among each material's scenes, with their
     // 64-particle chunks there
     std::vector<std::pair<int, int64_t>> mat_sets[3];
@@ -215,27 +217,17 @@ int enqueue_step(mpm_batch *b, int p, int p2g, int grid, int g2p_threads, int ti
     return MPM_OK;
 }
 
-// Re-sort every scene from parity p into parity 1 - p: keys, order and
-// tiles on one branch per scene (its own stream), joined back into the batch
-// stream, then one launch moves every scene's particles into the parity
-// 1 - p buffers in its new order (the warm-start lists, which name old
-// slots, are dropped).  The caller points the host views at parity 1 - p.
+// Re-sort every scene from parity p into parity 1 - p on the batch stream:
+// keys, order and tiles of all scenes with one launch per sort stage, then
+// one launch moves every scene's particles into the parity 1 - p buffers in
+// its new order (the warm-start lists, which name old slots, are dropped).
+// The caller points the host views at parity 1 - p.
 int enqueue_sort(mpm_batch *b, int p)
 {
-    MPM_HIP(hipEventRecord(b->ev_fork, b->stream));
-    for (size_t k = 0; k < b->sims.size(); ++k) {
-        mpm_sim *s = b->sims[k];
-        set_parity(b, k, p);
-        MPM_HIP(hipStreamWaitEvent(s->stream, b->ev_fork, 0));
-        int rc = sort_all(s, true);
-        if (!rc && s->warm_cnt) rc = check(launch_zero_u32(s->stream, s->warm_cnt, 2 * MAX_GROUPS), "zero");
-        s->g2p_move_pending = false;        // the move below applies the order
-        if (rc) return rc;
-        MPM_HIP(hipEventRecord(b->ev_join[k], s->stream));
-        MPM_HIP(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
-    }
-    return check(launch_permute_batch(b->stream, b->tab[p], b->tab[p ^ 1], (int)b->sims.size(), b->permute_wgs),
-                 "permute_batch");
+    const int n = (int)b->sims.size();
+    int rc = launch_sort_batch(b->stream, b->stab[p], n, b->splan);
+    if (rc) return rc;
+    return check(launch_permute_batch(b->stream, b->tab[p], b->tab[p ^ 1], n, b->permute_wgs), "permute_batch");
 }
 
 void drop_graphs(mpm_batch *b)
@@ -358,6 +350,17 @@ int build(mpm_batch *b)
             s->P = P; s->P_alt = Pa; s->pid = id; s->pid_alt = ida;
         }
         MPM_HIP(hipMemcpy(b->tab[p], h.data(), n * sizeof(BatchSlice), hipMemcpyHostToDevice));
+        std::vector<SortSlice> hs(n);
+        for (size_t k = 0; k < n; ++k) {
+            mpm_sim *s = b->sims[k];
+            float *P = s->P, *Pa = s->P_alt;
+            uint32_t *id = s->pid, *ida = s->pid_alt;
+            set_parity(b, k, p);
+            int rc = fill_sort_slice(s, hs[k]);
+            s->P = P; s->P_alt = Pa; s->pid = id; s->pid_alt = ida;
+            if (rc) return rc;
+        }
+        MPM_HIP(hipMemcpy(b->stab[p], hs.data(), n * sizeof(SortSlice), hipMemcpyHostToDevice));
     }
     b->sig.clear();
     for (size_t k = 0; k < n; ++k) b->sig.push_back(sig_of(b, k));
@@ -575,6 +578,9 @@ int mpm_batch_create(mpm_sim *const *sims, int32_t n, const mpm_batch_shape *sha
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(hip_fail(e, "stream"));
     if ((e = hipMalloc(&b->tab[0], n * sizeof(BatchSlice))) != hipSuccess) return fail(hip_fail(e, "hipMalloc"));
     if ((e = hipMalloc(&b->tab[1], n * sizeof(BatchSlice))) != hipSuccess) return fail(hip_fail(e, "hipMalloc"));
+    for (int p = 0; p < 2; ++p)
+        if ((e = hipMalloc(&b->stab[p], n * sizeof(SortSlice))) != hipSuccess) return fail(hip_fail(e, "hipMalloc"));
+    sort_batch_plan(b->sims.data(), n, b->splan);
     if ((e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming)) != hipSuccess) return fail(hip_fail(e, "event"));
     if ((e = hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming)) != hipSuccess) return fail(hip_fail(e, "event"));
     b->ev_join.assign(n, nullptr);
@@ -714,8 +720,10 @@ void mpm_batch_destroy(mpm_batch *b)
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < 2; ++p) {
         if (b->tab[p]) (void)hipFree(b->tab[p]);
+        if (b->stab[p]) (void)hipFree(b->stab[p]);
+    }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_done) (void)hipEventDestroy(b->ev_done);
     for (hipEvent_t e : b->ev_join)

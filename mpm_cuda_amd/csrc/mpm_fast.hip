@@ -64,37 +64,35 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total,
 
 constexpr int SCAN_T = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_T * SCAN_ITEMS;
 
-__global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const uint32_t *in, int n, uint32_t *partial)
+// Workgroup `wg` of a two-launch exclusive scan (SCAN_TILE values each):
+// its sum into partial[wg]; then its values scanned from the sum of the
+// partials before it (one wave adds them up).  lds: 40 words.
+__device__ __forceinline__ void scan_reduce_wg(const uint32_t *in, int n, uint32_t *partial, int wg, uint32_t *lds)
 {
-    __shared__ uint32_t lds[40];
-    int base = blockIdx.x * SCAN_TILE;
+    const int base = wg * SCAN_TILE;
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int i = base + k * SCAN_T + threadIdx.x;
+        const int i = base + k * SCAN_T + threadIdx.x;
         if (i < n) s += in[i];
     }
     uint32_t tot;
     block_excl_scan(s, tot, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+    if (threadIdx.x == 0) partial[wg] = tot;
 }
 
-// Each workgroup starts from the sum of the partials before it (one wave
-// adds them up), so the scan takes two launches.
-__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint32_t *out, int n,
-                                                       const uint32_t *partial, uint32_t *total_out)
+__device__ __forceinline__ void scan_apply_wg(const uint32_t *in, uint32_t *out, int n, const uint32_t *partial,
+                                              uint32_t *total_out, int wg, uint32_t *lds, uint32_t *wg_base)
 {
-    __shared__ uint32_t lds[40];
-    __shared__ uint32_t wg_base;
     if (threadIdx.x < 64) {
         uint32_t b = 0;
-        for (int j = (int)threadIdx.x; j < (int)blockIdx.x; j += 64) b += partial[j];
+        for (int j = (int)threadIdx.x; j < wg; j += 64) b += partial[j];
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) b += __shfl_xor(b, d);
-        if (threadIdx.x == 0) wg_base = b;
+        if (threadIdx.x == 0) *wg_base = b;
     }
     __syncthreads();
-    int i0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const int i0 = wg * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
 #pragma unroll
@@ -103,7 +101,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint3
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = wg_base + block_excl_scan(s, tot, lds);
+    uint32_t run = *wg_base + block_excl_scan(s, tot, lds);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         if (i0 + k < n) {
@@ -112,6 +110,20 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint3
         }
         run += v[k];
     }
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const uint32_t *in, int n, uint32_t *partial)
+{
+    __shared__ uint32_t lds[40];
+    scan_reduce_wg(in, n, partial, (int)blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t *in, uint32_t *out, int n,
+                                                       const uint32_t *partial, uint32_t *total_out)
+{
+    __shared__ uint32_t lds[40];
+    __shared__ uint32_t wg_base;
+    scan_apply_wg(in, out, n, partial, total_out, (int)blockIdx.x, lds, &wg_base);
 }
 
 // out = exclusive scan of in (in-place allowed); partial holds ceil(n / 2048) words.
@@ -158,35 +170,44 @@ __device__ __forceinline__ void hist_add(uint32_t *h, bool active, uint32_t d)
     else if (active && d != dl) atomicAdd(&h[d], 1u);
 }
 
+// Per-tile bodies of the radix kernels (`tile` is the workgroup's tile of
+// RS_TILE keys; LDS comes from the kernel), shared by the single sort and
+// the batched one.
 template <int DB>
-__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t dmask,
-                                                  uint32_t *hist, int ntiles)
+__device__ __forceinline__ void rs_hist_tile(const uint32_t *keys, int n, int shift, uint32_t dmask, uint32_t *hist,
+                                             int ntiles, int tile, uint32_t *h)
 {
     constexpr int NB = 1 << DB;
     const uint32_t MASK = dmask & (NB - 1);       // the last digit may be narrower
-    __shared__ uint32_t h[NB];
     for (int d = threadIdx.x; d < NB; d += RS_T) h[d] = 0;
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t dg[RS_ITERS];
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const int i = tile * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         dg[it] = i < n ? (keys[i] >> shift) & MASK : 0u;
     }
 #pragma unroll
-    for (int it = 0; it < RS_ITERS; ++it)
-        hist_add(h, blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane < n, dg[it]);
+    for (int it = 0; it < RS_ITERS; ++it) hist_add(h, tile * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane < n, dg[it]);
     __syncthreads();
-    for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + tile] = h[d];
 }
 
-// Workgroup d (one wave) replaces digit d's column hist[d * ntiles + t] by
-// its exclusive prefix over the tiles t and writes the digit's total.
-__global__ void __launch_bounds__(64) k_rs_spine(uint32_t *hist, int ntiles, uint32_t *totals)
+template <int DB>
+__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t dmask,
+                                                  uint32_t *hist, int ntiles)
 {
-    const int lane = threadIdx.x;
-    uint32_t *col = hist + (int64_t)blockIdx.x * ntiles;
+    __shared__ uint32_t h[1 << DB];
+    rs_hist_tile<DB>(keys, n, shift, dmask, hist, ntiles, (int)blockIdx.x, h);
+}
+
+// One wave replaces digit d's column hist[d * ntiles + t] by its exclusive
+// prefix over the tiles t and writes the digit's total.
+__device__ __forceinline__ void rs_spine_column(uint32_t *hist, int ntiles, uint32_t *totals, int d)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t *col = hist + (int64_t)d * ntiles;
     uint32_t carry = 0;
     for (int t0 = 0; t0 < ntiles; t0 += 64) {
         const int t = t0 + lane;
@@ -195,20 +216,26 @@ __global__ void __launch_bounds__(64) k_rs_spine(uint32_t *hist, int ntiles, uin
         if (t < ntiles) col[t] = carry + inc - v;
         carry += __shfl(inc, 63);
     }
-    if (lane == 0) totals[blockIdx.x] = carry;
+    if (lane == 0) totals[d] = carry;
 }
 
-// Scatter of one pass; vin null means the values are the input positions.
+// Workgroup d (one wave): digit d's column.
+__global__ void __launch_bounds__(64) k_rs_spine(uint32_t *hist, int ntiles, uint32_t *totals)
+{
+    rs_spine_column(hist, ntiles, totals, (int)blockIdx.x);
+}
+
+// Scatter of one tile; vin null means the values are the input positions.
+// cnt: RS_WAVES x NB words of LDS, lds: 40.
 template <int DB>
-__global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                                                     uint32_t *vout, int n, int shift, uint32_t dmask,
-                                                     const uint32_t *hist, const uint32_t *totals, int ntiles)
+__device__ __forceinline__ void rs_scatter_tile(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout,
+                                                int n, int shift, uint32_t dmask, const uint32_t *hist,
+                                                const uint32_t *totals, int ntiles, int tile, uint32_t (*cnt)[1 << DB],
+                                                uint32_t *lds)
 {
     constexpr int NB = 1 << DB;
     static_assert(NB <= RS_T, "one digit per thread");
     const uint32_t MASK = dmask & (NB - 1);
-    __shared__ uint32_t cnt[RS_WAVES][NB];
-    __shared__ uint32_t lds[40];
     const int t = threadIdx.x;
     for (int j = t; j < RS_WAVES * NB; j += RS_T) (&cnt[0][0])[j] = 0;
     // this tile's first output position of digit t: the digit's base (the
@@ -216,20 +243,20 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
     const uint32_t tot_t = t < NB ? totals[t] : 0u;
     uint32_t all;
     uint32_t first = block_excl_scan(tot_t, all, lds);
-    if (t < NB) first += hist[(int64_t)t * ntiles + blockIdx.x];
+    if (t < NB) first += hist[(int64_t)t * ntiles + tile];
     __syncthreads();
     const int w = t >> 6, lane = t & 63;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     uint32_t key[RS_ITERS], val[RS_ITERS], rank[RS_ITERS];
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const int i = tile * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         key[it] = i < n ? kin[i] : 0u;
         val[it] = i < n ? (vin ? vin[i] : (uint32_t)i) : 0u;
     }
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const int i = tile * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         const bool valid = i < n;
         const uint32_t d = (key[it] >> shift) & MASK;
         uint64_t m = __ballot(valid);
@@ -258,7 +285,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const int i = tile * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         if (i < n) {
             const uint32_t dst = cnt[w][(key[it] >> shift) & MASK] + rank[it];
             if (dst < (uint32_t)n) {      // always true for a consistent histogram
@@ -267,6 +294,16 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const 
             }
         }
     }
+}
+
+template <int DB>
+__global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                     uint32_t *vout, int n, int shift, uint32_t dmask,
+                                                     const uint32_t *hist, const uint32_t *totals, int ntiles)
+{
+    __shared__ uint32_t cnt[RS_WAVES][1 << DB];
+    __shared__ uint32_t lds[40];
+    rs_scatter_tile<DB>(kin, vin, kout, vout, n, shift, dmask, hist, totals, ntiles, (int)blockIdx.x, cnt, lds);
 }
 
 struct RadixResult {
@@ -345,20 +382,20 @@ __device__ __forceinline__ void base_of(const GridP &g, const PartView &pv, int 
 }
 
 // Block keys of the particles (linear index of the 4^3-node block holding
-// the base node) and the radix sort's first digit histogram (DB-bit digits,
-// dmask the first digit's bits) of every RS_TILE-particle tile, laid out as
-// k_rs_hist writes it.  The workgroups also clear the nb-entry block arrays
-// z0, z1, z2 that the run and tile kernels of this sort fill.
+// the base node) of tile `tile` (RS_TILE particles) and the radix sort's
+// first digit histogram (DB-bit digits, dmask the first digit's bits) of
+// the tile, laid out as k_rs_hist writes it; h: 2^DB words of LDS.  Before
+// that, the workgroup clears its share (from `first`, every `stride`) of the
+// nb-entry block arrays z0, z1, z2 that the run and tile kernels fill.
 template <int DB>
-__global__ void __launch_bounds__(RS_T) k_block_keys_hist(GridP g, PartView pv, uint32_t *keys, uint32_t dmask,
-                                                          uint32_t *hist, int ntiles, uint32_t *z0, uint32_t *z1,
-                                                          uint32_t *z2, int nb)
+__device__ __forceinline__ void keys_hist_tile(const GridP &g, const PartView &pv, uint32_t *keys, uint32_t dmask,
+                                               uint32_t *hist, int ntiles, int tile, uint32_t *z0, uint32_t *z1,
+                                               uint32_t *z2, int nb, int first, int stride, uint32_t *h)
 {
     constexpr int NB = 1 << DB;
     const uint32_t MASK = dmask & (NB - 1);
-    __shared__ uint32_t h[NB];
     for (int d = threadIdx.x; d < NB; d += RS_T) h[d] = 0;
-    for (int j = blockIdx.x * RS_T + threadIdx.x; j < nb; j += gridDim.x * RS_T) {
+    for (int j = first + (int)threadIdx.x; j < nb; j += stride) {
         z0[j] = 0u;
         z1[j] = 0u;
         z2[j] = 0u;
@@ -367,7 +404,7 @@ __global__ void __launch_bounds__(RS_T) k_block_keys_hist(GridP g, PartView pv, 
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int it = 0; it < RS_ITERS; ++it) {
-        const int i = blockIdx.x * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
+        const int i = tile * RS_TILE + w * RS_WAVE_KEYS + it * 64 + lane;
         const bool valid = i < pv.n;
         uint32_t key = 0;
         if (valid) {
@@ -380,7 +417,17 @@ __global__ void __launch_bounds__(RS_T) k_block_keys_hist(GridP g, PartView pv, 
         hist_add(h, valid, key & MASK);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d < NB; d += RS_T) hist[(int64_t)d * ntiles + tile] = h[d];
+}
+
+template <int DB>
+__global__ void __launch_bounds__(RS_T) k_block_keys_hist(GridP g, PartView pv, uint32_t *keys, uint32_t dmask,
+                                                          uint32_t *hist, int ntiles, uint32_t *z0, uint32_t *z1,
+                                                          uint32_t *z2, int nb)
+{
+    __shared__ uint32_t h[1 << DB];
+    keys_hist_tile<DB>(g, pv, keys, dmask, hist, ntiles, (int)blockIdx.x, z0, z1, z2, nb, (int)blockIdx.x * RS_T,
+                       (int)gridDim.x * RS_T, h);
 }
 
 // Gathers every permuted SoA plane and the particle ids.
@@ -401,15 +448,14 @@ __global__ void k_permute(const float *__restrict__ P, float *__restrict__ Pa, i
     pida[off + j] = pid[off + src];
 }
 
-// First and one-past-last sorted position of every block, and has[b] = 1
-// for every non-empty block (all three cleared before, untouched blocks
-// keep the zeros).
-__global__ void k_block_runs(const uint32_t *__restrict__ skeys, int n, uint32_t *blk_start, uint32_t *blk_end,
-                             uint32_t *has)
+// Sorted position i: first and one-past-last sorted position of its block,
+// and has[b] = 1 for every non-empty block (all three cleared before,
+// untouched blocks keep the zeros).
+__device__ __forceinline__ void block_run_at(const uint32_t *__restrict__ skeys, int n, int i, uint32_t *blk_start,
+                                             uint32_t *blk_end, uint32_t *has)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t b = skeys[i] >> KEY_LOW;
+    const uint32_t b = skeys[i] >> KEY_LOW;
     if (i == 0 || (skeys[i - 1] >> KEY_LOW) != b) {
         blk_start[b] = (uint32_t)i;
         has[b] = 1u;
@@ -417,22 +463,153 @@ __global__ void k_block_runs(const uint32_t *__restrict__ skeys, int n, uint32_t
     if (i == n - 1 || (skeys[i + 1] >> KEY_LOW) != b) blk_end[b] = (uint32_t)(i + 1);
 }
 
-// One tile per non-empty block: (first particle, count, block, tile id).
-// blk_tile maps the block to its global tile id (staging slot), every arena
-// face marked written until the block's first P2G.
-__global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end, const uint32_t *tile_index,
-                             int nblocks, uint4 *desc, uint32_t desc_cap, int32_t *blk_tile, uint32_t tile_base)
+__global__ void k_block_runs(const uint32_t *__restrict__ skeys, int n, uint32_t *blk_start, uint32_t *blk_end,
+                             uint32_t *has)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    block_run_at(skeys, n, (int)(blockIdx.x * blockDim.x + threadIdx.x), blk_start, blk_end, has);
+}
+
+// Block b's tile: (first particle, count, block, tile id); blk_tile maps the
+// block to its global tile id (staging slot), every arena face marked
+// written until the block's first P2G, or -1 for an empty block.
+__device__ __forceinline__ void emit_tile_at(const uint32_t *blk_start, const uint32_t *blk_end,
+                                             const uint32_t *tile_index, int nblocks, int b, uint4 *desc,
+                                             uint32_t desc_cap, int32_t *blk_tile, uint32_t tile_base)
+{
     if (b >= nblocks) return;
-    uint32_t c = blk_end[b] - blk_start[b];
-    uint32_t t = tile_index[b];
+    const uint32_t c = blk_end[b] - blk_start[b];
+    const uint32_t t = tile_index[b];
     if (c > 0 && t < desc_cap) {
         desc[t] = make_uint4(blk_start[b], c, (uint32_t)b, tile_base + t);
         blk_tile[b] = tile_entry(tile_base + t, TILE_ALL_FACES);
     } else {
         blk_tile[b] = -1;
     }
+}
+
+__global__ void k_emit_tiles(const uint32_t *blk_start, const uint32_t *blk_end, const uint32_t *tile_index,
+                             int nblocks, uint4 *desc, uint32_t desc_cap, int32_t *blk_tile, uint32_t tile_base)
+{
+    emit_tile_at(blk_start, blk_end, tile_index, nblocks, (int)(blockIdx.x * blockDim.x + threadIdx.x), desc,
+                 desc_cap, blk_tile, tile_base);
+}
+
+// ---------------------------------------------------------------------------
+// Batched re-sort: every scene of a SortSlice table through each stage of the
+// sort above in one launch per stage (8-bit digits, the same number of
+// passes for every scene).  A workgroup maps its index to a scene and to the
+// scene's own tile, digit columns or block range with a SliceScan of the
+// per-scene counts (the map is wave-uniform, so the bodies run as in the
+// single sort).
+constexpr int BS_DB = 8;
+
+template <class Count>
+__device__ __forceinline__ bool bsort_locate(const SortSlice *__restrict__ tab, int ns, Count count, uint32_t &sid,
+                                             int &local)
+{
+    SliceScan sc;
+    sc.build(ns, count);
+    if (blockIdx.x >= sc.total) return false;
+    sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)sc.find(blockIdx.x));
+    local = (int)(blockIdx.x - sc.start(sid));
+    return true;
+}
+
+__global__ void __launch_bounds__(RS_T) k_bsort_keys(const SortSlice *__restrict__ tab, int ns)
+{
+    __shared__ uint32_t h[1 << BS_DB];
+    uint32_t sid;
+    int tile;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)tab[s].ntiles; }, sid, tile)) return;
+    const SortSlice &t = tab[sid];
+    keys_hist_tile<BS_DB>(t.g, t.pv, t.keys, 0xFFu, t.hist, t.ntiles, tile, t.blk_start, t.blk_end, t.has, t.nblocks,
+                          tile * RS_T, t.ntiles * RS_T, h);
+}
+
+__global__ void __launch_bounds__(RS_T) k_bsort_hist(const SortSlice *__restrict__ tab, int ns, int pass)
+{
+    __shared__ uint32_t h[1 << BS_DB];
+    uint32_t sid;
+    int tile;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)tab[s].ntiles; }, sid, tile)) return;
+    const SortSlice &t = tab[sid];
+    rs_hist_tile<BS_DB>((pass & 1) ? t.keys_alt : t.keys, t.n, BS_DB * pass, 0xFFu, t.hist, t.ntiles, tile, h);
+}
+
+// Four digit columns per workgroup, one per wave.
+__global__ void __launch_bounds__(256) k_bsort_spine(const SortSlice *__restrict__ tab, int ns)
+{
+    uint32_t sid;
+    int q;
+    if (!bsort_locate(tab, ns, [&](int) { return (uint32_t)((1 << BS_DB) / 4); }, sid, q)) return;
+    const SortSlice &t = tab[sid];
+    rs_spine_column(t.hist, t.ntiles, t.scan, q * 4 + (int)(threadIdx.x >> 6));
+}
+
+// Pass `pass` reads the keys (values: the positions in pass 0) from the
+// buffers of its parity and writes the other ones; the last pass writes the
+// values into the scene's permutation.
+__global__ void __launch_bounds__(RS_T) k_bsort_scatter(const SortSlice *__restrict__ tab, int ns, int pass, int last)
+{
+    __shared__ uint32_t cnt[RS_WAVES][1 << BS_DB];
+    __shared__ uint32_t lds[40];
+    uint32_t sid;
+    int tile;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)tab[s].ntiles; }, sid, tile)) return;
+    const SortSlice &t = tab[sid];
+    const bool odd = (pass & 1) != 0;
+    const uint32_t *vin = pass == 0 ? nullptr : (odd ? t.vals_alt : t.vals);
+    uint32_t *vout = last ? t.perm : (odd ? t.vals : t.vals_alt);
+    rs_scatter_tile<BS_DB>(odd ? t.keys_alt : t.keys, vin, odd ? t.keys : t.keys_alt, vout, t.n, BS_DB * pass, 0xFFu,
+                           t.hist, t.scan, t.ntiles, tile, cnt, lds);
+}
+
+// keys_alt: the sorted keys ended in the *_alt buffers (odd pass count).
+__global__ void __launch_bounds__(256) k_bsort_runs(const SortSlice *__restrict__ tab, int ns, int keys_alt)
+{
+    uint32_t sid;
+    int q;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)((tab[s].n + 255) / 256); }, sid, q)) return;
+    const SortSlice &t = tab[sid];
+    block_run_at(keys_alt ? t.keys_alt : t.keys, t.n, q * 256 + (int)threadIdx.x, t.blk_start, t.blk_end, t.has);
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_bsort_scan_reduce(const SortSlice *__restrict__ tab, int ns)
+{
+    __shared__ uint32_t lds[40];
+    uint32_t sid;
+    int wg;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)((tab[s].nblocks + SCAN_TILE - 1) / SCAN_TILE); }, sid,
+                      wg))
+        return;
+    const SortSlice &t = tab[sid];
+    scan_reduce_wg(t.has, t.nblocks, t.scan, wg, lds);
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_bsort_scan_apply(const SortSlice *__restrict__ tab, int ns)
+{
+    __shared__ uint32_t lds[40];
+    __shared__ uint32_t wg_base;
+    uint32_t sid;
+    int wg;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)((tab[s].nblocks + SCAN_TILE - 1) / SCAN_TILE); }, sid,
+                      wg))
+        return;
+    const SortSlice &t = tab[sid];
+    scan_apply_wg(t.has, t.has, t.nblocks, t.scan, t.tile_count, wg, lds, &wg_base);
+}
+
+// Tiles, and the warm-start list counts cleared (their lists name old slots).
+__global__ void __launch_bounds__(256) k_bsort_emit(const SortSlice *__restrict__ tab, int ns)
+{
+    uint32_t sid;
+    int q;
+    if (!bsort_locate(tab, ns, [&](int s) { return (uint32_t)((tab[s].nblocks + 255) / 256); }, sid, q)) return;
+    const SortSlice &t = tab[sid];
+    if (q == 0 && t.warm_cnt)
+        for (int j = (int)threadIdx.x; j < 2 * MAX_GROUPS; j += 256) t.warm_cnt[j] = 0u;
+    emit_tile_at(t.blk_start, t.blk_end, t.has, t.nblocks, q * 256 + (int)threadIdx.x, t.desc, t.desc_cap, t.blk_tile,
+                 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1800,6 +1977,73 @@ int sort_keys_noalloc(const uint32_t *keys, int64_t n, int bits, uint32_t *sorte
             return hip_fail(e, "copy permutation");
     }
     return MPM_OK;
+}
+
+// Batched re-sort of a one-group simulator in its current parity.
+int fill_sort_slice(mpm_sim *s, SortSlice &t)
+{
+    if (s->groups.size() != 1) return set_error(MPM_ERR_INVALID, "a batched re-sort takes one particle group");
+    int rc = ensure_fast_buffers(s);
+    if (rc) return rc;
+    if (!s->g2p_perm || s->g2p_perm_cap < s->cap) return set_error(MPM_ERR_INVALID, "no re-sort permutation buffer");
+    const PartView pv = view_of(s, 0);
+    memset(&t, 0, sizeof t);
+    t.g = s->g;
+    t.pv = pv;
+    t.keys = s->keys;
+    t.vals = s->vals;
+    t.keys_alt = s->keys_alt;
+    t.vals_alt = s->vals_alt;
+    t.hist = s->sort_hist;
+    t.scan = s->sort_scan;
+    t.blk_start = s->blk_start;
+    t.blk_end = s->blk_count;
+    t.has = s->scan_tmp;
+    t.perm = s->g2p_perm;
+    t.tile_count = s->tile_counts;
+    t.warm_cnt = s->warm_cnt;
+    t.desc = (uint4 *)s->tile_desc;
+    t.blk_tile = s->blk_tile;
+    t.n = pv.n;
+    t.ntiles = nblk(pv.n, RS_TILE);
+    t.nblocks = (int)s->nblocks;
+    t.desc_cap = (uint32_t)tile_bound(s, pv.n);
+    return MPM_OK;
+}
+
+void sort_batch_plan(mpm_sim *const *sims, int ns, SortBatchPlan &pl)
+{
+    pl = SortBatchPlan{};
+    for (int k = 0; k < ns; ++k) {
+        const mpm_sim *s = sims[k];
+        int bits = KEY_LOW + 1;
+        while ((1ll << (bits - KEY_LOW)) < s->nblocks) ++bits;
+        pl.passes = std::max(pl.passes, (bits + BS_DB - 1) / BS_DB);
+        const int64_t n = s->groups[0].n;
+        pl.tiles += nblk(n, RS_TILE);
+        pl.runs += nblk(n, 256);
+        pl.scan += nblk(s->nblocks, SCAN_TILE);
+        pl.emit += nblk(s->nblocks, 256);
+    }
+    pl.spine = ns * ((1 << BS_DB) / 4);
+}
+
+int launch_sort_batch(hipStream_t st, const SortSlice *tab, int ns, const SortBatchPlan &pl)
+{
+    if (ns <= 0 || pl.tiles <= 0) return MPM_OK;
+    hipLaunchKernelGGL(k_bsort_keys, dim3(pl.tiles), dim3(RS_T), 0, st, tab, ns);
+    for (int pass = 0; pass < pl.passes; ++pass) {
+        if (pass) hipLaunchKernelGGL(k_bsort_hist, dim3(pl.tiles), dim3(RS_T), 0, st, tab, ns, pass);
+        hipLaunchKernelGGL(k_bsort_spine, dim3(pl.spine), dim3(256), 0, st, tab, ns);
+        hipLaunchKernelGGL(k_bsort_scatter, dim3(pl.tiles), dim3(RS_T), 0, st, tab, ns, pass,
+                           pass == pl.passes - 1 ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_bsort_runs, dim3(pl.runs), dim3(256), 0, st, tab, ns, pl.passes & 1);
+    hipLaunchKernelGGL(k_bsort_scan_reduce, dim3(pl.scan), dim3(SCAN_T), 0, st, tab, ns);
+    hipLaunchKernelGGL(k_bsort_scan_apply, dim3(pl.scan), dim3(SCAN_T), 0, st, tab, ns);
+    hipLaunchKernelGGL(k_bsort_emit, dim3(pl.emit), dim3(256), 0, st, tab, ns);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MPM_OK : hip_fail(e, "sort_batch");
 }
 
 // Batch descriptor of a one-group simulator in its current particle order.

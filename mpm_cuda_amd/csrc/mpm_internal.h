@@ -264,6 +264,27 @@ struct BatchSlice {
     int64_t nblocks;
 };
 
+// One scene of a batched re-sort (device table): its particles in the
+// parity being sorted, its sort buffers, tile outputs and the permutation
+// the batched move reads.  Keys are sorted with 8-bit digits in the batch's
+// common number of passes (extra passes of a scene with fewer key bits see
+// zero digits and keep the order).
+struct SortSlice {
+    GridP g;
+    PartView pv;
+    uint32_t *keys, *vals, *keys_alt, *vals_alt;
+    uint32_t *hist;                 // 256 x ntiles digit counts
+    uint32_t *scan;                 // radix digit totals, then the tile scan's partial sums
+    uint32_t *blk_start, *blk_end, *has;
+    uint32_t *perm;                 // slot j of the new order takes particle perm[j]
+    uint32_t *tile_count;
+    uint32_t *warm_cnt;             // warm-start list counts (cleared: the lists name old slots), or null
+    uint4 *desc;
+    int32_t *blk_tile;
+    int n, ntiles, nblocks;
+    uint32_t desc_cap;
+};
+
 // Inclusive prefix sums of per-scene work counts, held in one wave's
 // registers: lane l keeps scene l + 64 j in slot j (up to BATCH_MAX scenes).
 // Each wave that needs the map builds its own copy; find() turns a global

@@ -54,6 +54,14 @@ PartView view_of(mpm_sim *s, int g);
 // Batched stages over a BatchSlice table in device memory (mpm_step.hip,
 // mpm_fast.hip); host side in mpm_batch.hip.
 void fill_batch_slice(mpm_sim *s, BatchSlice &b);
+// Batched re-sort (mpm_fast.hip): launch sizes of a set of scenes, one
+// scene's table entry in its current parity, and the launches.
+struct SortBatchPlan {
+    int passes = 0, tiles = 0, spine = 0, runs = 0, scan = 0, emit = 0;
+};
+void sort_batch_plan(mpm_sim *const *sims, int ns, SortBatchPlan &pl);
+int fill_sort_slice(mpm_sim *s, SortSlice &t);
+int launch_sort_batch(hipStream_t st, const SortSlice *tab, int ns, const SortBatchPlan &pl);
 int launch_batch_zero(hipStream_t st, const BatchSlice *tab, int nslices);
 // Warm-start particle lists of a simulator (mpm_api.hip).
 int ensure_warm_lists(mpm_sim *s);
