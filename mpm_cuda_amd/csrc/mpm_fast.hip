@@ -366,9 +366,12 @@ static int radix_sort(hipStream_t st, uint32_t *keys, uint32_t *vals, uint32_t *
 // Keys, permutation and tiles.
 //
 // Sort key: linear index of the 4^3-node block holding the particle's base
-// node.  The stable sort keeps the previous order inside a block; P2G re-bins
-// every block's particles by their current base node in LDS.
-constexpr int KEY_LOW = 0;
+// node, above KEY_LOW bits of the base node's position inside the block, so
+// that a block's particles are stored cell by cell: the P2G's counting sort
+// of a chunk by base node then finds them nearly in order, and the item
+// gathers of its phase C read neighbouring particles.  P2G still re-bins
+// every chunk by its particles' current base nodes in LDS.
+constexpr int KEY_LOW = 6;
 
 __device__ __forceinline__ void base_of(const GridP &g, const PartView &pv, int i, int b[3])
 {
@@ -410,8 +413,9 @@ __device__ __forceinline__ void keys_hist_tile(const GridP &g, const PartView &p
         if (valid) {
             int b[3];
             base_of(g, pv, i, b);
-            key = (((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
-                   (uint32_t)(b[0] >> 2)) << KEY_LOW;
+            key = ((((uint32_t)(b[2] >> 2) * (uint32_t)g.nby + (uint32_t)(b[1] >> 2)) * (uint32_t)g.nbx +
+                    (uint32_t)(b[0] >> 2)) << KEY_LOW) |
+                  (uint32_t)((b[0] & 3) | ((b[1] & 3) << 2) | ((b[2] & 3) << 4));
             keys[i] = key;
         }
         hist_add(h, valid, key & MASK);
