@@ -377,8 +377,9 @@ int mpm_sort_keys_noalloc(const uint32_t *keys, int64_t n, int32_t bits, uint32_
  * No counterpart in the reference, which steps one MPMSimulator per process
  * (main.cpp:202-221).  For callers with many small scenes: every scene of a
  * batch advances one step (each at its own cfg.dt) with one launch per stage
- * for the whole batch -- P2G tiles, spill fallback, block compaction, grid
- * update, G2P -- replayed from a hipGraph; all scenes are re-sorted together
+ * for all the scenes of a material -- P2G tiles, spill fallback, block
+ * compaction, grid update, G2P; the materials side by side -- replayed from a
+ * hipGraph; all scenes are re-sorted together, one launch per sort stage,
  * every sort_every steps.  A scene is an mpm_sim in the FAST form holding
  * exactly one particle group; it may still be read (mpm_read_particles,
  * mpm_stats, ...) and stepped on its own between batch runs, which are
