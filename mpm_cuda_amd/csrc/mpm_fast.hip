@@ -1253,10 +1253,11 @@ __device__ __forceinline__ uint32_t p2g_tile(const GridP &g, const MatP &mat, co
     for (uint32_t c0 = 0; c0 < d.y; c0 += CH) {
         const int nc = (int)min((uint32_t)CH, d.y - c0);
         // A: all position loads of the chunk first (lanes past its end reload
-        // its last particle), then bin keys
+        // its last particle; passes past its end load nothing), then bin keys
         float xs[PASSES][3];
 #pragma unroll
         for (int it = 0; it < PASSES; ++it) {
+            if (it * 64 >= nc) break;
             const uint32_t slot = (uint32_t)min(it * 64 + lane, nc - 1);
 #pragma unroll
             for (int a = 0; a < 3; ++a) xs[it][a] = ld_plane(pv, PL_X + a, d.x + c0 + slot);
